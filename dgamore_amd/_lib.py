@@ -1,0 +1,81 @@
+"""ctypes binding of libdgami.so (the C ABI declared in include/dgami.h).
+
+The library is loaded from this package directory. There is no CPU fallback:
+if the library is missing or no GPU is present, the first call raises.
+
+Error mapping follows the reference's behaviour: argument/shape errors raise
+ValueError (four_point.py:119-120, 266-272), singular matrices raise
+numpy.linalg.LinAlgError (raised by np.linalg.inv/solve in the reference),
+HIP runtime failures raise RuntimeError.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libdgami.so")
+
+PLAIN_FP64 = 0
+MIXED_FP32_IR = 1
+MIXED_BF16X3_IR = 2
+
+_c = ctypes.c_void_p
+_i = ctypes.c_int
+_d = ctypes.c_double
+_i64 = ctypes.c_int64
+
+# name -> argument types (every function returns int except where noted)
+SIGNATURES = {
+    "dgami_last_error": ([], ctypes.c_char_p),
+    "dgami_version": ([], _i),
+    "dgami_fft_grid": ([_c, _c, _i64, _i, _i, _i, _i, _d, _c], _i),
+    "dgami_bubble_irrq": ([_i, _i, _i, _i, _i, _i, _i, _d, _c, _c, _i, _c, _c], _i),
+    "dgami_chi0_prepare": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _c], _i),
+    "dgami_ladder_solve_sum": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_vrg_chi_phys": ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c, _c, _c, _c, _c], _i),
+    "dgami_kernel_r": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _i, _c, _c], _i),
+    "dgami_dc_kernel": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _c], _i),
+    "dgami_sigma_from_kernel": ([_i, _i, _i, _i, _i, _i, _i, _d, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_pairing_setup": ([_i, _i, _i, _i, _i, _i, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_pairing_matvec": ([_i, _i, _i, _i, _i, _i, _d, _c, _c, _c, _i, _c, _c, _c], _i),
+}
+
+_lib = None
+
+
+def load():
+    """Load libdgami.so (once). Raises ImportError if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(f"{LIB_PATH} is missing: build it with `python -m dgamore_amd.build`")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = res
+    _lib = lib
+    return lib
+
+
+def last_error():
+    return load().dgami_last_error().decode()
+
+
+def check(rc, what):
+    """Map a C-ABI status code to the reference's exception types."""
+    if rc == 0:
+        return
+    msg = last_error()
+    if rc == -1:
+        raise ValueError(f"{what}: {msg}")
+    if rc > 0:
+        raise np.linalg.LinAlgError(f"{what}: {rc} singular matrices")
+    raise RuntimeError(f"{what}: {msg} (status {rc})")
+
+
+def call(name, *args):
+    check(getattr(load(), name)(*args), name)

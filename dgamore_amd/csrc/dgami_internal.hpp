@@ -1,0 +1,285 @@
+// Shared device/host helpers for libdgami (gfx950, CDNA4).
+//
+// Complex numbers are interleaved (re, im) doubles, i.e. the memory layout of
+// numpy.complex128; on the device they are handled as HIP's double2.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <string>
+
+namespace dgami {
+
+// ----------------------------------------------------------------------------
+// complex double helpers
+// ----------------------------------------------------------------------------
+__host__ __device__ __forceinline__ double2 cmk(double r, double i) { return make_double2(r, i); }
+__host__ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ double2 cscale(double2 a, double s) { return make_double2(a.x * s, a.y * s); }
+__host__ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+// a - b*c
+__host__ __device__ __forceinline__ double2 cfms(double2 a, double2 b, double2 c) {
+  double re = fma(-b.x, c.x, a.x);
+  re = fma(b.y, c.y, re);
+  double im = fma(-b.x, c.y, a.y);
+  im = fma(-b.y, c.x, im);
+  return make_double2(re, im);
+}
+// a + b*c
+__host__ __device__ __forceinline__ double2 cfma(double2 a, double2 b, double2 c) {
+  double re = fma(b.x, c.x, a.x);
+  re = fma(-b.y, c.y, re);
+  double im = fma(b.x, c.y, a.y);
+  im = fma(b.y, c.x, im);
+  return make_double2(re, im);
+}
+// 1/b with Smith's scaling (robust against over/underflow of |b|^2)
+__host__ __device__ __forceinline__ double2 cinv(double2 b) {
+  if (fabs(b.x) >= fabs(b.y)) {
+    double r = b.y / b.x;
+    double d = b.x + b.y * r;
+    return make_double2(1.0 / d, -r / d);
+  }
+  double r = b.x / b.y;
+  double d = b.x * r + b.y;
+  return make_double2(r / d, -1.0 / d);
+}
+__host__ __device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
+
+// ----------------------------------------------------------------------------
+// error reporting (thread-local last error; host side)
+// ----------------------------------------------------------------------------
+void set_error(const std::string& msg);
+const char* last_error();
+
+#define DGAMI_ARG_CHECK(cond, msg)      \
+  do {                                  \
+    if (!(cond)) {                      \
+      ::dgami::set_error(msg);          \
+      return -1;                        \
+    }                                   \
+  } while (0)
+
+#define DGAMI_HIP_CHECK(expr)                                                              \
+  do {                                                                                     \
+    hipError_t _e = (expr);                                                                \
+    if (_e != hipSuccess) {                                                                \
+      ::dgami::set_error(std::string("HIP error in " #expr ": ") + hipGetErrorString(_e)); \
+      return -2;                                                                           \
+    }                                                                                      \
+  } while (0)
+
+// Grow-only device scratch owned by the library, one slot per purpose id.
+// Returns nullptr (and sets the error) when the allocation fails.
+void* workspace(int slot, size_t bytes);
+
+inline unsigned ceil_div(size_t a, size_t b) { return static_cast<unsigned>((a + b - 1) / b); }
+
+// Allow a kernel to use more than the default 64 KiB of dynamic LDS (gfx950 has 160 KiB per CU).
+template <typename K>
+inline void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 48 * 1024)
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(kernel), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              static_cast<int>(bytes));
+}
+
+// ----------------------------------------------------------------------------
+// in-LDS mixed-radix FFT (radices 2, 3, 4, 5), Stockham autosort
+// ----------------------------------------------------------------------------
+// Factorisation of a transform length into radix stages (largest radices first).
+struct FftPlan1D {
+  int len;
+  int nstage;
+  int radix[8];
+};
+
+__host__ __device__ inline bool fft_plan(int len, FftPlan1D* p) {
+  p->len = len;
+  p->nstage = 0;
+  int m = len;
+  while (m > 1) {
+    int r = (m % 4 == 0) ? 4 : (m % 2 == 0) ? 2 : (m % 3 == 0) ? 3 : (m % 5 == 0) ? 5 : 0;
+    if (r == 0 || p->nstage == 8) return false;
+    p->radix[p->nstage++] = r;
+    m /= r;
+  }
+  return true;
+}
+
+// Small DFT kernels on registers; sign = -1 forward, +1 backward.
+template <int R>
+__device__ __forceinline__ void dft_small(double2* v, double sign);
+
+template <>
+__device__ __forceinline__ void dft_small<2>(double2* v, double) {
+  double2 a = v[0], b = v[1];
+  v[0] = cadd(a, b);
+  v[1] = csub(a, b);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<4>(double2* v, double sign) {
+  double2 a0 = cadd(v[0], v[2]), a1 = csub(v[0], v[2]);
+  double2 b0 = cadd(v[1], v[3]), b1 = csub(v[1], v[3]);
+  // multiply b1 by (sign * i)
+  double2 b1r = make_double2(-sign * b1.y, sign * b1.x);
+  v[0] = cadd(a0, b0);
+  v[2] = csub(a0, b0);
+  v[1] = cadd(a1, b1r);
+  v[3] = csub(a1, b1r);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<3>(double2* v, double sign) {
+  // w = exp(sign * 2 pi i / 3) = -1/2 + sign * i sqrt(3)/2
+  const double s3 = 0.86602540378443864676 * sign;
+  double2 t1 = cadd(v[1], v[2]);
+  double2 t2 = csub(v[1], v[2]);
+  double2 m = make_double2(v[0].x - 0.5 * t1.x, v[0].y - 0.5 * t1.y);
+  double2 r = make_double2(-s3 * t2.y, s3 * t2.x);
+  v[0] = cadd(v[0], t1);
+  v[1] = cadd(m, r);
+  v[2] = csub(m, r);
+}
+
+template <>
+__device__ __forceinline__ void dft_small<5>(double2* v, double sign) {
+  double2 in[5] = {v[0], v[1], v[2], v[3], v[4]};
+  for (int k = 0; k < 5; ++k) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int j = 0; j < 5; ++j) {
+      double s, c;
+      sincospi(sign * 2.0 * ((k * j) % 5) / 5.0, &s, &c);
+      acc = cfma(acc, in[j], make_double2(c, s));
+    }
+    v[k] = acc;
+  }
+}
+
+// One Stockham stage of radix R over `nfft` transforms of length L held in
+// LDS. Transform f starts at (f / nf2) * s1 + (f % nf2) * s2 and its elements
+// are `xstride` apart. The stage reads all inputs into registers,
+// synchronises, and writes back in place. `tw` holds exp(sign 2 pi i j / L),
+// j < L. Must be called by all threads of the block; the launch guarantees
+// nfft * L / R <= fft_max_bfly(R) * blockDim.x (see fft_block_threads).
+__host__ __device__ constexpr int fft_max_bfly(int R) { return R == 2 ? 8 : R == 3 ? 5 : R == 4 ? 4 : 3; }
+
+template <int R>
+__device__ __forceinline__ void fft_stage(double2* buf, int L, int nfft, int nf2, int s1, int s2, int xstride, int Ns,
+                                          const double2* tw, double sign) {
+  constexpr int MB = fft_max_bfly(R);
+  const int nbf = L / R;  // butterflies per transform
+  const int total = nbf * nfft;
+  double2 v[MB][R];
+#pragma unroll
+  for (int c = 0; c < MB; ++c) {
+    const int b = threadIdx.x + c * blockDim.x;
+    if (b < total) {
+      const int f = b / nbf, j = b - f * nbf;
+      const int k = j % Ns;
+      const double2* base = buf + (f / nf2) * s1 + (f % nf2) * s2;
+#pragma unroll
+      for (int r = 0; r < R; ++r) v[c][r] = base[(j + r * nbf) * xstride];
+      if (Ns > 1) {
+        const int step = L / (Ns * R);
+#pragma unroll
+        for (int r = 1; r < R; ++r) v[c][r] = cmul(v[c][r], tw[(k * r * step) % L]);
+      }
+      dft_small<R>(v[c], sign);
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int c = 0; c < MB; ++c) {
+    const int b = threadIdx.x + c * blockDim.x;
+    if (b < total) {
+      const int f = b / nbf, j = b - f * nbf;
+      const int k = j % Ns;
+      const int d = (j / Ns) * Ns * R + k;
+      double2* base = buf + (f / nf2) * s1 + (f % nf2) * s2;
+#pragma unroll
+      for (int r = 0; r < R; ++r) base[(d + r * Ns) * xstride] = v[c][r];
+    }
+  }
+  __syncthreads();
+}
+
+// Full 1D FFT of `nfft` transforms of length plan.len (layout as in fft_stage).
+__device__ inline void fft_1d_lds(double2* buf, const FftPlan1D& plan, int nfft, int nf2, int s1, int s2, int xstride,
+                                  const double2* tw, double sign) {
+  int Ns = 1;
+  for (int s = 0; s < plan.nstage; ++s) {
+    const int R = plan.radix[s];
+    if (R == 4)
+      fft_stage<4>(buf, plan.len, nfft, nf2, s1, s2, xstride, Ns, tw, sign);
+    else if (R == 2)
+      fft_stage<2>(buf, plan.len, nfft, nf2, s1, s2, xstride, Ns, tw, sign);
+    else if (R == 3)
+      fft_stage<3>(buf, plan.len, nfft, nf2, s1, s2, xstride, Ns, tw, sign);
+    else
+      fft_stage<5>(buf, plan.len, nfft, nf2, s1, s2, xstride, Ns, tw, sign);
+    Ns *= R;
+  }
+}
+
+// Twiddle table exp(sign 2 pi i j / L), j < L, filled cooperatively (no barrier).
+__device__ inline void fft_twiddles(double2* tw, int L, double sign) {
+  for (int j = threadIdx.x; j < L; j += blockDim.x) {
+    double s, c;
+    sincospi(sign * 2.0 * j / L, &s, &c);
+    tw[j] = make_double2(c, s);
+  }
+}
+
+// Momentum-grid FFT plan: lengths and radix stages of the three axes.
+struct FftPlan3D {
+  FftPlan1D p[3];
+  int n[3];
+  int ntot;
+};
+
+inline bool fft_plan_3d(int nx, int ny, int nz, FftPlan3D* p) {
+  p->n[0] = nx;
+  p->n[1] = ny;
+  p->n[2] = nz;
+  p->ntot = nx * ny * nz;
+  return fft_plan(nx, &p->p[0]) && fft_plan(ny, &p->p[1]) && fft_plan(nz, &p->p[2]);
+}
+
+// Threads per block so that every stage of a 3D transform of one grid fits the
+// per-thread butterfly budget (a multiple of 64, at least 64).
+inline int fft_block_threads(const FftPlan3D& p, int min_threads = 64) {
+  int need = min_threads;
+  for (int d = 0; d < 3; ++d)
+    for (int s = 0; s < p.p[d].nstage; ++s) {
+      const int R = p.p[d].radix[s];
+      const int bfly = p.ntot / R;
+      const int t = (bfly + fft_max_bfly(R) - 1) / fft_max_bfly(R);
+      if (t > need) need = t;
+    }
+  return ((need + 63) / 64) * 64;
+}
+
+// LDS twiddle storage: exp(sign 2 pi i j / n_d) for the three axes, concatenated.
+__device__ inline void fft_twiddles_3d(double2* tw, const FftPlan3D& p, double sign) {
+  fft_twiddles(tw, p.n[0], sign);
+  fft_twiddles(tw + p.n[0], p.n[1], sign);
+  fft_twiddles(tw + p.n[0] + p.n[1], p.n[2], sign);
+}
+
+// 3D FFT of one [nx][ny][nz] slab in LDS (z fastest), unnormalised, sign -1
+// forward (e^{-i k R}) / +1 backward. Axes of length 1 are skipped.
+__device__ inline void fft_3d_lds(double2* buf, const FftPlan3D& p, const double2* tw, double sign) {
+  const int nx = p.n[0], ny = p.n[1], nz = p.n[2];
+  if (nz > 1) fft_1d_lds(buf, p.p[2], nx * ny, 1, nz, 0, 1, tw + nx + ny, sign);
+  if (ny > 1) fft_1d_lds(buf, p.p[1], nx * nz, nz, ny * nz, 1, nz, tw + nx, sign);
+  if (nx > 1) fft_1d_lds(buf, p.p[0], ny * nz, ny * nz, 0, 1, ny * nz, tw, sign);
+}
+
+}  // namespace dgami

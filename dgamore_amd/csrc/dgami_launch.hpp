@@ -1,0 +1,26 @@
+// Host-side launchers shared between translation units of libdgami.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace dgami {
+
+// Workspace slots (see workspace() in dgami_internal.hpp); one purpose each.
+enum WorkspaceSlot {
+  WS_GK = 0,         // two-point object in [nb][nb][nv][nk] layout
+  WS_GR = 1,         // its momentum-space FFT
+  WS_SIGMA_PART = 2, // per-frequency-chunk partial self-energies
+  WS_LADDER = 3,     // ladder matrices / factors (large-n path)
+  WS_LADDER_AUX = 4, // ladder refinement vectors
+  WS_DC = 5,         // double-counting kernel operands
+  WS_PAIR = 6,       // Eliashberg matvec intermediates
+  WS_MISC = 7,
+};
+
+int launch_fft_grid(const double2* in, double2* out, int64_t batch, int nx, int ny, int nz, int sign, double scale,
+                    hipStream_t stream);
+int launch_ref2_to_gk(const double2* in, double2* out, int nk, int nb, int nv_in, int v0, int nv, hipStream_t stream);
+
+}  // namespace dgami

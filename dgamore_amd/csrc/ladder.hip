@@ -1,0 +1,51 @@
+// Entry point of the batched ladder solves: picks the register-resident
+// kernel for small compound sizes and the blocked workspace LU otherwise.
+#include "dgami.h"
+#include "dgami_internal.hpp"
+#include "dgami_launch.hpp"
+#include "ladder_common.hpp"
+
+namespace dgami {
+
+int ladder_reg_variant(int n, int nrhs);
+int ladder_reg_launch(int variant, const LadderArgs& a, hipStream_t stream);
+int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream);
+
+int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
+  const int v = ladder_reg_variant(a.n, a.T);
+  if (v >= 0) return ladder_reg_launch(v, a, stream);
+  return ladder_blocked_fp64(a, stream);
+}
+
+}  // namespace dgami
+
+extern "C" int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double beta,
+                                      const dgami_c128* chi0_inv, const dgami_c128* gamma, const dgami_c128* u_r,
+                                      const dgami_c128* v_r, int precision, dgami_c128* chi_aux_sum, int32_t* info,
+                                      void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && chi_aux_sum, "ladder: null pointer");
+  DGAMI_ARG_CHECK(nchan >= 1 && nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0,
+                  "ladder: invalid sizes");
+  DGAMI_ARG_CHECK(precision == DGAMI_PLAIN_FP64 || precision == DGAMI_MIXED_FP32_IR ||
+                      precision == DGAMI_MIXED_BF16X3_IR,
+                  "ladder: unknown precision mode");
+  LadderArgs a;
+  a.chi0_inv = reinterpret_cast<const double2*>(chi0_inv);
+  a.gamma = reinterpret_cast<const double2*>(gamma);
+  a.u_r = reinterpret_cast<const double2*>(u_r);
+  a.v_r = reinterpret_cast<const double2*>(v_r);
+  a.out = reinterpret_cast<double2*>(chi_aux_sum);
+  a.info = info;
+  a.nchan = nchan;
+  a.nq = nq;
+  a.nw = nw;
+  a.T = nb * nb;
+  a.nv = 2 * niv;
+  a.n = a.T * a.nv;
+  a.inv_beta2 = 1.0 / (beta * beta);
+  a.inv_beta = 1.0 / beta;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == DGAMI_PLAIN_FP64) return ladder_solve_fp64(a, s);
+  return ladder_solve_mixed(a, precision, s);
+}

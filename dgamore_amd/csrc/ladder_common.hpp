@@ -1,0 +1,52 @@
+// Shared definitions of the ladder solves (plain FP64 and mixed precision).
+#pragma once
+
+#include "dgami_internal.hpp"
+
+namespace dgami {
+
+// One batch of ladder problems: item = (chan, w, q), q fastest.
+struct LadderArgs {
+  const double2* chi0_inv;  // QF [nw][T*T][nv][nq]
+  const double2* gamma;     // [nchan][nw][n][n]
+  const double2* u_r;       // [nchan][T*T]
+  const double2* v_r;       // [nchan][nq][T*T] or nullptr
+  double2* out;             // [nchan][nw][T*T][nv][nq]
+  int32_t* info;            // [nchan][nw][nq]
+  int nchan, nq, nw;
+  int T;     // nb^2 (number of right-hand sides)
+  int nv;    // 2 * niv
+  int n;     // T * nv
+  double inv_beta2, inv_beta;
+};
+
+// Element (i, j) of the ladder matrix A(chan, w, q) in compound indices
+// (nonlocal_sde.py:93-95): chi0_inv delta_vv' + (gamma - u_r - v_r)/beta^2.
+__device__ __forceinline__ double2 ladder_elem(const LadderArgs& a, const double2* __restrict__ G,
+                                               const double2* __restrict__ U, const double2* __restrict__ V, int w,
+                                               int q, int i, int j) {
+  const int nv = a.nv, T = a.T;
+  const int ri = i / nv, vi = i - ri * nv;
+  const int cj = j / nv, vj = j - cj * nv;
+  double2 uv = U[ri * T + cj];
+  if (V) uv = cadd(uv, V[ri * T + cj]);
+  double2 val = cscale(csub(G[static_cast<size_t>(i) * a.n + j], uv), a.inv_beta2);
+  if (vi == vj) val = cadd(val, a.chi0_inv[((static_cast<size_t>(w) * T * T + ri * T + cj) * nv + vi) * a.nq + q]);
+  return val;
+}
+
+// Indicator right-hand side B[i][r] = 1 iff the orbital pair of row i is r (four_point.py:661-668).
+__device__ __forceinline__ double2 ladder_rhs(const LadderArgs& a, int i, int r) {
+  return make_double2((i / a.nv) == r ? 1.0 : 0.0, 0.0);
+}
+
+// Output element chi_aux_sum[chan][w][(ri, r)][vi][q] = x[i][r] / beta.
+__device__ __forceinline__ size_t ladder_out_index(const LadderArgs& a, int chan, int w, int q, int i, int r) {
+  const int ri = i / a.nv, vi = i - ri * a.nv;
+  return ((((static_cast<size_t>(chan) * a.nw + w) * a.T * a.T + ri * a.T + r) * a.nv) + vi) * a.nq + q;
+}
+
+int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream);
+int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream);
+
+}  // namespace dgami

@@ -1,0 +1,206 @@
+// Linearised Eliashberg operator (eliashberg_solver.py:376-428).
+//
+// Set-up: Gamma_pp(q) is mapped irr -> full zone, scaled by the channel sign
+// s, Fourier transformed over q, and stored per lattice vector R as two dense
+// matrices with rows (a,b,v) and columns (c,d,v'):
+//   g_hat[R][(a,b,v)][(c,d,p)]      = Ghat(R)_{a c b d}(v, p)
+//   g_hat_flip[R][(a,b,v)][(c,d,p)] = Ghat(-R)_{a d b c}(v, np2-1-p)
+// (the reference's einsum "xyzacbdvp,xyzcdp->xyzabv" and its flipped copy).
+// Matvec for nvec vectors: gg = fft_k(chi0pp . gap); per R one pass over
+// both matrices (memory bound: the matrices are read once per application,
+// shared by all nvec vectors); then ifft_k with the 0.5/(N beta) norm.
+#include "dgami.h"
+#include "dgami_internal.hpp"
+#include "dgami_launch.hpp"
+
+namespace dgami {
+
+constexpr int PAIR_MAXVEC = 8;
+
+__global__ void pair_gather_kernel(const double2* __restrict__ girr, const int32_t* __restrict__ irrk_inv,
+                                   double2* __restrict__ out, int N, int nq, int nelem, double sign) {
+  const size_t total = static_cast<size_t>(nelem) * N;
+  for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>(idx % N);
+    const size_t e = idx / N;
+    const int qi = irrk_inv[k];
+    out[idx] = (qi >= 0 && qi < nq) ? cscale(girr[static_cast<size_t>(qi) * nelem + e], sign)
+                                    : make_double2(__builtin_nan(""), __builtin_nan(""));
+  }
+}
+
+// hat: [nb][nb][nb][nb][np2][np2][N] (reference element order, R fastest)
+__global__ void pair_scatter_kernel(const double2* __restrict__ hat, double2* __restrict__ g_hat,
+                                    double2* __restrict__ g_flip, FftPlan3D plan, int nb, int np2) {
+  const int N = plan.ntot;
+  const int n = nb * nb * np2;
+  const size_t total = static_cast<size_t>(N) * n * n;
+  for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int col = static_cast<int>(idx % n);
+    const int row = static_cast<int>((idx / n) % n);
+    const int R = static_cast<int>(idx / (static_cast<size_t>(n) * n));
+    const int a = row / (nb * np2), b = (row / np2) % nb, v = row % np2;
+    const int c = col / (nb * np2), d = (col / np2) % nb, p = col % np2;
+    const int z = R % plan.n[2], y = (R / plan.n[2]) % plan.n[1], x = R / (plan.n[2] * plan.n[1]);
+    const int Rm = (((plan.n[0] - x) % plan.n[0]) * plan.n[1] + (plan.n[1] - y) % plan.n[1]) * plan.n[2] +
+                   (plan.n[2] - z) % plan.n[2];
+    const size_t e1 = (((((static_cast<size_t>(a) * nb + c) * nb + b) * nb + d) * np2 + v) * np2 + p);
+    const size_t e2 = (((((static_cast<size_t>(a) * nb + d) * nb + b) * nb + c) * np2 + v) * np2 + (np2 - 1 - p));
+    g_hat[idx] = hat[e1 * N + R];
+    g_flip[idx] = hat[e2 * N + Rm];
+  }
+}
+
+// t[vec][(a,b,v)][k] = sum_cd chi0pp[k][abcd][v] gap[vec][k][c][d][v]
+__global__ void pair_bubble_kernel(const double2* __restrict__ chi0pp, const double2* __restrict__ gap,
+                                   double2* __restrict__ t, int N, int nb, int np2, int nvec) {
+  const int T = nb * nb;
+  const size_t total = static_cast<size_t>(nvec) * T * np2 * N;
+  for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int k = static_cast<int>(idx % N);
+    size_t r = idx / N;
+    const int v = static_cast<int>(r % np2);
+    r /= np2;
+    const int ab = static_cast<int>(r % T);
+    const int vec = static_cast<int>(r / T);
+    double2 acc = make_double2(0.0, 0.0);
+    for (int cd = 0; cd < T; ++cd)
+      acc = cfma(acc, chi0pp[(static_cast<size_t>(k) * T * T + ab * T + cd) * np2 + v],
+                 gap[((static_cast<size_t>(vec) * N + k) * T + cd) * np2 + v]);
+    t[idx] = acc;
+  }
+}
+
+// y[vec][row][R] = sum_col g_hat[R][row][col] x[vec][col][R] + s g_flip[R][row][col] x[vec][col][-R]
+__global__ void __launch_bounds__(256) pair_gemv_kernel(const double2* __restrict__ g_hat,
+                                                       const double2* __restrict__ g_flip, const double2* __restrict__ x,
+                                                       double2* __restrict__ y, FftPlan3D plan, int n, int nvec,
+                                                       double sign) {
+  extern __shared__ double2 s_x[];  // [2][nvec][n]: x(R) and x(-R)
+  const int N = plan.ntot;
+  const int R = blockIdx.x;
+  const int z = R % plan.n[2], yy = (R / plan.n[2]) % plan.n[1], xx = R / (plan.n[2] * plan.n[1]);
+  const int Rm = (((plan.n[0] - xx) % plan.n[0]) * plan.n[1] + (plan.n[1] - yy) % plan.n[1]) * plan.n[2] +
+                 (plan.n[2] - z) % plan.n[2];
+  for (int e = threadIdx.x; e < nvec * n; e += blockDim.x) {
+    s_x[e] = x[static_cast<size_t>(e) * N + R];
+    s_x[nvec * n + e] = x[static_cast<size_t>(e) * N + Rm];
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const double2* M1 = g_hat + static_cast<size_t>(R) * n * n;
+  const double2* M2 = g_flip + static_cast<size_t>(R) * n * n;
+  for (int row = wave; row < n; row += 4) {
+    double2 acc[PAIR_MAXVEC];
+#pragma unroll
+    for (int vv = 0; vv < PAIR_MAXVEC; ++vv) acc[vv] = make_double2(0.0, 0.0);
+    for (int col = lane; col < n; col += 64) {
+      const double2 m1 = M1[static_cast<size_t>(row) * n + col];
+      const double2 m2 = cscale(M2[static_cast<size_t>(row) * n + col], sign);
+#pragma unroll
+      for (int vv = 0; vv < PAIR_MAXVEC; ++vv) {
+        if (vv < nvec) {
+          acc[vv] = cfma(acc[vv], m1, s_x[vv * n + col]);
+          acc[vv] = cfma(acc[vv], m2, s_x[nvec * n + vv * n + col]);
+        }
+      }
+    }
+#pragma unroll
+    for (int vv = 0; vv < PAIR_MAXVEC; ++vv) {
+      if (vv >= nvec) break;
+      double re = acc[vv].x, im = acc[vv].y;
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) {
+        re += __shfl_xor(re, off);
+        im += __shfl_xor(im, off);
+      }
+      if (lane == 0) y[(static_cast<size_t>(vv) * n + row) * N + R] = make_double2(re, im);
+    }
+  }
+}
+
+// out[vec][k][a][b][v] <- t[vec][(a,b,v)][k]
+__global__ void pair_out_kernel(const double2* __restrict__ t, double2* __restrict__ out, int N, int n, int nvec) {
+  const size_t total = static_cast<size_t>(nvec) * N * n;
+  for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int row = static_cast<int>(idx % n);
+    const int k = static_cast<int>((idx / n) % N);
+    const int vec = static_cast<int>(idx / (static_cast<size_t>(n) * N));
+    out[idx] = t[(static_cast<size_t>(vec) * n + row) * N + k];
+  }
+}
+
+inline unsigned grid_for(size_t total) { return ceil_div(total, 256) > 262144u ? 262144u : ceil_div(total, 256); }
+
+}  // namespace dgami
+
+extern "C" int dgami_pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel,
+                                   const dgami_c128* gamma_pp_irr, const int32_t* irrk_inv, int nq_irr,
+                                   dgami_c128* g_hat, dgami_c128* g_hat_flip, void* stream_) {
+  using namespace dgami;
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  DGAMI_ARG_CHECK(gamma_pp_irr && irrk_inv && g_hat && g_hat_flip, "pairing_setup: null pointer");
+  DGAMI_ARG_CHECK(nb >= 1 && np2 >= 1 && nq_irr >= 1, "pairing_setup: invalid sizes");
+  DGAMI_ARG_CHECK(channel == 0 || channel == 1, "pairing_setup: channel must be 0 (singlet) or 1 (triplet)");
+  FftPlan3D plan;
+  DGAMI_ARG_CHECK(nx > 0 && ny > 0 && nz > 0 && fft_plan_3d(nx, ny, nz, &plan),
+                  "pairing_setup: grid lengths must factor into 2, 3 and 5");
+  const int N = plan.ntot;
+  const int nelem = nb * nb * nb * nb * np2 * np2;
+  double2* hat = static_cast<double2*>(workspace(WS_PAIR, static_cast<size_t>(nelem) * N * sizeof(double2)));
+  if (!hat) return -2;
+  const double sign = channel == 0 ? 1.0 : -1.0;
+  hipLaunchKernelGGL(pair_gather_kernel, dim3(grid_for(static_cast<size_t>(nelem) * N)), dim3(256), 0, stream,
+                     reinterpret_cast<const double2*>(gamma_pp_irr), irrk_inv, hat, N, nq_irr, nelem, sign);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  int rc = launch_fft_grid(hat, hat, nelem, nx, ny, nz, -1, 1.0, stream);
+  if (rc) return rc;
+  const int n = nb * nb * np2;
+  hipLaunchKernelGGL(pair_scatter_kernel, dim3(grid_for(static_cast<size_t>(N) * n * n)), dim3(256), 0, stream, hat,
+                     reinterpret_cast<double2*>(g_hat), reinterpret_cast<double2*>(g_hat_flip), plan, nb, np2);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int dgami_pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta,
+                                    const dgami_c128* g_hat, const dgami_c128* g_hat_flip, const dgami_c128* chi0pp,
+                                    int nvec, const dgami_c128* gap, dgami_c128* out, void* stream_) {
+  using namespace dgami;
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  DGAMI_ARG_CHECK(g_hat && g_hat_flip && chi0pp && gap && out, "pairing_matvec: null pointer");
+  DGAMI_ARG_CHECK(nvec >= 1 && nvec <= PAIR_MAXVEC, "pairing_matvec: nvec must be 1..8");
+  DGAMI_ARG_CHECK(channel == 0 || channel == 1, "pairing_matvec: channel must be 0 or 1");
+  DGAMI_ARG_CHECK(nb >= 1 && np2 >= 1 && beta > 0, "pairing_matvec: invalid sizes");
+  FftPlan3D plan;
+  DGAMI_ARG_CHECK(nx > 0 && ny > 0 && nz > 0 && fft_plan_3d(nx, ny, nz, &plan),
+                  "pairing_matvec: grid lengths must factor into 2, 3 and 5");
+  const int N = plan.ntot;
+  const int n = nb * nb * np2;
+  const size_t lds = static_cast<size_t>(2) * nvec * n * sizeof(double2);
+  DGAMI_ARG_CHECK(lds <= 160 * 1024, "pairing_matvec: nvec * nb^2 * np2 too large for LDS");
+  const size_t vsz = static_cast<size_t>(nvec) * n * N;
+  double2* t = static_cast<double2*>(workspace(WS_MISC, 2 * vsz * sizeof(double2)));
+  if (!t) return -2;
+  double2* yv = t + vsz;
+  hipLaunchKernelGGL(pair_bubble_kernel, dim3(grid_for(vsz)), dim3(256), 0, stream,
+                     reinterpret_cast<const double2*>(chi0pp), reinterpret_cast<const double2*>(gap), t, N, nb, np2,
+                     nvec);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  int rc = launch_fft_grid(t, t, static_cast<int64_t>(nvec) * n, nx, ny, nz, -1, 1.0, stream);
+  if (rc) return rc;
+  allow_lds(pair_gemv_kernel, lds);
+  hipLaunchKernelGGL(pair_gemv_kernel, dim3(N), dim3(256), lds, stream, reinterpret_cast<const double2*>(g_hat),
+                     reinterpret_cast<const double2*>(g_hat_flip), t, yv, plan, n, nvec, channel == 0 ? 1.0 : -1.0);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  rc = launch_fft_grid(yv, yv, static_cast<int64_t>(nvec) * n, nx, ny, nz, 1, 0.5 / (static_cast<double>(N) * beta) / N,
+                       stream);
+  if (rc) return rc;
+  hipLaunchKernelGGL(pair_out_kernel, dim3(grid_for(vsz)), dim3(256), 0, stream, yv, reinterpret_cast<double2*>(out), N,
+                     n, nvec);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}

@@ -1,0 +1,87 @@
+// Batched complex128 GEMM on LDS tiles (vector FMA), C = alpha A B + beta C.
+// Row-major operands with leading dimensions and batch strides (elements).
+#pragma once
+
+#include "dgami_internal.hpp"
+
+namespace dgami {
+
+struct ZgemmArgs {
+  const double2* A;
+  const double2* B;
+  double2* C;
+  int M, N, K;
+  int lda, ldb, ldc;
+  long long sa, sb, sc;  // batch strides
+  double2 alpha, beta;
+};
+
+constexpr int ZG_TM = 64, ZG_TN = 64, ZG_TK = 16;
+
+// grid: (ceil(N/64), ceil(M/64), batch); block 256
+__global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g) {
+  const int b = blockIdx.z;
+  const double2* A = g.A + b * g.sa;
+  const double2* B = g.B + b * g.sb;
+  double2* C = g.C + b * g.sc;
+  const int m0 = blockIdx.y * ZG_TM, n0 = blockIdx.x * ZG_TN;
+  __shared__ double2 s_a[ZG_TM][ZG_TK + 1];
+  __shared__ double2 s_b[ZG_TK][ZG_TN + 1];
+  const int tid = threadIdx.x;
+  const int ty = tid / 16, tx = tid % 16;
+  double2 acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+  for (int k0 = 0; k0 < g.K; k0 += ZG_TK) {
+    for (int e = tid; e < ZG_TM * ZG_TK; e += 256) {
+      const int r = e / ZG_TK, s = e % ZG_TK;
+      s_a[r][s] = (m0 + r < g.M && k0 + s < g.K) ? A[static_cast<size_t>(m0 + r) * g.lda + k0 + s]
+                                                  : make_double2(0.0, 0.0);
+    }
+    for (int e = tid; e < ZG_TK * ZG_TN; e += 256) {
+      const int s = e / ZG_TN, c = e % ZG_TN;
+      s_b[s][c] = (k0 + s < g.K && n0 + c < g.N) ? B[static_cast<size_t>(k0 + s) * g.ldb + n0 + c]
+                                                  : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int s = 0; s < ZG_TK; ++s) {
+      double2 av[4], bv[4];
+#pragma unroll
+      for (int i = 0; i < 4; ++i) av[i] = s_a[ty + 16 * i][s];
+#pragma unroll
+      for (int j = 0; j < 4; ++j) bv[j] = s_b[s][tx + 16 * j];
+#pragma unroll
+      for (int i = 0; i < 4; ++i)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc[i][j] = cfma(acc[i][j], av[i], bv[j]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const int r = m0 + ty + 16 * i;
+    if (r >= g.M) continue;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int c = n0 + tx + 16 * j;
+      if (c >= g.N) continue;
+      double2* e = C + static_cast<size_t>(r) * g.ldc + c;
+      double2 val = cmul(g.alpha, acc[i][j]);
+      if (g.beta.x != 0.0 || g.beta.y != 0.0) val = cfma(val, g.beta, *e);
+      *e = val;
+    }
+  }
+}
+
+inline int launch_zgemm(const ZgemmArgs& g, int batch, hipStream_t stream) {
+  if (g.M <= 0 || g.N <= 0 || batch <= 0) return 0;
+  dim3 grid(ceil_div(g.N, ZG_TN), ceil_div(g.M, ZG_TM), batch);
+  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, stream, g);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace dgami

@@ -1,0 +1,214 @@
+"""Device-tensor wrappers over the libdgami C ABI.
+
+All arrays are torch tensors on the current CUDA (HIP) device; complex data is
+torch.complex128 (the memory layout of numpy.complex128). Layout names (REF2,
+QF, QW, QO, CMP, M order) are those of include/dgami.h. torch is used for
+device memory, streams and layout permutations only; every arithmetic step of
+the hot path runs in libdgami.
+"""
+
+import ctypes
+
+import numpy as np
+import torch
+
+from dgamore_amd import _lib
+
+C128 = torch.complex128
+
+
+def _p(t, dtype=C128, name="array"):
+    if t is None:
+        return None
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise ValueError(f"{name} must be a torch tensor on the GPU")
+    if t.dtype != dtype:
+        raise ValueError(f"{name} must have dtype {dtype}, got {t.dtype}")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream():
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def device():
+    if not torch.cuda.is_available():
+        raise RuntimeError("dgamore_amd needs a ROCm GPU (gfx950); none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def to_dev(x, dtype=C128):
+    """numpy -> contiguous device tensor."""
+    return torch.as_tensor(np.ascontiguousarray(x)).to(device=device(), dtype=dtype).contiguous()
+
+
+# ---------------------------------------------------------------------------
+# layout conversions (pure permutations)
+# ---------------------------------------------------------------------------
+def ref4_to_qf(x):
+    """[q,o1,o2,o3,o4,w,v] -> QF [w][(o1,o2),(o4,o3)][v][q]."""
+    nq, nb, nw, nv = x.shape[0], x.shape[1], x.shape[5], x.shape[6]
+    return x.permute(5, 1, 2, 4, 3, 6, 0).reshape(nw, nb**4, nv, nq).contiguous()
+
+
+def qf_to_ref4(x, nb):
+    """QF [w][nb^4][v][q] -> [q,o1,o2,o3,o4,w,v]."""
+    nw, _, nv, nq = x.shape
+    return x.reshape(nw, nb, nb, nb, nb, nv, nq).permute(6, 1, 2, 4, 3, 0, 5).contiguous()
+
+
+def ref4w_to_qw(x):
+    """[q,o1,o2,o3,o4,w] -> QW [w][nb^4 M order][q]."""
+    nq, nb, nw = x.shape[0], x.shape[1], x.shape[5]
+    return x.permute(5, 1, 2, 4, 3, 0).reshape(nw, nb**4, nq).contiguous()
+
+
+def qw_to_ref4w(x, nb):
+    nw, _, nq = x.shape
+    return x.reshape(nw, nb, nb, nb, nb, nq).permute(5, 1, 2, 4, 3, 0).contiguous()
+
+
+def orb_to_m(u):
+    """[..., o1,o2,o3,o4] (reference order) -> [..., nb^4] M order ((o1,o2),(o4,o3))."""
+    nb = u.shape[-1]
+    return u.transpose(-1, -2).reshape(*u.shape[:-4], nb**4).contiguous()
+
+
+def vertex_to_cmp(g):
+    """Local vertex [o1,o2,o3,o4,w,v,v'] -> CMP [w][n][n] (local_four_point.py:206-229)."""
+    nb, nw, nv = g.shape[0], g.shape[4], g.shape[5]
+    n = nb * nb * nv
+    return g.permute(4, 0, 1, 5, 3, 2, 6).reshape(nw, n, n).contiguous()
+
+
+# ---------------------------------------------------------------------------
+# kernels
+# ---------------------------------------------------------------------------
+def fft_grid(x, nk, sign, scale=1.0, out=None):
+    """Batched FFT over the trailing nx*ny*nz axis of x ([batch][nk])."""
+    nx, ny, nz = nk
+    out = x if out is None else out
+    batch = x.numel() // (nx * ny * nz)
+    _lib.call("dgami_fft_grid", _p(x), _p(out), batch, nx, ny, nz, sign, float(scale), _stream())
+    return out
+
+
+def bubble_irrq(giwk, nk, niw, niv, beta, irrk_ind):
+    """giwk: REF2 [nx,ny,nz,nb,nb,2niv_g]; irrk_ind: int32 [nq]. Returns chi0 QF [niw+1][nb^4][2niv][nq]."""
+    nx, ny, nz = nk
+    nb = giwk.shape[3]
+    niv_g = giwk.shape[-1] // 2
+    nq = irrk_ind.shape[0]
+    chi0 = torch.empty((niw + 1, nb**4, 2 * niv, nq), dtype=C128, device=giwk.device)
+    _lib.call("dgami_bubble_irrq", nx, ny, nz, nb, niv_g, niw, niv, float(beta), _p(giwk, name="giwk"),
+              _p(irrk_ind, torch.int32, "irrk_ind"), nq, _p(chi0), _stream())
+    return chi0
+
+
+def chi0_prepare(chi0, nb, niv_core, beta):
+    """chi0 QF (niv_full) -> (chi0_inv QF core, full_sum QW, core_sum QW)."""
+    nw, _, nvf, nq = chi0.shape
+    inv = torch.empty((nw, nb**4, 2 * niv_core, nq), dtype=C128, device=chi0.device)
+    full = torch.empty((nw, nb**4, nq), dtype=C128, device=chi0.device)
+    core = torch.empty_like(full)
+    _lib.call("dgami_chi0_prepare", nq, nb, nw, nvf // 2, niv_core, float(beta), _p(chi0), _p(inv), _p(full),
+              _p(core), _stream())
+    return inv, full, core
+
+
+def ladder_solve_sum(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, precision=_lib.PLAIN_FP64, check_info=True):
+    """Batched ladder solves for nchan channels.
+
+    chi0_inv: QF [nw][nb^4][2niv][nq]; gamma_cmp: [nchan][nw][n][n]; u_r_m: [nchan][nb^4];
+    v_r_m: [nchan][nq][nb^4] or None. Returns (chi_aux_sum [nchan] x QF, info int32 [nchan][nw][nq]).
+    """
+    nw, nb4, nv, nq = chi0_inv.shape
+    nb = int(round(nb4 ** 0.25))
+    nchan = gamma_cmp.shape[0]
+    n = nb * nb * nv
+    if gamma_cmp.shape[1:] != (nw, n, n):
+        raise ValueError(f"gamma must be [nchan,{nw},{n},{n}], got {tuple(gamma_cmp.shape)}")
+    out = torch.empty((nchan, nw, nb4, nv, nq), dtype=C128, device=chi0_inv.device)
+    info = torch.zeros((nchan, nw, nq), dtype=torch.int32, device=chi0_inv.device)
+    _lib.call("dgami_ladder_solve_sum", nchan, nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp),
+              _p(u_r_m), _p(v_r_m), int(precision), _p(out), _p(info, torch.int32), _stream())
+    if check_info:
+        bad = torch.nonzero(info).cpu().numpy()
+        if len(bad):
+            raise np.linalg.LinAlgError(f"singular ladder matrix at (chan, w, q) = {bad[:8].tolist()}")
+    return out, info
+
+
+def vrg_chi_phys(chi0_inv, aux, full, core, u_r_m, v_r_m, nb, beta, want_vrg=True):
+    """-> (vrg QF or None, chi_phys QW)."""
+    nw, _, nv, nq = aux.shape
+    vrg = torch.empty_like(aux) if want_vrg else None
+    chi = torch.empty((nw, nb**4, nq), dtype=C128, device=aux.device)
+    info = torch.zeros((nw, nq), dtype=torch.int32, device=aux.device)
+    _lib.call("dgami_vrg_chi_phys", nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(aux), _p(full), _p(core),
+              _p(u_r_m), _p(v_r_m), _p(vrg), _p(chi), _p(info, torch.int32), _stream())
+    return vrg, chi
+
+
+def kernel_r(vrg, chi_phys, u_r_m, v_r_m, nb, channel, weight, kernel=None, accumulate=False):
+    """kernel (QF) = [kernel +] weight * K_r; channel 'dens' or 'magn'."""
+    nw, _, nv, nq = vrg.shape
+    if kernel is None:
+        kernel = torch.empty_like(vrg)
+        accumulate = False
+    ch = {"dens": 0, "magn": 1}[channel]
+    _lib.call("dgami_kernel_r", nq, nb, nw, nv // 2, ch, float(weight), _p(vrg), _p(chi_phys), _p(u_r_m),
+              _p(v_r_m), int(bool(accumulate)), _p(kernel), _stream())
+    return kernel
+
+
+def dc_kernel(chi0, u_ref, f_magn_loc, nb, niv_core, beta):
+    """-K_dc in QF core layout. chi0 QF (niv_full); u_ref [nb]*4; f_magn_loc [nb]*4+[nw,2nvf,2nvf]."""
+    nw, _, nvf, nq = chi0.shape
+    out = torch.empty((nw, nb**4, 2 * niv_core, nq), dtype=C128, device=chi0.device)
+    _lib.call("dgami_dc_kernel", nq, nb, nw, nvf // 2, niv_core, float(beta), _p(chi0), _p(u_ref),
+              _p(f_magn_loc), _p(out), _stream())
+    return out
+
+
+def sigma_from_kernel(kernel_irr, irrk_inv, giwk, nk, niw, beta):
+    """Sigma(k, v) REF2 [nx,ny,nz,nb,nb,2niv_core] from the irreducible kernel (QF)."""
+    nx, ny, nz = nk
+    nb = giwk.shape[3]
+    niv_g = giwk.shape[-1] // 2
+    nv = kernel_irr.shape[2]
+    nq = kernel_irr.shape[3]
+    sigma = torch.empty((nx, ny, nz, nb, nb, nv), dtype=C128, device=giwk.device)
+    _lib.call("dgami_sigma_from_kernel", nx, ny, nz, nb, niw, nv // 2, niv_g, float(beta), _p(kernel_irr),
+              _p(irrk_inv, torch.int32, "irrk_inv"), nq, _p(giwk), _p(sigma), _stream())
+    return sigma
+
+
+def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel):
+    """gamma_pp_irr [nq,nb,nb,nb,nb,np2,np2] -> (g_hat, g_hat_flip) [N][n][n]."""
+    nx, ny, nz = nk
+    nq, nb, np2 = gamma_pp_irr.shape[0], gamma_pp_irr.shape[1], gamma_pp_irr.shape[-1]
+    n = nb * nb * np2
+    N = nx * ny * nz
+    g_hat = torch.empty((N, n, n), dtype=C128, device=gamma_pp_irr.device)
+    g_flip = torch.empty_like(g_hat)
+    ch = {"sing": 0, "trip": 1}[channel]
+    _lib.call("dgami_pairing_setup", nx, ny, nz, nb, np2, ch, _p(gamma_pp_irr), _p(irrk_inv, torch.int32), nq,
+              _p(g_hat), _p(g_flip), _stream())
+    return g_hat, g_flip
+
+
+def pairing_matvec(g_hat, g_flip, chi0pp, gap, nk, channel, beta):
+    """gap [nvec][nx,ny,nz,nb,nb,np2] (or without the vec axis) -> same shape."""
+    nx, ny, nz = nk
+    squeeze = gap.dim() == 6
+    g = gap.unsqueeze(0) if squeeze else gap
+    nvec, nb, np2 = g.shape[0], g.shape[4], g.shape[-1]
+    g = g.contiguous()
+    out = torch.empty_like(g)
+    ch = {"sing": 0, "trip": 1}[channel]
+    _lib.call("dgami_pairing_matvec", nx, ny, nz, nb, np2, ch, float(beta), _p(g_hat), _p(g_flip), _p(chi0pp),
+              nvec, _p(g), _p(out), _stream())
+    return out[0] if squeeze else out

@@ -1,0 +1,166 @@
+/*
+ * libdgami -- MI355X (gfx950) kernels for the nonlocal ladder-DGA hot path of
+ * DGAmore (Julpe/DGAmore). C ABI: plain pointers, sizes and a stream handle.
+ *
+ * Conventions
+ * -----------
+ * - Complex numbers are dgami_c128: interleaved (re, im) doubles, the memory
+ *   layout of numpy.complex128.
+ * - Every array pointer is a DEVICE pointer owned by the caller (hipMalloc or
+ *   a torch tensor on cuda:0); the library never frees caller memory. Scratch
+ *   memory is a grow-only per-process workspace owned by the library.
+ * - `stream` is a hipStream_t passed as void* (NULL: the default stream).
+ *   Calls are asynchronous with respect to the host.
+ * - Return value: 0 on success; -1 invalid argument (text in
+ *   dgami_last_error()); -2 HIP runtime error; >0 numerical problem (the
+ *   number of singular matrices; per-matrix codes are in the `info` array).
+ *   The Python layer maps -1 to ValueError, -2 to RuntimeError and >0 to
+ *   numpy.linalg.LinAlgError, the errors the reference raises.
+ *
+ * Layout names
+ * ------------
+ * REF2  two-point object in the reference layout [nx][ny][nz][nb][nb][2*niv]
+ *       (GreensFunction.mat / SelfEnergy.mat, C order).
+ * QF    4-orbital object per (q, w, v): [nw][nb^4][nv][nq]  (q fastest).
+ * QW    4-orbital object per (q, w):    [nw][nb^4][nq].
+ * QO    4-orbital object per q:         [nq][nb^4].
+ * nb^4 is in "M order": orbital tuple (o1,o2,o3,o4) is stored at
+ *       ((o1*nb + o2)*nb + o4)*nb + o3, i.e. the reference's compound block
+ *       [(o1,o2),(o4,o3)] (four_point.py:184-221) row-major, so that the
+ *       reference's contraction X_abcd Y_dcef is the matrix product M(X)M(Y).
+ * CMP   compound ladder matrix per (w): [nw][n][n], n = nb^2 * 2*niv, rows
+ *       (o1,o2,v) and cols (o4,o3,v') with v fastest (four_point.py:216-219).
+ * Momentum index k = (x*ny + y)*nz + z everywhere.
+ */
+#ifndef DGAMI_H
+#define DGAMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  double re, im;
+} dgami_c128;
+
+/* Ladder precision modes (dgami_ladder_solve_sum). */
+enum {
+  DGAMI_PLAIN_FP64 = 0,    /* complex128 LU, the reference's algorithm in FP64 */
+  DGAMI_MIXED_FP32_IR = 1, /* FP32 LU + FP64 residual iterative refinement */
+  DGAMI_MIXED_BF16X3_IR = 2 /* split-BF16 (3-term) MFMA trailing updates + FP64 IR */
+};
+
+/* Text of the last error on the calling host thread ("" if none). */
+const char* dgami_last_error(void);
+
+/* ABI version of the library (incremented on incompatible changes). */
+int dgami_version(void);
+
+/* Batched FFT over momentum grids. in/out: [batch][nx*ny*nz] (may alias).
+ * sign -1: forward, unnormalised, exp(-i k.R); sign +1: backward exp(+i k.R);
+ * the result is multiplied by `scale`.
+ * Replaces scipy.fft.fftn / ifftn over the momentum axes
+ * (n_point_base.py:509-551, mpi_utils.py:499-542, eliashberg_solver.py:421-428). */
+int dgami_fft_grid(const dgami_c128* in, dgami_c128* out, int64_t batch, int nx, int ny, int nz, int sign,
+                   double scale, void* stream);
+
+/* Bare bubble on the irreducible q-points, FFT form:
+ *   chi0[w][abcd][v][q] = -beta/N sum_k G_ad(k, v) G_cb(k - q, v - w),
+ *   w in [0, niw], v in [-niv, niv), then |Re|,|Im| < 1e-12 -> 0.
+ * g_k: REF2 with 2*niv_g frequencies, niv_g >= niv + niw. chi0: QF with nv = 2*niv, nq = nq_irr.
+ * irrk_ind: [nq_irr] flat momentum index of each irreducible q (KGrid.irrk_ind).
+ * Replaces BubbleGenerator.create_generalized_chi0_q_fft_{cpu,gpu,auto} (bubble_gen.py:33-144). */
+int dgami_bubble_irrq(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
+                      const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0, void* stream);
+
+/* chi0 post-processing for one SDE iteration:
+ *   chi0_full_sum = (1/beta^2) sum_v chi0           (all 2*niv_full frequencies)
+ *   chi0_core_sum = (1/beta^2) sum_{|v|<niv_core} chi0
+ *   chi0_inv      = per-(q,w,v) inverse of the core blocks [(o1,o2),(o4,o3)]
+ * chi0: QF with nv = 2*niv_full; chi0_inv: QF with nv = 2*niv_core; sums: QW. Any output may be NULL.
+ * Replaces cut_niv + FourPoint.invert(copy=False) for num_vn=1 + sum_over_all_vn
+ * (nonlocal_sde.py:824-835, four_point.py:601-613, local_four_point.py:164-177). */
+int dgami_chi0_prepare(int nq, int nb, int nw, int niv_full, int niv_core, double beta, const dgami_c128* chi0,
+                       dgami_c128* chi0_inv, dgami_c128* chi0_full_sum, dgami_c128* chi0_core_sum, void* stream);
+
+/* Batched ladder solves with the nu' sum, for nchan channels at once:
+ *   A(c,q,w) = chi0_inv(q,w,v) delta_vv' + (gamma[c][w] - u_r[c] - v_r[c][q]) / beta^2   (CMP)
+ *   chi_aux_sum[c][w][(o1,o2),(o4,o3)][v][q] = (1/beta) sum_v' [A^-1]_{(o1,o2,v),(o4,o3,v')}
+ * computed as an LU solve with the nb^2 indicator right-hand sides.
+ * chi0_inv: QF (nv = 2*niv); gamma: [nchan] x CMP; u_r: [nchan][nb^4] (M order);
+ * v_r: [nchan] x QO or NULL (V = 0); chi_aux_sum: [nchan] x QF; info: [nchan][nw][nq] int32,
+ * 0 or k+1 for an exactly singular pivot at elimination step k.
+ * precision: DGAMI_PLAIN_FP64 / DGAMI_MIXED_FP32_IR / DGAMI_MIXED_BF16X3_IR.
+ * Replaces create_auxiliary_chi_r_q_sum_v1/_v3 (nonlocal_sde.py:85-156) with
+ * FourPoint.invert_and_sum_over_last_vn/_v2 (four_point.py:620-684). */
+int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                           const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r, int precision,
+                           dgami_c128* chi_aux_sum, int32_t* info, void* stream);
+
+/* Three-leg vertex and shell-corrected physical susceptibility of one channel:
+ *   vrg      = beta chi0_inv . chi_aux_sum                              (QF)
+ *   chi_phys = ((sum_v chi_aux_sum / beta + chi0_full_sum - chi0_core_sum)^-1 + u_r + v_r)^-1  (QW)
+ * u_r: [nb^4] M order; v_r: QO or NULL. info: [nw][nq], nonzero where a block was singular.
+ * Replaces create_vrg_r_q + sum_over_all_vn + create_generalized_chi_q_with_shell_correction
+ * (nonlocal_sde.py:159-182, 289-303). */
+int dgami_vrg_chi_phys(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                       const dgami_c128* chi_aux_sum, const dgami_c128* chi0_full_sum,
+                       const dgami_c128* chi0_core_sum, const dgami_c128* u_r, const dgami_c128* v_r,
+                       dgami_c128* vrg, dgami_c128* chi_phys, int32_t* info, void* stream);
+
+/* Self-energy kernel of one channel, accumulated:
+ *   K_r = u . (vrg - vrg . u . chi_phys - [magn] 2/3 1),  u = u_r + v_r(q)
+ *   kernel = accumulate ? kernel + weight K_r : weight K_r                   (QF)
+ * channel: 0 density, 1 magnetic.
+ * Replaces calculate_kernel_r_q (nonlocal_sde.py:201-215) and the channel sum
+ * kernel += K_dens + 3 K_magn (nonlocal_sde.py:838-857). */
+int dgami_kernel_r(int nq, int nb, int nw, int niv, int channel, double weight, const dgami_c128* vrg,
+                   const dgami_c128* chi_phys, const dgami_c128* u_r, const dgami_c128* v_r, int accumulate,
+                   dgami_c128* kernel, void* stream);
+
+/* Double-counting kernel, written with the reference's sign (kernel = -K_dc):
+ *   K_dc[q][abef][w][p] = sum_{cd,v} (1/beta^2) (U_adcb-permuted . chi0)_{abcd}(q,w,v) f_dc_{dcef}(w,v,p),
+ *   f_dc = 2 F_magn,loc[abcd->cbad], then p cut to [niv_full-niv_core, niv_full+niv_core).
+ * chi0: QF (nv = 2*niv_full); u: [nb^4] in the reference order abcd;
+ * f_magn_loc: reference layout [nb][nb][nb][nb][nw][2*niv_full][2*niv_full];
+ * kernel: QF with nv = 2*niv_core (overwritten).
+ * Replaces calculate_sigma_dc_kernel and its caller's prefactors (nonlocal_sde.py:185-198, 817-820). */
+int dgami_dc_kernel(int nq, int nb, int nw, int niv_full, int niv_core, double beta, const dgami_c128* chi0,
+                    const dgami_c128* u, const dgami_c128* f_magn_loc, dgami_c128* kernel, void* stream);
+
+/* Non-local self-energy from the kernel, R-space FFT form, in the full fermionic range:
+ *   Sigma_ij(k, v) = -1/(2 beta N) sum_{q, w in [-niw,niw]} sum_{ad} K_aijd(q, w, v) G_ad(k+q, v-w),
+ *   v >= 0, and Sigma(k, -v) = conj(Sigma(k, v)).
+ * kernel_irr: QF on the irreducible q (nv = 2*niv_core); irrk_inv: [nx*ny*nz] (KGrid.irrk_inv);
+ * g_k: REF2 with 2*niv_g frequencies (niv_g >= niw + niv_core); sigma: REF2 with 2*niv_core.
+ * Replaces map_irrbz_fullbz + calculate_sigma_from_kernel_fft_{cpu,gpu,auto} and the caller's
+ * ifft / to_full_niv_range (mpi_utils.py:123-131, nonlocal_sde.py:544-674, 874-879). */
+int dgami_sigma_from_kernel(int nx, int ny, int nz, int nb, int niw, int niv_core, int niv_g, double beta,
+                            const dgami_c128* kernel_irr, const int32_t* irrk_inv, int nq_irr, const dgami_c128* g_k,
+                            dgami_c128* sigma, void* stream);
+
+/* Pairing-vertex preparation for the linearised Eliashberg operator:
+ *   g_hat = s fft_q Gamma_pp (q mapped irr -> FBZ), g_hat_flip = flip_k flip_v' perm(abcd->adcb) g_hat,
+ * stored for the matvec as [nk][(a,b,v)][(c,d,v')] (row (a,b,v) = out, col (c,d,v') = in), n = nb^2*2np.
+ * gamma_pp_irr: [nq_irr][nb][nb][nb][nb][2np][2np] (reference layout, pp orbital order acbd).
+ * channel: 0 singlet (s=+1), 1 triplet (s=-1).
+ * Replaces the set-up of solve_eliashberg_lanczos (eliashberg_solver.py:390-398). */
+int dgami_pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel, const dgami_c128* gamma_pp_irr,
+                        const int32_t* irrk_inv, int nq_irr, dgami_c128* g_hat, dgami_c128* g_hat_flip, void* stream);
+
+/* One application of the linearised gap operator to nvec vectors:
+ *   gg = fft_k(chi0pp . gap); y = ifft_k( (0.5/(N beta)) [g_hat gg + s g_hat_flip flip_k(gg)] ).
+ * chi0pp: [nk][nb^4 (reference order abcd)][np2]; gap, out: [nvec][nk][nb][nb][np2] (reference layout).
+ * Replaces mv() of solve_eliashberg_lanczos (eliashberg_solver.py:420-428). */
+int dgami_pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta, const dgami_c128* g_hat,
+                         const dgami_c128* g_hat_flip, const dgami_c128* chi0pp, int nvec, const dgami_c128* gap,
+                         dgami_c128* out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* DGAMI_H */

@@ -1,0 +1,506 @@
+"""NumPy complex128 restatement of the reference ladder-DGA hot path.
+
+TEST INFRASTRUCTURE ONLY. Imported by ``tests/``, ``__graft_entry__.smoke()``
+and the ``cpu_baseline`` leg of ``bench.py``; never by the product package.
+
+The reference (Julpe/DGAmore) stores every n-point object as complex64
+(``dgamore/n_point_base.py:64-74``). This restatement keeps the reference's
+arithmetic structure with that cast removed, so it computes in complex128.
+Array layouts are the reference's C-order layouts (``FourPoint.mat``):
+
+* two-point objects   ``[nx, ny, nz, o1, o2, v]``
+* 4-point, irr. q     ``[q, o1, o2, o3, o4, w(>=0), v]`` (one fermionic axis)
+* local vertex        ``[o1, o2, o3, o4, w(>=0), v, v']``
+* compound index      rows ``(o1, o2, v)``, cols ``(o4, o3, v')``
+  (``dgamore/four_point.py:184-221``)
+
+Parity pins (see ``PARITY``): each function lists the check in ``tests/``
+that pins it against reference-held vectors, known answers, or mathematics.
+"""
+
+import numpy as np
+import scipy.fft as spfft
+import scipy.linalg as spla
+
+# -----------------------------------------------------------------------------
+# Parity status of each oracle function: name -> how it is pinned (tests/...).
+# -----------------------------------------------------------------------------
+PARITY = {
+    "kgrid_irr_maps": "known irreducible counts 6/45/91/153/325/561 + brute-force orbit check (test_oracle_grid.py)",
+    "g_full": "closed form: (iv+mu-ek-sigma) @ G == 1 (test_oracle_golden.py)",
+    "bubble_fft": "reference-held gchi0_q_rank_0.npy (test_oracle_golden.py) + brute-force k-sum bubble_direct",
+    "bubble_direct": "reference-held gchi0_q_rank_0.npy (test_oracle_golden.py)",
+    "block_invert_vn1": "reference-held gchi0_q_inv_rank_0.npy (test_oracle_golden.py)",
+    "ladder_sum_inv": "v1 == v3 == dense inverse of the explicitly built matrix, FP64 residual, beta scaling (test_oracle_ladder.py)",
+    "ladder_sum_solve": "same as ladder_sum_inv",
+    "vrg": "reference-held chain vrg_q -> gchi_aux_q_*_sum (test_oracle_golden.py)",
+    "chi_phys_shell": "reference-held gchi_aux_q_{dens,magn}_sum_rank_0.npy (test_oracle_golden.py)",
+    "kernel_r": "closed form for nb=1: K = u*(g - g*u*chi - [magn]2/3) (test_oracle_sigma.py)",
+    "dc_kernel": "brute-force loops on a tiny case (test_oracle_sigma.py)",
+    "sigma_fft": "equals the direct q-sum sigma_direct (nonlocal_sde.py:404-427) on small grids (test_oracle_sigma.py)",
+    "sigma_direct": "brute-force loop over (k,q,w,a,d) on a tiny grid (test_oracle_sigma.py)",
+    "find_lambda": "reference known answers lambda=-37.450340, 4.328781, 4.281153 + fixtures (test_oracle_golden.py)",
+    "pairing_matvec": "equals the explicit dense pairing operator built element by element (test_oracle_eliashberg.py)",
+}
+
+
+# -----------------------------------------------------------------------------
+# Matsubara grids  (dgamore/matsubara_frequencies.py:21-62)
+# -----------------------------------------------------------------------------
+def vn_int(niv):
+    """Fermionic integers in [-niv, niv) (matsubara_frequencies.py:42-52)."""
+    return np.arange(-niv, niv)
+
+
+def wn_int(niw, positive=False):
+    """Bosonic integers in [-niw, niw] or [0, niw] (matsubara_frequencies.py:21-30)."""
+    return np.arange(0, niw + 1) if positive else np.arange(-niw, niw + 1)
+
+
+def vn(niv, beta):
+    """nu_n = (2n+1) pi / beta (matsubara_frequencies.py:54-62)."""
+    return np.pi / beta * (2 * vn_int(niv) + 1)
+
+
+# -----------------------------------------------------------------------------
+# k-grid symmetry reduction  (dgamore/brillouin_zone.py:128-218, 349-363)
+# -----------------------------------------------------------------------------
+def _inv_sym(mat, axis):
+    """Inversion along one axis of a [0, 2pi) grid (brillouin_zone.py:128-142)."""
+    n = mat.shape[axis]
+    half, odd = n // 2, n % 2
+    src = [slice(None)] * mat.ndim
+    dst = [slice(None)] * mat.ndim
+    src[axis] = slice(1, half + odd)
+    dst[axis] = slice(half + 1, None)
+    mat[tuple(dst)] = np.flip(mat[tuple(src)], axis=axis)
+
+
+def _pair_sym(mat, a, b):
+    """Exchange symmetry of two axes (brillouin_zone.py:166-176)."""
+    if mat.shape[a] != mat.shape[b]:
+        return
+    merged = np.minimum(mat, mat.swapaxes(a, b))
+    mat[...] = np.minimum(merged, merged.swapaxes(a, b))
+
+
+SYMMETRY_SETS = {
+    # brillouin_zone.py:78-118
+    "two_dimensional_square": ("x_inv", "y_inv", "x_y_sym"),
+    "three_dimensional_cubic": ("x_inv", "y_inv", "z_inv", "x_y_sym", "x_z_sym", "y_z_sym"),
+    "two_dimensional_nematic": ("x_inv", "y_inv"),
+    "quasi_two_dimensional_square": ("x_inv", "y_inv", "z_inv", "x_y_sym"),
+    "none": (),
+}
+
+
+def kgrid_irr_maps(nk, symmetries="two_dimensional_square"):
+    """Irreducible-k maps of ``KGrid`` (brillouin_zone.py:349-363).
+
+    Returns a dict with irrk_ind, irrk_inv, irrk_count (np.unique of the
+    symmetrised fbz2irrk index grid), the full and irreducible index lists.
+    """
+    fbz = np.arange(int(np.prod(nk))).reshape(nk)
+    for s in SYMMETRY_SETS[symmetries] if isinstance(symmetries, str) else symmetries:
+        if s == "x_inv":
+            _inv_sym(fbz, 0)
+        elif s == "y_inv":
+            _inv_sym(fbz, 1)
+        elif s == "z_inv":
+            _inv_sym(fbz, 2)
+        elif s == "x_y_sym":
+            _pair_sym(fbz, 0, 1)
+        elif s == "x_z_sym":
+            _pair_sym(fbz, 0, 2)
+        elif s == "y_z_sym":
+            _pair_sym(fbz, 1, 2)
+        else:
+            raise ValueError(s)
+    _, ind, inv, cnt = np.unique(fbz, return_index=True, return_inverse=True, return_counts=True)
+    mesh = np.array(np.meshgrid(*[np.arange(n) for n in nk], indexing="ij")).reshape(3, -1)
+    return {
+        "irrk_ind": ind,
+        "irrk_inv": inv.ravel(),
+        "irrk_count": cnt,
+        "q_list": mesh.T.copy(),
+        "irrq_list": mesh[:, ind].T.copy(),
+    }
+
+
+# -----------------------------------------------------------------------------
+# One-particle objects
+# -----------------------------------------------------------------------------
+def g_full(sigma, mu, ek, beta):
+    """G(k,v) = [(iv + mu) 1 - e_k - Sigma(k,v)]^-1 (greens_function.py:125-142).
+
+    sigma: [nx,ny,nz,nb,nb,2niv] (or broadcastable [1,1,1,...]); ek: [nx,ny,nz,nb,nb].
+    """
+    nb = ek.shape[-1]
+    niv = sigma.shape[-1] // 2
+    eye = np.eye(nb)
+    iv = 1j * vn(niv, beta)
+    mat = (iv[None, None, :] * eye[..., None] + mu * eye[..., None])[None, None, None] - ek[..., None] - sigma
+    inv = np.linalg.inv(mat.transpose(0, 1, 2, 5, 3, 4))
+    return inv.transpose(0, 1, 2, 4, 5, 3)
+
+
+def read_hk(path):
+    """wannier90-style H(k) text file -> ([nk, nb, nb] complex, kpoints) (hamiltonian.py:234-318)."""
+    with open(path, encoding="utf-8") as f:
+        header = f.readline().split("#")[0].split()
+        nkp, nbands = int(header[0]), int(header[1])
+        data = np.array(f.read().split(), dtype=float)
+    data = data.reshape(-1, 3 + 2 * nbands * nbands)[:nkp]
+    hk = data[:, 3:].reshape(nkp, nbands, nbands, 2)
+    hk = hk[..., 0] + 1j * hk[..., 1]
+    # the reference transposes the wannier (row, col) order into its G convention
+    return hk.transpose(0, 2, 1), data[:, :3]
+
+
+def filter_small(x, threshold=1e-12):
+    """Zero entries whose |Re| and |Im| are both below threshold (n_point_base.py:205-211)."""
+    x = x.copy()
+    x[(np.abs(x.real) < threshold) & (np.abs(x.imag) < threshold)] = 0.0
+    return x
+
+
+def cut_niv_g(g, niv_cut):
+    """Symmetric fermionic cut of a two-point object (local_n_point.py:134-154)."""
+    niv = g.shape[-1] // 2
+    if niv_cut > niv:
+        return g
+    return g[..., niv - niv_cut: niv + niv_cut]
+
+
+# -----------------------------------------------------------------------------
+# Bubble  (dgamore/bubble_gen.py)
+# -----------------------------------------------------------------------------
+def bubble_fft(giwk, niw, niv, irrk_ind, beta, filter_values=True):
+    """chi0^{q w v}_{abcd} = -beta/N sum_k G_ad(k,v) G_cb(k-q,v-w), FFT form (bubble_gen.py:33-69).
+
+    giwk: [nx,ny,nz,nb,nb,2niv_g] with niv_g >= niv+niw. Returns [nq_irr,nb,nb,nb,nb,niw+1,2niv].
+    """
+    nk = giwk.shape[:3]
+    ntot = int(np.prod(nk))
+    nb = giwk.shape[3]
+    g_k = cut_niv_g(giwk, niv + niw)
+    g_r = spfft.fftn(g_k, axes=(0, 1, 2))
+    # G(-R): flip + roll(1) (n_point_base.py:553-574), then transpose orbitals
+    g_r_rev = np.roll(np.flip(g_r, axis=(0, 1, 2)), 1, axis=(0, 1, 2)).swapaxes(3, 4)
+    start, end = niw, niw + 2 * niv
+    g_v = g_r[..., start:end]
+    out = np.zeros((len(irrk_ind), nb, nb, nb, nb, niw + 1, 2 * niv), dtype=np.complex128)
+    for iw in range(niw + 1):
+        g_vw = g_r_rev[..., start - iw: end - iw]
+        buf = g_v[:, :, :, :, None, None, :, :] * g_vw[:, :, :, None, :, :, None, :]
+        out[..., iw, :] = np.fft.ifftn(buf, axes=(0, 1, 2)).reshape((ntot, nb, nb, nb, nb, 2 * niv))[irrk_ind]
+    out *= -beta / ntot
+    return filter_small(out) if filter_values else out
+
+
+def bubble_direct(giwk, niw, niv, q_list, beta, filter_values=True):
+    """Direct k-sum form of the bubble (bubble_gen.py:147-183). Oracle for bubble_fft."""
+    nk = giwk.shape[:3]
+    nb = giwk.shape[3]
+    g = cut_niv_g(giwk, niv + niw)
+    gniv = g.shape[-1] // 2
+    g_left = g[..., gniv - niv: gniv + niv]
+    out = np.zeros((len(q_list), nb, nb, nb, nb, niw + 1, 2 * niv), dtype=np.complex128)
+    idx = [np.arange(n) for n in nk]
+    for iq, q in enumerate(q_list):
+        g_r = g[(idx[0] - q[0]) % nk[0]][:, (idx[1] - q[1]) % nk[1]][:, :, (idx[2] - q[2]) % nk[2]]
+        for iw in range(niw + 1):
+            s = gniv - niv - iw
+            out[iq, ..., iw, :] = np.einsum("xyzadv,xyzcbv->abcdv", g_left, g_r[..., s: s + 2 * niv])
+    out *= -beta / int(np.prod(nk))
+    return filter_small(out) if filter_values else out
+
+
+def chi0_pp_w0(giwk, niv_pp):
+    """chi0pp_{abcd}(k,v) = G_ad(k,v) conj(G_cb(k,v)), no -beta (bubble_gen.py:268-279).
+
+    Returns [nk_tot, nb, nb, nb, nb, 2niv_pp].
+    """
+    g = cut_niv_g(giwk, niv_pp)
+    g = g.reshape((-1,) + g.shape[3:])
+    return filter_small(g[:, :, None, None, :, :] * np.conj(g.swapaxes(1, 2))[:, None, :, :, None, :])
+
+
+# -----------------------------------------------------------------------------
+# Compound-index helpers  (dgamore/four_point.py:184-293)
+# -----------------------------------------------------------------------------
+def block_invert_vn1(x):
+    """Per-(q,w,v) inverse of the nb^2 x nb^2 block [(o1,o2),(o4,o3)] (four_point.py:601-613).
+
+    x: [q,o1,o2,o3,o4,w,v] -> same shape.
+    """
+    nq, nb = x.shape[0], x.shape[1]
+    nw, nv = x.shape[5], x.shape[6]
+    m = x.transpose(0, 5, 6, 1, 2, 4, 3).reshape(nq, nw, nv, nb * nb, nb * nb)
+    m = np.linalg.inv(m)
+    return m.reshape(nq, nw, nv, nb, nb, nb, nb).transpose(0, 3, 4, 6, 5, 1, 2)
+
+
+def block_invert_vn0(x):
+    """Per-(q,w) inverse of the nb^2 x nb^2 block (four_point.py:207-209, 251-260, 615-618).
+
+    x: [q,o1,o2,o3,o4,w] -> same shape.
+    """
+    nq, nb, nw = x.shape[0], x.shape[1], x.shape[5]
+    m = x.transpose(0, 5, 1, 2, 4, 3).reshape(nq, nw, nb * nb, nb * nb)
+    m = np.linalg.inv(m)
+    return m.reshape(nq, nw, nb, nb, nb, nb).transpose(0, 2, 3, 5, 4, 1)
+
+
+def compound_local_vertex(gamma):
+    """Local vertex [o1,o2,o3,o4,w,v,v'] -> compound [w, n, n] (local_four_point.py:206-229)."""
+    nb, nw, nv = gamma.shape[0], gamma.shape[4], gamma.shape[5]
+    n = nb * nb * nv
+    return gamma.transpose(4, 0, 1, 5, 3, 2, 6).reshape(nw, n, n)
+
+
+# -----------------------------------------------------------------------------
+# Channel algebra  (dgamore/interaction.py:43-65, 160-182)
+# -----------------------------------------------------------------------------
+def u_as_channel(u, channel):
+    """Local U_abcd in a spin channel (interaction.py:43-65)."""
+    ut = np.einsum("abcd->adcb", u)
+    return {"dens": 2 * u - ut, "magn": -ut, "sing": u + ut, "trip": u - ut}[channel]
+
+
+def v_as_channel(v, channel):
+    """Non-local V^q_abcd in a spin channel (interaction.py:160-182)."""
+    return {"dens": 2 * v, "magn": 0 * v, "sing": v, "trip": v}[channel]
+
+
+# -----------------------------------------------------------------------------
+# Ladder  (dgamore/nonlocal_sde.py:85-156, dgamore/four_point.py:620-684)
+# -----------------------------------------------------------------------------
+def ladder_matrix(chi0_inv_q, gamma, u_r, v_r_q, beta):
+    """A(q,w) = chi0^-1(q,w,v) delta_vv' + (Gamma_r(w) - U_r - V_r^q)/beta^2 in compound form.
+
+    chi0_inv_q: [o1,o2,o3,o4,w,v] (one q); gamma: [o1,o2,o3,o4,w,v,v'];
+    u_r: [nb]*4; v_r_q: [nb]*4. Returns [w, n, n] (nonlocal_sde.py:93-95,
+    four_point.py:332-430 with local_n_point.py:163-175).
+    """
+    nb, nw, nv = gamma.shape[0], gamma.shape[4], gamma.shape[5]
+    full = gamma / beta**2 - ((u_r + v_r_q) / beta**2)[..., None, None, None]
+    eye = np.eye(nv)
+    full = full + chi0_inv_q[..., None] * eye
+    return compound_local_vertex(full)
+
+
+def ladder_sum_inv(chi0_inv, gamma, u_r, v_r, beta):
+    """sum_v' [A^-1] / beta via explicit inverse, v1 path (four_point.py:620-642).
+
+    chi0_inv: [q,o1..o4,w,v]; gamma: [o1..o4,w,v,v']; u_r: [nb]*4; v_r: [q,nb,nb,nb,nb].
+    Returns chi*_sum [q,o1,o2,o3,o4,w,v].
+    """
+    nq, nb, nw, nv = chi0_inv.shape[0], chi0_inv.shape[1], chi0_inv.shape[5], chi0_inv.shape[6]
+    out = np.empty_like(chi0_inv)
+    for iq in range(nq):
+        a = ladder_matrix(chi0_inv[iq], gamma, u_r, v_r[iq], beta)
+        inv = np.linalg.inv(a).reshape(nw, nb, nb, nv, nb, nb, nv)
+        out[iq] = inv.transpose(1, 2, 5, 4, 0, 3, 6).sum(axis=-1)
+    return out / beta
+
+
+def ladder_rhs(nb, nv):
+    """Indicator right-hand sides B[(o4,o3,v'), (o4,o3)] = 1 (four_point.py:661-668)."""
+    n = nb * nb * nv
+    idx = np.arange(n)
+    rhs = np.zeros((n, nb * nb))
+    rhs[idx, (idx // (nb * nv)) * nb + (idx // nv) % nb] = 1.0
+    return rhs
+
+
+def ladder_sum_solve(chi0_inv, gamma, u_r, v_r, beta):
+    """Same as ladder_sum_inv through LU solves with nb^2 indicator RHS (four_point.py:644-684)."""
+    nq, nb, nw, nv = chi0_inv.shape[0], chi0_inv.shape[1], chi0_inv.shape[5], chi0_inv.shape[6]
+    rhs = ladder_rhs(nb, nv)
+    out = np.empty_like(chi0_inv)
+    for iq in range(nq):
+        a = ladder_matrix(chi0_inv[iq], gamma, u_r, v_r[iq], beta)
+        x = np.linalg.solve(a, np.broadcast_to(rhs, (nw,) + rhs.shape))
+        out[iq] = x.reshape(nw, nb, nb, nv, nb, nb).transpose(1, 2, 5, 4, 0, 3)
+    return out / beta
+
+
+def vrg(chi0_inv, chi_aux_sum, beta):
+    """gamma_r = beta chi0^-1 . chi*_sum per (q,w,v) (nonlocal_sde.py:159-164, four_point.py:522-557)."""
+    return beta * np.einsum("qabcdwv,qdcefwv->qabefwv", chi0_inv, chi_aux_sum)
+
+
+def sum_over_vn(x, beta):
+    """(1/beta) sum over the last fermionic axis (four_point.py:114-144)."""
+    return x.sum(axis=-1) / beta
+
+
+def chi_phys_shell(chi_aux_sum_v, chi0_full_sum, chi0_core_sum, u_r, v_r):
+    """chi_r = ((chi*_sum + chi0_full - chi0_core)^-1 + U_r + V_r^q)^-1 (nonlocal_sde.py:167-182).
+
+    All inputs [q,o1..o4,w] except u_r [nb]*4, v_r [q,nb]*4.
+    """
+    x = block_invert_vn0(chi_aux_sum_v + chi0_full_sum - chi0_core_sum)
+    x = x + (u_r[None] + v_r)[..., None]
+    return block_invert_vn0(x)
+
+
+def identity_vn1(nq, nb, nw, nv):
+    """Compound identity delta_ad delta_bc (four_point.py:713-742)."""
+    e = np.einsum("ad,bc->abcd", np.eye(nb), np.eye(nb))
+    return np.broadcast_to(e[None, ..., None, None], (nq, nb, nb, nb, nb, nw, nv)).copy()
+
+
+def kernel_r(vrg_r, chi_phys, u_r, v_r, channel):
+    """K_r = u_r . (gamma_r - gamma_r . u_r . chi_r - [magn] 2/3 1) (nonlocal_sde.py:201-215)."""
+    ur = u_r[None] + v_r  # [q,nb,nb,nb,nb]
+    gu = np.einsum("qabijwv,qjief->qabefwv", vrg_r, ur)
+    kern = vrg_r - np.einsum("qabcdwv,qdcefw->qabefwv", gu, chi_phys)
+    if channel == "magn":
+        nq, nb, nw, nv = vrg_r.shape[0], vrg_r.shape[1], vrg_r.shape[5], vrg_r.shape[6]
+        kern = kern - 2.0 / 3.0 * identity_vn1(nq, nb, nw, nv)
+    return np.einsum("qabij,qjiefwv->qabefwv", ur, kern)
+
+
+def dc_kernel(f_magn_loc, chi0, u, beta, niv_core):
+    """Double-counting kernel K_dc (nonlocal_sde.py:185-198, 817-820 prefactor 2 and abcd->cbad).
+
+    f_magn_loc: [o1..o4,w,v,v'] on the chi0 fermionic box; chi0: [q,o1..o4,w,v].
+    Returns +K_dc [q,o1..o4,w,v'] cut to niv_core (the caller subtracts it).
+    """
+    f_dc = 2.0 * np.einsum("abcd->cbad", f_magn_loc)
+    up = np.einsum("abcd->adcb", u)
+    k0 = np.einsum("abij,qjiefwv->qabefwv", up, chi0) / beta**2
+    out = np.einsum("qabcdwv,dcefwvp->qabefwp", k0, f_dc)
+    niv = out.shape[-1] // 2
+    return out[..., niv - niv_core: niv + niv_core]
+
+
+# -----------------------------------------------------------------------------
+# Self-energy  (dgamore/nonlocal_sde.py:404-427, 544-594)
+# -----------------------------------------------------------------------------
+def to_full_niw(x, n_vn_axes):
+    """Half -> full bosonic range: F(-w) = conj(F(w)) with all frequency axes flipped (local_n_point.py:191-225)."""
+    w_axis = x.ndim - 1 - n_vn_axes
+    freq_axes = tuple(range(w_axis, x.ndim))
+    pos = x
+    neg = np.conj(np.flip(np.take(x, np.arange(1, x.shape[w_axis]), axis=w_axis), axis=freq_axes))
+    return np.concatenate([neg, pos], axis=w_axis)
+
+
+def map_irr_to_fbz(x_irr, irrk_inv):
+    """K_FBZ[k] = K_irr[irrk_inv[k]] (n_point_base.py:462-507 without orbital transform)."""
+    return x_irr[irrk_inv]
+
+
+def sigma_fft(kernel_irr, giwk, irrk_inv, beta, niv_core, niw):
+    """Sigma(k,v) via the R-space FFT form (nonlocal_sde.py:544-594, 874-879).
+
+    kernel_irr: [q_irr,o1..o4,w(>=0),2niv_core]; giwk: [nx,ny,nz,nb,nb,2niv_g].
+    Returns Sigma [nx,ny,nz,nb,nb,2niv_core] in the full fermionic range.
+    """
+    nk = giwk.shape[:3]
+    ntot = int(np.prod(nk))
+    nb = giwk.shape[3]
+    gniv = giwk.shape[-1] // 2
+    k_fbz = map_irr_to_fbz(kernel_irr, irrk_inv)
+    k_full = to_full_niw(k_fbz, 1)[..., niv_core:]  # [N, nb^4, 2niw+1, niv_core]
+    k_full = k_full.reshape(nk + k_full.shape[1:])
+    k_hat = np.conj(spfft.fftn(np.conj(k_full), axes=(0, 1, 2))).reshape((ntot,) + k_full.shape[3:])
+    g_r = spfft.fftn(giwk, axes=(0, 1, 2)).reshape(ntot, nb, nb, -1)
+    mat = np.zeros((ntot, nb, nb, niv_core), dtype=np.complex128)
+    for iw, w in enumerate(wn_int(niw)):
+        g_slice = g_r[..., gniv - w: gniv + niv_core - w]
+        mat += np.einsum("Radv,Raijdv->Rijv", g_slice, k_hat[..., iw, :])
+    mat *= -0.5 / beta / ntot
+    sig = spfft.ifftn(mat.reshape(nk + mat.shape[1:]), axes=(0, 1, 2))
+    return np.concatenate([np.conj(np.flip(sig, axis=-1)), sig], axis=-1)
+
+
+def sigma_direct(kernel_irr, giwk, irrk_inv, beta, niv_core, niw):
+    """Direct q-sum form Sigma = -1/(2 beta N) sum_{q,w} K(q,w,v) G(k+q,v-w) (nonlocal_sde.py:404-427)."""
+    nk = giwk.shape[:3]
+    ntot = int(np.prod(nk))
+    nb = giwk.shape[3]
+    gniv = giwk.shape[-1] // 2
+    k_full = to_full_niw(map_irr_to_fbz(kernel_irr, irrk_inv), 1)
+    q_list = np.array(np.meshgrid(*[np.arange(n) for n in nk], indexing="ij")).reshape(3, -1).T
+    mat = np.zeros(nk + (nb, nb, niv_core), dtype=np.complex128)
+    for iq, q in enumerate(q_list):
+        shifted = np.roll(giwk, [-i for i in q], axis=(0, 1, 2))
+        for iw, w in enumerate(wn_int(niw)):
+            g_qk = shifted[..., gniv - w: gniv + niv_core - w]
+            mat += np.einsum("aijdv,xyzadv->xyzijv", k_full[iq, ..., iw, niv_core:], g_qk)
+    mat *= -0.5 / beta / ntot
+    return np.concatenate([np.conj(np.flip(mat, axis=-1)), mat], axis=-1)
+
+
+# -----------------------------------------------------------------------------
+# Lambda correction, FP64  (dgamore/lambda_correction.py:13-73)
+# -----------------------------------------------------------------------------
+def find_lambda(chi_r_full, chi_loc_sum, irrk_count, beta, nk_tot, delta=0.1, eps=1e-7, maxiter=1000):
+    """Newton-like root find of sum_q w_q chi/(1+lambda chi)/(beta N) = chi_loc_sum (lambda_correction.py:28-60).
+
+    chi_r_full: [q_irr, 2niw+1] (full bosonic range, single band).
+    """
+    w0 = chi_r_full.shape[-1] // 2
+    lam_start = -np.min(1.0 / chi_r_full[..., w0].real)
+    lam = lam_start + delta
+    factor = 1.0 / beta / nk_tot
+    for _ in range(maxiter):
+        chi_lam = 1.0 / (1.0 / chi_r_full + lam)
+        f = (irrk_count[:, None] * chi_lam).sum() * factor - chi_loc_sum
+        fp = -(irrk_count[:, None] * chi_lam**2).sum() * factor
+        lam_new = lam - (f / fp).real
+        if abs(f.real) < eps:
+            return lam_new
+        if lam_new < lam:
+            delta /= 2
+            lam = lam_start + delta
+        else:
+            lam = lam_new
+    return lam
+
+
+def apply_lambda(chi, lam):
+    """chi_lambda = 1/(1/chi + lambda) (lambda_correction.py:21-25)."""
+    return 1.0 / (1.0 / chi + lam)
+
+
+# -----------------------------------------------------------------------------
+# Eliashberg pairing matvec  (dgamore/eliashberg_solver.py:376-428)
+# -----------------------------------------------------------------------------
+def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel):
+    """Gamma_hat = s fft_q Gamma_pp and its flipped copy (eliashberg_solver.py:390-398).
+
+    gamma_pp_irr: [q_irr, o1..o4, v, v'] (pp, w=0). Returns (g_hat, g_hat_flip, sign) with
+    arrays [nx,ny,nz,o1..o4,v,v'].
+    """
+    sign = 1 if channel == "sing" else -1
+    g = gamma_pp_irr[irrk_inv].reshape(tuple(nk) + gamma_pp_irr.shape[1:])
+    g_hat = sign * spfft.fftn(g, axes=(0, 1, 2))
+    flip = np.roll(np.flip(g_hat, axis=(0, 1, 2)), 1, axis=(0, 1, 2))
+    flip = np.flip(flip, axis=-1)
+    flip = np.einsum("...abcdvp->...adcbvp", flip)
+    return g_hat, flip, sign
+
+
+def pairing_matvec(g_hat, g_hat_flip, chi0pp, gap, sign, beta):
+    """One application of the linearised gap operator (eliashberg_solver.py:420-428).
+
+    chi0pp: [nx,ny,nz,o1..o4,v]; gap: [nx,ny,nz,o1,o2,v]. Returns the new gap, same shape.
+    """
+    nk = gap.shape[:3]
+    norm = 0.5 / int(np.prod(nk)) / beta
+    gg = np.fft.fftn(np.einsum("xyzabcdv,xyzcdv->xyzabv", chi0pp, gap), axes=(0, 1, 2))
+    gg_flip = np.roll(np.flip(gg, axis=(0, 1, 2)), 1, axis=(0, 1, 2))
+    new = np.einsum("xyzacbdvp,xyzcdp->xyzabv", g_hat, gg) + sign * np.einsum(
+        "xyzacbdvp,xyzcdp->xyzabv", g_hat_flip, gg_flip
+    )
+    return np.fft.ifftn(norm * new, axes=(0, 1, 2))
+
+
+def dense_solve_reference(a, b):
+    """LAPACK reference solve, used by tests as an independent check."""
+    return spla.solve(a, b)

@@ -1,0 +1,38 @@
+import os
+import sys
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+GOLDEN = os.path.join(ROOT, "tests", "golden")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU and the built HIP library")
+
+
+def golden(name):
+    """Load a reference-held fixture copied by tests/golden/make_golden.py (complex64 -> complex128)."""
+    a = np.load(os.path.join(GOLDEN, name), allow_pickle=False)
+    return a.astype(np.complex128) if np.iscomplexobj(a) else a
+
+
+@pytest.fixture(scope="session")
+def e2e_inputs():
+    """The reference's end_2_end model: 2 orbitals, 4x4x1, beta=12.5, niw=20, niv_core=20, niv_shell=10."""
+    from oracle import dga_oracle as O
+
+    hk, _ = O.read_hk(os.path.join(GOLDEN, "e2e_wannier.hk"))
+    ek = hk.reshape(4, 4, 1, 2, 2)
+    sigma = golden("e2e_sigma_dmft.npy")
+    mu = float(golden("e2e_mu_history.npy")[0])
+    beta = 12.5
+    giwk = O.g_full(sigma, mu, ek, beta)
+    u = np.zeros((2, 2, 2, 2))
+    u[0, 0, 0, 0] = u[1, 1, 1, 1] = 8.0  # test_local_sde_end_to_end.py:106-113
+    return {"ek": ek, "sigma": sigma, "mu": mu, "beta": beta, "giwk": giwk, "u": u,
+            "nk": (4, 4, 1), "niw": 20, "niv_core": 20, "niv_shell": 10}

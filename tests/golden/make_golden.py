@@ -1,0 +1,58 @@
+"""Copy the reference-held fixture files this project's tests use into tests/golden/.
+
+These are data files that the reference's own test suite holds under
+``tests/test_data`` (inputs and expected outputs of the reference's
+complex64 pipeline). Nothing of the reference's source is copied; the
+reference is not imported or executed.
+
+Run once in a container that has the reference mounted:
+
+    python tests/golden/make_golden.py /root/reference
+"""
+
+import os
+import shutil
+import sys
+
+FILES = {
+    # end_2_end: 2-orbital model, 4x4x1 k-grid, beta=12.5, niw=20, niv_core=20, niv_shell=10
+    "end_2_end/wannier.hk": "e2e_wannier.hk",
+    "end_2_end/sigma_dmft.npy": "e2e_sigma_dmft.npy",
+    "end_2_end/mu_history.npy": "e2e_mu_history.npy",
+    "end_2_end/gchi0_q_rank_0.npy": "e2e_gchi0_q.npy",
+    "end_2_end/gchi0_q_inv_rank_0.npy": "e2e_gchi0_q_inv.npy",
+    "end_2_end/vrg_q_dens_rank_0.npy": "e2e_vrg_q_dens.npy",
+    "end_2_end/vrg_q_magn_rank_0.npy": "e2e_vrg_q_magn.npy",
+    "end_2_end/gchi_aux_q_dens_sum_rank_0.npy": "e2e_chi_phys_q_dens.npy",
+    "end_2_end/gchi_aux_q_magn_sum_rank_0.npy": "e2e_chi_phys_q_magn.npy",
+    "end_2_end/giwk_dga.npy": "e2e_giwk_dga.npy",
+    "end_2_end/gamma_irrq_sing_pp.npy": "e2e_gamma_irrq_sing_pp.npy",
+    "end_2_end/gamma_irrq_trip_pp.npy": "e2e_gamma_irrq_trip_pp.npy",
+    # lambda correction: 1 band, 4x4x1, beta=12.5
+    "lambda_correction/sp/chi_dens_loc.npy": "lc_sp_chi_dens_loc.npy",
+    "lambda_correction/sp/chi_magn_loc.npy": "lc_sp_chi_magn_loc.npy",
+    "lambda_correction/sp/chi_phys_q_dens_before_lambda.npy": "lc_sp_chi_phys_q_dens_before.npy",
+    "lambda_correction/sp/chi_phys_q_magn_before_lambda.npy": "lc_sp_chi_phys_q_magn_before.npy",
+    "lambda_correction/sp/chi_phys_q_magn.npy": "lc_sp_chi_phys_q_magn.npy",
+    "lambda_correction/spch/chi_dens_loc.npy": "lc_spch_chi_dens_loc.npy",
+    "lambda_correction/spch/chi_magn_loc.npy": "lc_spch_chi_magn_loc.npy",
+    "lambda_correction/spch/chi_phys_q_dens_before_lambda.npy": "lc_spch_chi_phys_q_dens_before.npy",
+    "lambda_correction/spch/chi_phys_q_magn_before_lambda.npy": "lc_spch_chi_phys_q_magn_before.npy",
+    "lambda_correction/spch/chi_phys_q_dens.npy": "lc_spch_chi_phys_q_dens.npy",
+    "lambda_correction/spch/chi_phys_q_magn.npy": "lc_spch_chi_phys_q_magn.npy",
+    # SrVO3: realistic 3-orbital local irreducible vertex [3,3,3,3,11,20,20]
+    "srvo3_end2end/gamma_dens_loc.npy": "srvo3_gamma_dens_loc.npy",
+    "srvo3_end2end/gamma_magn_loc.npy": "srvo3_gamma_magn_loc.npy",
+}
+
+
+def main(ref_root):
+    src_root = os.path.join(ref_root, "tests", "test_data")
+    dst_root = os.path.dirname(os.path.abspath(__file__))
+    for src, dst in FILES.items():
+        shutil.copyfile(os.path.join(src_root, src), os.path.join(dst_root, dst))
+        print(f"{src} -> {dst}")
+
+
+if __name__ == "__main__":
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")

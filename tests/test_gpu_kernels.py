@@ -1,0 +1,243 @@
+"""Parity of every libdgami entry point with the CPU oracle (needs an MI355X).
+
+Each test prints the case it runs before touching the GPU. Tolerances: the
+plain path is complex128 end to end and is held to 1e-10 relative (max-abs
+error over max-abs value) against the oracle; agreement with the reference's
+own complex64 fixture files is held to 1e-6.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dga_oracle as O
+from tests.conftest import golden
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+RTOL = 1e-10
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+    return np.abs(a - b).max() / max(np.abs(b).max(), 1e-300)
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def K():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dgamore_amd import kernels
+
+    return kernels
+
+
+# ----------------------------------------------------------------------------- FFT
+@pytest.mark.parametrize("nk", [(16, 16, 1), (24, 24, 1), (4, 4, 4), (12, 20, 1), (64, 64, 1)])
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_fft_grid(K, nk, sign):
+    say(f"fft_grid nk={nk} sign={sign}")
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((5,) + nk) + 1j * rng.standard_normal((5,) + nk)
+    ref = np.fft.fftn(x, axes=(1, 2, 3)) if sign < 0 else np.fft.ifftn(x, axes=(1, 2, 3)) * np.prod(nk)
+    d = K.to_dev(x.reshape(5, -1))
+    out = K.fft_grid(d, nk, sign, 1.0)
+    torch.cuda.synchronize()
+    assert rel(out.reshape((5,) + nk), ref) < 1e-13
+
+
+# ----------------------------------------------------------------------------- bubble
+def test_bubble_e2e(K, e2e_inputs):
+    d = e2e_inputs
+    say("bubble end_2_end 4x4x1 nb=2 niw=20 niv=30")
+    maps = O.kgrid_irr_maps(d["nk"])
+    niv = d["niv_core"] + d["niv_shell"]
+    ref = O.bubble_fft(d["giwk"], d["niw"], niv, maps["irrk_ind"], d["beta"])
+    chi0 = K.bubble_irrq(K.to_dev(d["giwk"]), d["nk"], d["niw"], niv, d["beta"],
+                         K.to_dev(maps["irrk_ind"], torch.int32))
+    got = K.qf_to_ref4(chi0, 2)
+    torch.cuda.synchronize()
+    assert rel(got, ref) < RTOL
+    assert rel(got, golden("e2e_gchi0_q.npy")) < 1e-6
+
+
+def test_bubble_synthetic_1band(K):
+    from dgamore_amd import synthetic as S
+
+    say("bubble synthetic 1 band 16x16 niw=6 niv=8")
+    m = S.make_model(0, niw=6, niv=8, with_vertices=False)
+    g = m.giwk()
+    maps = O.kgrid_irr_maps(m.nk)
+    ref = O.bubble_fft(g, 6, 8, maps["irrk_ind"], m.beta)
+    chi0 = K.bubble_irrq(K.to_dev(g), m.nk, 6, 8, m.beta, K.to_dev(maps["irrk_ind"], torch.int32))
+    assert rel(K.qf_to_ref4(chi0, 1), ref) < RTOL
+
+
+# ----------------------------------------------------------------------------- chi0 prepare
+def test_chi0_prepare(K):
+    say("chi0_prepare end_2_end fixture nb=2")
+    chi0 = golden("e2e_gchi0_q.npy")
+    beta = 12.5
+    inv, full, core = K.chi0_prepare(K.ref4_to_qf(K.to_dev(chi0)), 2, 20, beta)
+    assert rel(K.qf_to_ref4(inv, 2), O.block_invert_vn1(chi0[..., 10:50])) < RTOL
+    assert rel(K.qw_to_ref4w(full, 2), chi0.sum(-1) / beta**2) < RTOL
+    assert rel(K.qw_to_ref4w(core, 2), chi0[..., 10:50].sum(-1) / beta**2) < RTOL
+
+
+# ----------------------------------------------------------------------------- ladder
+def _ladder_case(nb, niv, niw, nq, seed, gamma=None, beta=10.0, with_v=True):
+    """Random but well-conditioned ladder inputs in the reference layouts."""
+    rng = np.random.default_rng(seed)
+    nw = niw + 1
+    nv = 2 * niv
+    vv = O.vn(niv, beta)
+    # chi0 blocks ~ diagonal-dominant with frequency decay
+    base = np.einsum("ad,bc->abcd", np.eye(nb), np.eye(nb))
+    chi0 = np.empty((nq, nb, nb, nb, nb, nw, nv), dtype=complex)
+    for iq in range(nq):
+        for w in range(nw):
+            pert = 0.2 * (rng.standard_normal((nb,) * 4 + (nv,)) + 1j * rng.standard_normal((nb,) * 4 + (nv,)))
+            chi0[iq, ..., w, :] = (base[..., None] + pert) * (beta / (1.0 + vv**2))
+    chi0_inv = O.block_invert_vn1(chi0)
+    if gamma is None:
+        gamma = rng.standard_normal((nb,) * 4 + (nw, nv, nv)) + 1j * rng.standard_normal((nb,) * 4 + (nw, nv, nv))
+        gamma *= 2.0
+    u = rng.standard_normal((nb,) * 4)
+    v = 0.3 * (rng.standard_normal((nq,) + (nb,) * 4) + 1j * rng.standard_normal((nq,) + (nb,) * 4)) if with_v \
+        else np.zeros((nq,) + (nb,) * 4)
+    return chi0_inv, gamma, u, v, beta
+
+
+def _run_ladder(K, chi0_inv, gammas, us, vs, beta, precision=0):
+    nb = chi0_inv.shape[1]
+    g = torch.stack([K.vertex_to_cmp(K.to_dev(x)) for x in gammas])
+    u = torch.stack([K.orb_to_m(K.to_dev(x)) for x in us])
+    v = torch.stack([K.orb_to_m(K.to_dev(x)) for x in vs])
+    out, info = K.ladder_solve_sum(K.ref4_to_qf(K.to_dev(chi0_inv)), g, u, v, beta, precision)
+    return [K.qf_to_ref4(out[c], nb) for c in range(len(gammas))], info
+
+
+@pytest.mark.parametrize("nb,niv,niw,nq", [
+    (1, 20, 4, 7),    # n = 40 (register kernel)
+    (1, 60, 2, 3),    # n = 120 (register kernel, 1-band GPU-scale box)
+    (2, 8, 3, 5),     # n = 32, 4 right-hand sides
+    (2, 15, 2, 4),    # n = 60
+    (3, 4, 2, 3),     # n = 72, 9 right-hand sides
+    (2, 20, 2, 3),    # n = 160 (blocked LU path)
+    (3, 12, 1, 2),    # n = 216 (blocked)
+])
+def test_ladder_plain_fp64(K, nb, niv, niw, nq):
+    say(f"ladder fp64 nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=nb * 100 + niv)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta)
+    assert int(info.abs().sum()) == 0
+    assert rel(got[0], ref) < RTOL
+
+
+def test_ladder_two_channels_realistic_vertex(K):
+    """SrVO3's local irreducible vertices (reference fixture, 3 orbitals, n=180) with synthetic chi0."""
+    say("ladder fp64 srvo3 Gamma dens+magn nb=3 niv=10 n=180")
+    gd = golden("srvo3_gamma_dens_loc.npy")[:, :, :, :, :3]
+    gm = golden("srvo3_gamma_magn_loc.npy")[:, :, :, :, :3]
+    chi0_inv, _, u, v, beta = _ladder_case(3, 10, 2, 3, seed=7, gamma=gd)
+    ud, um = O.u_as_channel(u, "dens"), O.u_as_channel(u, "magn")
+    vd, vm = O.v_as_channel(v, "dens"), O.v_as_channel(v, "magn")
+    got, _ = _run_ladder(K, chi0_inv, [gd, gm], [ud, um], [vd, vm], beta)
+    assert rel(got[0], O.ladder_sum_solve(chi0_inv, gd, ud, vd, beta)) < RTOL
+    assert rel(got[1], O.ladder_sum_solve(chi0_inv, gm, um, vm, beta)) < RTOL
+
+
+def test_ladder_singular_raises(K):
+    say("ladder singular matrix -> LinAlgError")
+    nb, niv, nw, nq = 1, 4, 1, 2
+    chi0_inv = np.zeros((nq, 1, 1, 1, 1, nw, 2 * niv), dtype=complex)
+    gamma = np.zeros((1, 1, 1, 1, nw, 2 * niv, 2 * niv), dtype=complex)
+    u = np.ones((1, 1, 1, 1))
+    with pytest.raises(np.linalg.LinAlgError):
+        _run_ladder(K, chi0_inv, [gamma], [u], [np.zeros((nq, 1, 1, 1, 1))], 10.0)
+
+
+# ----------------------------------------------------------------------------- vrg / chi_phys / kernel
+@pytest.mark.parametrize("nb", [1, 2, 3])
+def test_vrg_chi_phys_kernel(K, nb):
+    say(f"vrg/chi_phys/kernel nb={nb}")
+    rng = np.random.default_rng(nb)
+    nq, nw, niv, beta = 5, 3, 4, 7.0
+    chi0_inv, _, u, v, _ = _ladder_case(nb, niv, nw - 1, nq, seed=nb)
+    aux = (rng.standard_normal(chi0_inv.shape) + 1j * rng.standard_normal(chi0_inv.shape)) * 0.3
+    full = rng.standard_normal(chi0_inv.shape[:-1]) + 1j * rng.standard_normal(chi0_inv.shape[:-1])
+    core = 0.5 * (rng.standard_normal(chi0_inv.shape[:-1]) + 1j * rng.standard_normal(chi0_inv.shape[:-1]))
+    for ch in ("dens", "magn"):
+        ur, vr = O.u_as_channel(u, ch), O.v_as_channel(v, ch)
+        vrg_ref = O.vrg(chi0_inv, aux, beta)
+        chi_ref = O.chi_phys_shell(O.sum_over_vn(aux, beta), full, core, ur, vr)
+        ker_ref = O.kernel_r(vrg_ref, chi_ref, ur, vr, ch)
+        d_inv, d_aux = K.ref4_to_qf(K.to_dev(chi0_inv)), K.ref4_to_qf(K.to_dev(aux))
+        d_full, d_core = K.ref4w_to_qw(K.to_dev(full)), K.ref4w_to_qw(K.to_dev(core))
+        d_u, d_v = K.orb_to_m(K.to_dev(ur)), K.orb_to_m(K.to_dev(vr))
+        vrg, chi = K.vrg_chi_phys(d_inv, d_aux, d_full, d_core, d_u, d_v, nb, beta)
+        assert rel(K.qf_to_ref4(vrg, nb), vrg_ref) < RTOL
+        assert rel(K.qw_to_ref4w(chi, nb), chi_ref) < RTOL
+        ker = K.kernel_r(vrg, chi, d_u, d_v, nb, ch, 3.0)
+        assert rel(K.qf_to_ref4(ker, nb), 3.0 * ker_ref) < RTOL
+
+
+# ----------------------------------------------------------------------------- DC kernel
+@pytest.mark.parametrize("nb,niv_core,niv_shell", [(1, 6, 0), (2, 4, 2), (3, 3, 0)])
+def test_dc_kernel(K, nb, niv_core, niv_shell):
+    say(f"dc_kernel nb={nb} niv_core={niv_core} niv_shell={niv_shell}")
+    rng = np.random.default_rng(11)
+    nq, nw, beta = 4, 3, 9.0
+    nvf = 2 * (niv_core + niv_shell)
+    chi0 = rng.standard_normal((nq,) + (nb,) * 4 + (nw, nvf)) + 1j * rng.standard_normal((nq,) + (nb,) * 4 + (nw, nvf))
+    f = rng.standard_normal((nb,) * 4 + (nw, nvf, nvf)) + 1j * rng.standard_normal((nb,) * 4 + (nw, nvf, nvf))
+    u = rng.standard_normal((nb,) * 4)
+    ref = -O.dc_kernel(f, chi0, u, beta, niv_core)
+    got = K.dc_kernel(K.ref4_to_qf(K.to_dev(chi0)), K.to_dev(u), K.to_dev(f), nb, niv_core, beta)
+    assert rel(K.qf_to_ref4(got, nb), ref) < RTOL
+
+
+# ----------------------------------------------------------------------------- Sigma
+@pytest.mark.parametrize("nb,nk,niw,niv", [(1, (16, 16, 1), 5, 6), (2, (4, 4, 1), 3, 4), (3, (6, 6, 1), 2, 3)])
+def test_sigma_from_kernel(K, nb, nk, niw, niv):
+    say(f"sigma nb={nb} nk={nk} niw={niw} niv={niv}")
+    rng = np.random.default_rng(5)
+    maps = O.kgrid_irr_maps(nk)
+    nq = len(maps["irrk_ind"])
+    kern = rng.standard_normal((nq,) + (nb,) * 4 + (niw + 1, 2 * niv)) \
+        + 1j * rng.standard_normal((nq,) + (nb,) * 4 + (niw + 1, 2 * niv))
+    niv_g = niw + niv + 3
+    g = rng.standard_normal(nk + (nb, nb, 2 * niv_g)) + 1j * rng.standard_normal(nk + (nb, nb, 2 * niv_g))
+    beta = 6.0
+    ref = O.sigma_fft(kern, g, maps["irrk_inv"], beta, niv, niw)
+    got = K.sigma_from_kernel(K.ref4_to_qf(K.to_dev(kern)), K.to_dev(maps["irrk_inv"], torch.int32),
+                              K.to_dev(g), nk, niw, beta)
+    assert rel(got, ref) < RTOL
+
+
+# ----------------------------------------------------------------------------- Eliashberg
+@pytest.mark.parametrize("channel", ["sing", "trip"])
+def test_pairing_matvec_e2e(K, channel):
+    say(f"pairing matvec end_2_end fixture {channel}")
+    gam = golden(f"e2e_gamma_irrq_{channel}_pp.npy")
+    giwk = golden("e2e_giwk_dga.npy")
+    nk, beta = (4, 4, 1), 12.5
+    maps = O.kgrid_irr_maps(nk)
+    np2 = gam.shape[-1]
+    chi0pp = O.chi0_pp_w0(giwk, np2 // 2)  # [N, nb^4 (abcd), np2]
+    ghat, gflip, s = O.pairing_setup(gam, maps["irrk_inv"], nk, channel)
+    rng = np.random.default_rng(3)
+    gaps = rng.standard_normal((3,) + nk + (2, 2, np2)) + 1j * rng.standard_normal((3,) + nk + (2, 2, np2))
+    chi0pp_k = chi0pp.reshape(nk + (2, 2, 2, 2, np2))
+    ref = np.stack([O.pairing_matvec(ghat, gflip, chi0pp_k, gp, s, beta) for gp in gaps])
+    d_hat, d_flip = K.pairing_setup(K.to_dev(gam), K.to_dev(maps["irrk_inv"], torch.int32), nk, channel)
+    got = K.pairing_matvec(d_hat, d_flip, K.to_dev(chi0pp.reshape(16, 16, np2)), K.to_dev(gaps), nk, channel, beta)
+    assert rel(got, ref) < RTOL
