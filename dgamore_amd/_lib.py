@@ -34,6 +34,9 @@ SIGNATURES = {
     "dgami_bubble_irrq": ([_i, _i, _i, _i, _i, _i, _i, _d, _c, _c, _i, _c, _c], _i),
     "dgami_chi0_prepare": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _c], _i),
     "dgami_ladder_solve_sum": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_solve_sum_dense": ([_i, _i, _i, _d, _c, _i, _c, _c, _c], _i),
+    "dgami_block_invert": ([_i, _i, _c, _c, _c, _c], _i),
+    "dgami_g_full": ([_i, _i, _i, _d, _d, _c, _c, _i, _i, _c, _c], _i),
     "dgami_vrg_chi_phys": ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c, _c, _c, _c, _c], _i),
     "dgami_kernel_r": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _i, _c, _c], _i),
     "dgami_dc_kernel": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _c], _i),

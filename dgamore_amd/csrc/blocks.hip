@@ -9,78 +9,9 @@
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
+#include "team.hpp"
 
 namespace dgami {
-
-__device__ __forceinline__ double2 shfl2(double2 v, int src, int width) {
-  return make_double2(__shfl(v.x, src, width), __shfl(v.y, src, width));
-}
-
-// In-place inverse of the team's block. Lane r (< T) holds row r of `m`;
-// on return it holds row r of m^-1. Returns true if a pivot was exactly zero.
-template <int T, int TS>
-__device__ __forceinline__ bool team_invert(double2 (&m)[T], int r) {
-  double2 e[T];
-#pragma unroll
-  for (int j = 0; j < T; ++j) e[j] = make_double2(r == j ? 1.0 : 0.0, 0.0);
-  bool singular = false;
-#pragma unroll
-  for (int k = 0; k < T; ++k) {
-    double val = (r >= k && r < T) ? cabs1(m[k]) : -1.0;
-    int idx = r;
-#pragma unroll
-    for (int off = TS / 2; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(val, off, TS);
-      const int oi = __shfl_xor(idx, off, TS);
-      if (ov > val || (ov == val && oi < idx)) {
-        val = ov;
-        idx = oi;
-      }
-    }
-    const int p = idx;
-    if (!(val > 0.0)) singular = true;
-    const int src = (r == k) ? p : ((r == p) ? k : r);
-#pragma unroll
-    for (int j = 0; j < T; ++j) {
-      m[j] = shfl2(m[j], src, TS);
-      e[j] = shfl2(e[j], src, TS);
-    }
-    const double2 piv = shfl2(m[k], k, TS);
-    const double2 pinv = (val > 0.0) ? cinv(piv) : make_double2(0.0, 0.0);
-    if (r == k) {
-#pragma unroll
-      for (int j = 0; j < T; ++j) {
-        m[j] = cmul(m[j], pinv);
-        e[j] = cmul(e[j], pinv);
-      }
-    }
-    const double2 f = m[k];
-#pragma unroll
-    for (int j = 0; j < T; ++j) {
-      const double2 mk = shfl2(m[j], k, TS);
-      const double2 ek = shfl2(e[j], k, TS);
-      if (r != k) {
-        m[j] = cfms(m[j], f, mk);
-        e[j] = cfms(e[j], f, ek);
-      }
-    }
-  }
-#pragma unroll
-  for (int j = 0; j < T; ++j) m[j] = e[j];
-  return singular;
-}
-
-// Row r of the team product A.B, lane r holding row r of both operands.
-template <int T, int TS>
-__device__ __forceinline__ void team_matmul(const double2 (&a)[T], const double2 (&b)[T], double2 (&out)[T]) {
-#pragma unroll
-  for (int j = 0; j < T; ++j) out[j] = make_double2(0.0, 0.0);
-#pragma unroll
-  for (int c = 0; c < T; ++c) {
-#pragma unroll
-    for (int j = 0; j < T; ++j) out[j] = cfma(out[j], a[c], shfl2(b[j], c, TS));
-  }
-}
 
 // ---------------------------------------------------------------------------
 // chi0: full/core frequency sums  (thread per (w, blk, q))
@@ -365,6 +296,103 @@ extern "C" int dgami_kernel_r(int nq, int nb, int nw, int niv, int channel, doub
 #define LAUNCH(T, TS)                                                                                               \
   hipLaunchKernelGGL((kernel_r_kernel<T, TS>), dim3(team_blocks<T, TS>(nteam)), dim3(256), 0, stream, vrg, chi, ur, \
                      vr, ker, nq, nw, nv, channel, weight, accumulate)
+  DGAMI_DISPATCH_NB(nb, LAUNCH)
+#undef LAUNCH
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------------------
+// G(k, v) = [(i v + mu) 1 - e_k - Sigma(k, v)]^-1  (team per (k, v), T = nb)
+// ---------------------------------------------------------------------------
+namespace dgami {
+template <int T, int TS>
+__global__ void __launch_bounds__(256) g_full_kernel(const double2* __restrict__ ek, const double2* __restrict__ sigma,
+                                                    double2* __restrict__ g, int nk, int nv, int sigma_nk, int s_nv,
+                                                    int s_v0, double mu, double beta) {
+  const int lane = threadIdx.x % TS;
+  const size_t team = (blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x) / TS;
+  const size_t nteam = static_cast<size_t>(nk) * nv;
+  const bool active = team < nteam;
+  const size_t tt = active ? team : 0;
+  const int v = static_cast<int>(tt % nv);
+  const int k = static_cast<int>(tt / nv);
+  const int r = lane < T ? lane : 0;
+  const double iv = M_PI / beta * (2 * (v - nv / 2) + 1);
+  const int ks = sigma_nk == 1 ? 0 : k;
+  double2 m[T];
+#pragma unroll
+  for (int c = 0; c < T; ++c) {
+    const double2 e = ek[(static_cast<size_t>(k) * T + r) * T + c];
+    const double2 sg = sigma[((static_cast<size_t>(ks) * T + r) * T + c) * s_nv + s_v0 + v];
+    double2 x = make_double2(-e.x - sg.x, -e.y - sg.y);
+    if (c == r) x = make_double2(x.x + mu, x.y + iv);
+    m[c] = x;
+  }
+  team_invert<T, TS>(m, lane);
+  if (active && lane < T) {
+#pragma unroll
+    for (int c = 0; c < T; ++c) g[((static_cast<size_t>(k) * T + r) * T + c) * nv + v] = m[c];
+  }
+}
+
+// generic block inverse: in/out [T*T][nmat] (element-major, matrix index fastest)
+template <int T, int TS>
+__global__ void __launch_bounds__(256) block_invert_kernel(const double2* __restrict__ in, double2* __restrict__ out,
+                                                          int32_t* info, int nmat) {
+  const int lane = threadIdx.x % TS;
+  const size_t team = (blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x) / TS;
+  const bool active = team < static_cast<size_t>(nmat);
+  const int q = active ? static_cast<int>(team) : 0;
+  const int r = lane < T ? lane : 0;
+  double2 m[T];
+#pragma unroll
+  for (int c = 0; c < T; ++c) m[c] = in[static_cast<size_t>(r * T + c) * nmat + q];
+  const bool sing = team_invert<T, TS>(m, lane);
+  if (active && lane < T) {
+#pragma unroll
+    for (int c = 0; c < T; ++c) out[static_cast<size_t>(r * T + c) * nmat + q] = m[c];
+    if (lane == 0 && info) info[q] = sing ? 1 : 0;
+  }
+}
+}  // namespace dgami
+
+extern "C" int dgami_g_full(int nk, int nb, int niv, double beta, double mu, const dgami_c128* ek_,
+                            const dgami_c128* sigma_, int sigma_nk, int sigma_niv, dgami_c128* g_, void* stream_) {
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  DGAMI_ARG_CHECK(ek_ && sigma_ && g_, "g_full: null pointer");
+  DGAMI_ARG_CHECK(nk >= 1 && nb >= 1 && nb <= 4 && niv >= 1 && beta > 0, "g_full: invalid sizes");
+  DGAMI_ARG_CHECK(sigma_nk == 1 || sigma_nk == nk, "g_full: sigma must be local (1 point) or on the k-grid");
+  DGAMI_ARG_CHECK(sigma_niv >= niv, "g_full: sigma box smaller than the requested G box");
+  const double2* ek = reinterpret_cast<const double2*>(ek_);
+  const double2* sg = reinterpret_cast<const double2*>(sigma_);
+  double2* g = reinterpret_cast<double2*>(g_);
+  const int nv = 2 * niv;
+  const size_t nteam = static_cast<size_t>(nk) * nv;
+#define LAUNCH(T, TS)                                                                                             \
+  hipLaunchKernelGGL((g_full_kernel<T, TS>), dim3(team_blocks<T, TS>(nteam)), dim3(256), 0, stream, ek, sg, g, nk, \
+                     nv, sigma_nk, 2 * sigma_niv, sigma_niv - niv, mu, beta)
+  switch (nb) {
+    case 1: LAUNCH(1, 1); break;
+    case 2: LAUNCH(2, 2); break;
+    case 3: LAUNCH(3, 4); break;
+    default: LAUNCH(4, 4); break;
+  }
+#undef LAUNCH
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+extern "C" int dgami_block_invert(int nmat, int nb, const dgami_c128* in_, dgami_c128* out_, int32_t* info,
+                                  void* stream_) {
+  hipStream_t stream = static_cast<hipStream_t>(stream_);
+  DGAMI_ARG_CHECK(in_ && out_, "block_invert: null pointer");
+  DGAMI_ARG_CHECK(nmat >= 1, "block_invert: nmat must be positive");
+  const double2* in = reinterpret_cast<const double2*>(in_);
+  double2* out = reinterpret_cast<double2*>(out_);
+#define LAUNCH(T, TS)                                                                                        \
+  hipLaunchKernelGGL((block_invert_kernel<T, TS>), dim3(team_blocks<T, TS>(nmat)), dim3(256), 0, stream, in, \
+                     out, info, nmat)
   DGAMI_DISPATCH_NB(nb, LAUNCH)
 #undef LAUNCH
   DGAMI_HIP_CHECK(hipGetLastError());

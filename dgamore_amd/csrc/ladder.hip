@@ -45,6 +45,36 @@ extern "C" int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv
   a.n = a.T * a.nv;
   a.inv_beta2 = 1.0 / (beta * beta);
   a.inv_beta = 1.0 / beta;
+  a.dense = 0;
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  if (precision == DGAMI_PLAIN_FP64) return ladder_solve_fp64(a, s);
+  return ladder_solve_mixed(a, precision, s);
+}
+
+extern "C" int dgami_solve_sum_dense(int nmat, int nb, int niv, double beta, const dgami_c128* a_cmp, int precision,
+                                     dgami_c128* out, int32_t* info, void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(a_cmp && out, "solve_sum_dense: null pointer");
+  DGAMI_ARG_CHECK(nmat >= 1 && nb >= 1 && nb <= 4 && niv >= 1 && beta > 0, "solve_sum_dense: invalid sizes");
+  DGAMI_ARG_CHECK(precision == DGAMI_PLAIN_FP64 || precision == DGAMI_MIXED_FP32_IR ||
+                      precision == DGAMI_MIXED_BF16X3_IR,
+                  "solve_sum_dense: unknown precision mode");
+  LadderArgs a;
+  a.chi0_inv = nullptr;
+  a.gamma = reinterpret_cast<const double2*>(a_cmp);
+  a.u_r = nullptr;
+  a.v_r = nullptr;
+  a.out = reinterpret_cast<double2*>(out);
+  a.info = info;
+  a.nchan = 1;
+  a.nq = nmat;
+  a.nw = 1;
+  a.T = nb * nb;
+  a.nv = 2 * niv;
+  a.n = a.T * a.nv;
+  a.inv_beta2 = 1.0;
+  a.inv_beta = 1.0 / beta;
+  a.dense = 1;
   hipStream_t s = static_cast<hipStream_t>(stream);
   if (precision == DGAMI_PLAIN_FP64) return ladder_solve_fp64(a, s);
   return ladder_solve_mixed(a, precision, s);

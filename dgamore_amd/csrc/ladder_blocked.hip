@@ -39,9 +39,9 @@ __global__ void blk_assemble_kernel(LadderArgs a, BlockedState st, int item0, in
     const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
     double2 val = make_double2(0.0, 0.0);
     if (j < st.n) {
-      const double2* G = a.gamma + (static_cast<size_t>(chan) * a.nw + w) * st.n * st.n;
-      const double2* U = a.u_r + chan * a.T * a.T;
-      const double2* V = a.v_r ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
+      const double2* G = ladder_matrix_ptr(a, chan, w, q);
+      const double2* U = a.dense ? nullptr : a.u_r + chan * a.T * a.T;
+      const double2* V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
       val = ladder_elem(a, G, U, V, w, q, i, j);
     } else if (j < st.ncol) {
       val = ladder_rhs(a, i, j - st.n);

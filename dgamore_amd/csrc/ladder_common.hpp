@@ -14,6 +14,7 @@ struct LadderArgs {
   double2* out;             // [nchan][nw][T*T][nv][nq]
   int32_t* info;            // [nchan][nw][nq]
   int nchan, nq, nw;
+  int dense;  // 1: gamma holds the assembled matrix of every item ([item][n][n]), out is [item][n][T]
   int T;     // nb^2 (number of right-hand sides)
   int nv;    // 2 * niv
   int n;     // T * nv
@@ -25,6 +26,7 @@ struct LadderArgs {
 __device__ __forceinline__ double2 ladder_elem(const LadderArgs& a, const double2* __restrict__ G,
                                                const double2* __restrict__ U, const double2* __restrict__ V, int w,
                                                int q, int i, int j) {
+  if (a.dense) return G[static_cast<size_t>(i) * a.n + j];
   const int nv = a.nv, T = a.T;
   const int ri = i / nv, vi = i - ri * nv;
   const int cj = j / nv, vj = j - cj * nv;
@@ -42,8 +44,16 @@ __device__ __forceinline__ double2 ladder_rhs(const LadderArgs& a, int i, int r)
 
 // Output element chi_aux_sum[chan][w][(ri, r)][vi][q] = x[i][r] / beta.
 __device__ __forceinline__ size_t ladder_out_index(const LadderArgs& a, int chan, int w, int q, int i, int r) {
+  if (a.dense) return ((static_cast<size_t>(chan) * a.nw + w) * a.nq + q) * a.n * a.T + static_cast<size_t>(i) * a.T + r;
   const int ri = i / a.nv, vi = i - ri * a.nv;
   return ((((static_cast<size_t>(chan) * a.nw + w) * a.T * a.T + ri * a.T + r) * a.nv) + vi) * a.nq + q;
+}
+
+// Matrix of one item: the shared vertex Gamma(chan, w), or the item's own matrix in dense mode.
+__device__ __forceinline__ const double2* ladder_matrix_ptr(const LadderArgs& a, int chan, int w, int q) {
+  const size_t nn = static_cast<size_t>(a.n) * a.n;
+  if (a.dense) return a.gamma + ((static_cast<size_t>(chan) * a.nw + w) * a.nq + q) * nn;
+  return a.gamma + (static_cast<size_t>(chan) * a.nw + w) * nn;
 }
 
 int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream);

@@ -60,9 +60,9 @@ __global__ void __launch_bounds__(TR* TC) ladder_reg_kernel(LadderArgs a) {
   double2* s_y = s_dyn;
   double2* s_uc = s_dyn + n * T;
 
-  const double2* G = a.gamma + (static_cast<size_t>(chan) * a.nw + w) * n * n;
-  const double2* U = a.u_r + chan * T * T;
-  const double2* V = a.v_r ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * T * T : nullptr;
+  const double2* G = ladder_matrix_ptr(a, chan, w, q);
+  const double2* U = a.dense ? nullptr : a.u_r + chan * T * T;
+  const double2* V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * T * T : nullptr;
 
   double2 A[RPT][CPT];
 #pragma unroll

@@ -1,0 +1,99 @@
+"""Device-resident non-local SDE iteration on one GPU.
+
+One call of SDEPipeline.step runs the hot loop of calculate_self_energy_q
+(nonlocal_sde.py:777-890) from the self-energy to the new self-energy, with
+every intermediate kept in HBM in the library's q-fastest layouts:
+
+    G(k)            dgami_g_full            (greens_function.py:125-142)
+    chi0(q_irr)     dgami_bubble_irrq       (bubble_gen.py:33-69)
+    chi0^-1, sums   dgami_chi0_prepare      (nonlocal_sde.py:824-835)
+    kernel = -K_dc  dgami_dc_kernel         (nonlocal_sde.py:185-198, 817-820)
+    chi*_sum        dgami_ladder_solve_sum  (both channels in one launch; nonlocal_sde.py:85-156)
+    vrg, chi_r      dgami_vrg_chi_phys      (nonlocal_sde.py:159-182)
+    kernel += K_d + 3 K_m   dgami_kernel_r  (nonlocal_sde.py:201-215, 838-857)
+    Sigma(k)        dgami_sigma_from_kernel (nonlocal_sde.py:544-594, 874-879)
+
+The MPI distribution of the reference (irreducible q over ranks) becomes the
+(channel, w, q) work list of a single ladder launch.
+"""
+
+import numpy as np
+import torch
+
+from dgamore_amd import _lib
+from dgamore_amd import kernels as K
+from dgamore_amd.brillouin_zone import KGrid
+
+
+def u_as_channel(u, channel):
+    """Local interaction in a spin channel (interaction.py:43-65)."""
+    ut = np.einsum("abcd->adcb", u)
+    return {"dens": 2 * u - ut, "magn": -ut, "sing": u + ut, "trip": u - ut}[channel]
+
+
+class SDEPipeline:
+    """Inputs of one model resident on the device; step() runs one SDE iteration."""
+
+    CHANNELS = (("dens", 1.0), ("magn", 3.0))
+
+    def __init__(self, model, precision=_lib.PLAIN_FP64, symmetries="two_dimensional_square"):
+        self.m = model
+        self.precision = precision
+        self.grid = KGrid(model.nk, symmetries)
+        self.nq = self.grid.nk_irr
+        self.nk = model.nk
+        self.N = int(np.prod(model.nk))
+        nb = model.nb
+        self.irrk_ind = K.to_dev(self.grid.irrk_ind, torch.int32)
+        self.irrk_inv = K.to_dev(self.grid.irrk_inv, torch.int32)
+        self.ek = K.to_dev(model.ek.reshape(self.N, nb, nb))
+        self.gamma = torch.stack([K.vertex_to_cmp(K.to_dev(model.gamma[ch])) for ch, _ in self.CHANNELS]).contiguous()
+        self.u_r = torch.stack([K.orb_to_m(K.to_dev(u_as_channel(model.u, ch))) for ch, _ in self.CHANNELS])
+        self.u_ref = K.to_dev(model.u)
+        self.f_magn = K.to_dev(model.f_magn)
+        self.sigma_dmft = K.to_dev(model.sigma_loc[None])  # local Sigma as [1][nb][nb][2 niv_s]
+        self.niv_s = model.sigma_loc.shape[-1] // 2
+        self.last = {}
+
+    # -- pieces -------------------------------------------------------------------------------
+    def g_full(self, sigma, mu, niv):
+        """sigma: [1 or N][nb][nb][2 niv_sigma] device tensor."""
+        nb = self.m.nb
+        nks = sigma.shape[0]
+        g = torch.empty((self.N, nb, nb, 2 * niv), dtype=K.C128, device=sigma.device)
+        _lib.call("dgami_g_full", self.N, nb, niv, float(self.m.beta), float(mu), K._p(self.ek), K._p(sigma), nks,
+                  sigma.shape[-1] // 2, K._p(g), K._stream())
+        return g
+
+    def step(self, sigma=None, mu=None, keep=False):
+        """One non-local SDE iteration; returns Sigma(k, v) [nx,ny,nz,nb,nb,2 niv_core] on the device."""
+        m = self.m
+        nb, beta = m.nb, m.beta
+        sigma = self.sigma_dmft if sigma is None else sigma
+        mu = m.mu if mu is None else mu
+        g = self.g_full(sigma, mu, m.niv_g).reshape(m.nk + (nb, nb, 2 * m.niv_g))
+        chi0 = K.bubble_irrq(g, m.nk, m.niw, m.niv_full, beta, self.irrk_ind)
+        inv, full, core = K.chi0_prepare(chi0, nb, m.niv, beta)
+        kern = K.dc_kernel(chi0, self.u_ref, self.f_magn, nb, m.niv, beta)
+        aux, info = K.ladder_solve_sum(inv, self.gamma, self.u_r, None, beta, self.precision, check_info=False)
+        vrgs, chis = [], []
+        for c, (ch, weight) in enumerate(self.CHANNELS):
+            vrg, chi = K.vrg_chi_phys(inv, aux[c], full, core, self.u_r[c], None, nb, beta)
+            K.kernel_r(vrg, chi, self.u_r[c], None, nb, ch, weight, kernel=kern, accumulate=True)
+            vrgs.append(vrg)
+            chis.append(chi)
+        sig = K.sigma_from_kernel(kern, self.irrk_inv, g, m.nk, m.niw, beta)
+        if keep:
+            self.last = dict(g=g, chi0=chi0, chi0_inv=inv, full=full, core=core, aux=aux, info=info, vrg=vrgs,
+                             chi_phys=chis, kernel=kern, sigma=sig)
+        return sig
+
+    # -- accounting ---------------------------------------------------------------------------
+    def solves_per_step(self):
+        """(q, w, channel) ladder solves in one iteration: 2 channels x nq_irr x (niw+1)."""
+        return len(self.CHANNELS) * self.nq * (self.m.niw + 1)
+
+    def ladder_flops_per_solve(self):
+        """Algorithmic flops of one solve: complex LU (8/3 n^3) + nb^2 triangular RHS pairs (8 nb^2 n^2)."""
+        n = self.m.nb**2 * 2 * self.m.niv
+        return 8.0 / 3.0 * n**3 + 8.0 * self.m.nb**2 * n**2

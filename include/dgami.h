@@ -100,6 +100,27 @@ int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double be
                            const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r, int precision,
                            dgami_c128* chi_aux_sum, int32_t* info, void* stream);
 
+/* Dense-matrix variant of the ladder solve: a_cmp holds nmat assembled compound matrices
+ * [nmat][n][n] (n = nb^2 * 2*niv); out[nmat][n][nb^2] = (1/beta) sum_v' [A^-1]_{(o1,o2,v),(o4,o3,v')}
+ * at row (o1,o2,v), column (o4,o3). info: [nmat].
+ * Replaces FourPoint.invert_and_sum_over_last_vn / _v2 applied to an assembled object
+ * (four_point.py:620-684). */
+int dgami_solve_sum_dense(int nmat, int nb, int niv, double beta, const dgami_c128* a_cmp, int precision,
+                          dgami_c128* out, int32_t* info, void* stream);
+
+/* Batched inverse of nmat nb^2 x nb^2 blocks stored element-major: in/out [nb^4 (M order)][nmat].
+ * info: [nmat] (1 where a pivot was exactly zero) or NULL.
+ * Replaces FourPoint.invert for objects without fermionic axes (four_point.py:207-209, 615-618)
+ * and LocalFourPoint.invert of such objects. */
+int dgami_block_invert(int nmat, int nb, const dgami_c128* in, dgami_c128* out, int32_t* info, void* stream);
+
+/* Lattice Green's function G(k, v) = [(i v + mu) 1 - e_k - Sigma(k, v)]^-1 on the box
+ * v in [-niv, niv). ek: [nk][nb][nb]; sigma: REF2 with sigma_nk = 1 (local) or nk points
+ * and 2*sigma_niv >= 2*niv frequencies (centred cut); g: REF2 [nk][nb][nb][2*niv].
+ * Replaces GreensFunction.get_g_full (greens_function.py:125-142). */
+int dgami_g_full(int nk, int nb, int niv, double beta, double mu, const dgami_c128* ek, const dgami_c128* sigma,
+                 int sigma_nk, int sigma_niv, dgami_c128* g, void* stream);
+
 /* Three-leg vertex and shell-corrected physical susceptibility of one channel:
  *   vrg      = beta chi0_inv . chi_aux_sum                              (QF)
  *   chi_phys = ((sum_v chi_aux_sum / beta + chi0_full_sum - chi0_core_sum)^-1 + u_r + v_r)^-1  (QW)

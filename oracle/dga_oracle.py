@@ -369,7 +369,7 @@ def dc_kernel(f_magn_loc, chi0, u, beta, niv_core):
     f_magn_loc: [o1..o4,w,v,v'] on the chi0 fermionic box; chi0: [q,o1..o4,w,v].
     Returns +K_dc [q,o1..o4,w,v'] cut to niv_core (the caller subtracts it).
     """
-    f_dc = 2.0 * np.einsum("abcd->cbad", f_magn_loc)
+    f_dc = 2.0 * np.einsum("abcd...->cbad...", f_magn_loc)
     up = np.einsum("abcd->adcb", u)
     k0 = np.einsum("abij,qjiefwv->qabefwv", up, chi0) / beta**2
     out = np.einsum("qabcdwv,dcefwvp->qabefwp", k0, f_dc)

@@ -241,3 +241,87 @@ def test_pairing_matvec_e2e(K, channel):
     d_hat, d_flip = K.pairing_setup(K.to_dev(gam), K.to_dev(maps["irrk_inv"], torch.int32), nk, channel)
     got = K.pairing_matvec(d_hat, d_flip, K.to_dev(chi0pp.reshape(16, 16, np2)), K.to_dev(gaps), nk, channel, beta)
     assert rel(got, ref) < RTOL
+
+
+# ----------------------------------------------------------------------------- small helpers
+def test_g_full(K):
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import SDEPipeline
+
+    say("g_full 3-orbital 6x6 local sigma")
+    m = S.make_model(2, nk=(6, 6, 1), niw=2, niv=4, with_vertices=False)
+    ek = K.to_dev(m.ek.reshape(36, 3, 3))
+    sig = K.to_dev(m.sigma_loc[None])
+    niv = 9
+    g = torch.empty((36, 3, 3, 2 * niv), dtype=torch.complex128, device=ek.device)
+    from dgamore_amd import _lib
+    _lib.call("dgami_g_full", 36, 3, niv, float(m.beta), float(m.mu), K._p(ek), K._p(sig), 1,
+              m.sigma_loc.shape[-1] // 2, K._p(g), K._stream())
+    ns = m.sigma_loc.shape[-1] // 2
+    ref = O.g_full(m.sigma_loc[None, None, None, ..., ns - niv: ns + niv], m.mu, m.ek, m.beta)
+    assert rel(g.reshape(ref.shape), ref) < RTOL
+
+
+@pytest.mark.parametrize("nb", [1, 2, 3])
+def test_block_invert(K, nb):
+    from dgamore_amd import _lib
+
+    say(f"block_invert nb={nb}")
+    rng = np.random.default_rng(nb)
+    x = rng.standard_normal((7,) + (nb,) * 4 + (3,)) + 1j * rng.standard_normal((7,) + (nb,) * 4 + (3,))
+    x += 3 * np.einsum("ad,bc->abcd", np.eye(nb), np.eye(nb))[None, ..., None]
+    d = K.ref4w_to_qw(K.to_dev(x))  # [w][nb^4][q] == element-major over nmat = (w, q)
+    flat = d.permute(1, 0, 2).reshape(nb**4, -1).contiguous()
+    out = torch.empty_like(flat)
+    _lib.call("dgami_block_invert", flat.shape[1], nb, K._p(flat), K._p(out), None, K._stream())
+    got = out.reshape(nb**4, 3, 7).permute(1, 0, 2).contiguous()
+    assert rel(K.qw_to_ref4w(got, nb), O.block_invert_vn0(x)) < RTOL
+
+
+@pytest.mark.parametrize("nb,niv", [(1, 20), (2, 12), (3, 10)])
+def test_solve_sum_dense(K, nb, niv):
+    from dgamore_amd import _lib
+
+    say(f"solve_sum_dense nb={nb} niv={niv}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, 1, 3, seed=3)
+    mats = np.stack([O.ladder_matrix(chi0_inv[q], gamma, u, v[q], beta)[w] for q in range(3) for w in range(2)])
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)  # [q,o1,o2,o3,o4,w,v]
+    n = mats.shape[-1]
+    d = K.to_dev(mats)
+    out = torch.empty((6, n, nb * nb), dtype=torch.complex128, device=d.device)
+    info = torch.zeros(6, dtype=torch.int32, device=d.device)
+    _lib.call("dgami_solve_sum_dense", 6, nb, niv, float(beta), K._p(d), 0, K._p(out), K._p(info, torch.int32),
+              K._stream())
+    got = out.reshape(3, 2, nb, nb, 2 * niv, nb, nb).cpu().numpy()  # [q,w,o1,o2,v,o4,o3]
+    got = got.transpose(0, 2, 3, 6, 5, 1, 4)
+    assert rel(got, ref) < RTOL
+
+
+# ----------------------------------------------------------------------------- whole SDE iteration
+@pytest.mark.parametrize("cfg,nk,niw,niv", [(0, (8, 8, 1), 4, 6), (2, (6, 6, 1), 2, 4)])
+def test_pipeline_step_matches_oracle(K, cfg, nk, niw, niv):
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import SDEPipeline, u_as_channel
+
+    say(f"pipeline step config {cfg} nk={nk} niw={niw} niv={niv}")
+    m = S.make_model(cfg, nk=nk, niw=niw, niv=niv)
+    pipe = SDEPipeline(m)
+    sig = pipe.step(keep=True)
+    torch.cuda.synchronize()
+    g = m.giwk()
+    maps = O.kgrid_irr_maps(m.nk)
+    nq = len(maps["irrk_ind"])
+    chi0 = O.bubble_fft(g, m.niw, m.niv, maps["irrk_ind"], m.beta)
+    inv = O.block_invert_vn1(chi0)
+    kern = -O.dc_kernel(m.f_magn, chi0, m.u, m.beta, m.niv)
+    zero_v = np.zeros((nq,) + (m.nb,) * 4)
+    for ch, wgt in (("dens", 1.0), ("magn", 3.0)):
+        ur = u_as_channel(m.u, ch)
+        aux = O.ladder_sum_solve(inv, m.gamma[ch], ur, zero_v, m.beta)
+        vrg = O.vrg(inv, aux, m.beta)
+        chi = O.chi_phys_shell(O.sum_over_vn(aux, m.beta), chi0.sum(-1) / m.beta**2, chi0.sum(-1) / m.beta**2,
+                               ur, zero_v)
+        kern = kern + wgt * O.kernel_r(vrg, chi, ur, zero_v, ch)
+    ref = O.sigma_fft(kern, g, maps["irrk_inv"], m.beta, m.niv, m.niw)
+    assert rel(pipe.last["kernel"], K.ref4_to_qf(K.to_dev(kern)).cpu().numpy()) < RTOL
+    assert rel(sig, ref) < RTOL

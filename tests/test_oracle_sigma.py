@@ -1,0 +1,96 @@
+"""Pins of the oracle's kernel, double-counting and self-energy functions (CPU only)."""
+import numpy as np
+import pytest
+
+from oracle import dga_oracle as O
+
+
+def test_dc_kernel_brute_force():
+    """K_dc[q,a,b,e,f,w,p] = sum_{cdv} (1/b^2) sum_ij u_{ajib} chi0_{jicd}(q,w,v) 2 F_{ecdf}(w,v,p)."""
+    rng = np.random.default_rng(0)
+    nb, nq, nw, nv, beta, niv_core = 2, 2, 2, 4, 3.0, 1
+    chi0 = rng.standard_normal((nq, nb, nb, nb, nb, nw, nv)) + 1j * rng.standard_normal((nq, nb, nb, nb, nb, nw, nv))
+    f = rng.standard_normal((nb,) * 4 + (nw, nv, nv)) + 1j * rng.standard_normal((nb,) * 4 + (nw, nv, nv))
+    u = rng.standard_normal((nb,) * 4)
+    got = O.dc_kernel(f, chi0, u, beta, niv_core)
+    ref = np.zeros((nq, nb, nb, nb, nb, nw, nv), dtype=complex)
+    for q in range(nq):
+        for a in range(nb):
+            for b in range(nb):
+                for e in range(nb):
+                    for ff in range(nb):
+                        for w in range(nw):
+                            for p in range(nv):
+                                acc = 0
+                                for c in range(nb):
+                                    for d in range(nb):
+                                        for v in range(nv):
+                                            k0 = sum(u[a, j, i, b] * chi0[q, j, i, c, d, w, v]
+                                                     for i in range(nb) for j in range(nb)) / beta**2
+                                            acc += k0 * 2 * f[e, c, d, ff, w, v, p]
+                                ref[q, a, b, e, ff, w, p] = acc
+    assert np.abs(got - ref[..., nv // 2 - niv_core: nv // 2 + niv_core]).max() < 1e-12 * np.abs(ref).max()
+
+
+def test_kernel_r_single_band_closed_form():
+    """nb = 1: K = u (g - g u chi - [magn] 2/3) with u = U_r + V_r(q)."""
+    rng = np.random.default_rng(1)
+    nq, nw, nv = 3, 2, 4
+    g = rng.standard_normal((nq, 1, 1, 1, 1, nw, nv)) + 1j * rng.standard_normal((nq, 1, 1, 1, 1, nw, nv))
+    chi = rng.standard_normal((nq, 1, 1, 1, 1, nw)) + 1j * rng.standard_normal((nq, 1, 1, 1, 1, nw))
+    u = np.full((1, 1, 1, 1), 2.5)
+    v = rng.standard_normal((nq, 1, 1, 1, 1))
+    for ch, shift in (("dens", 0.0), ("magn", 2.0 / 3.0)):
+        got = O.kernel_r(g, chi, u, v, ch)
+        uu = (2.5 + v[:, 0, 0, 0, 0])[:, None, None]
+        gg, cc = g[:, 0, 0, 0, 0], chi[:, 0, 0, 0, 0][..., None]
+        ref = uu * (gg - gg * uu * cc - shift)
+        assert np.abs(got[:, 0, 0, 0, 0] - ref).max() < 1e-13
+
+
+@pytest.mark.parametrize("nb,nk,niw,niv", [(1, (6, 6, 1), 2, 3), (2, (4, 4, 1), 1, 2)])
+def test_sigma_fft_equals_direct(nb, nk, niw, niv):
+    rng = np.random.default_rng(2)
+    maps = O.kgrid_irr_maps(nk)
+    nq = len(maps["irrk_ind"])
+    kern = rng.standard_normal((nq,) + (nb,) * 4 + (niw + 1, 2 * niv)) \
+        + 1j * rng.standard_normal((nq,) + (nb,) * 4 + (niw + 1, 2 * niv))
+    g = rng.standard_normal(nk + (nb, nb, 2 * (niw + niv + 2))) + 1j * rng.standard_normal(nk + (nb, nb, 2 * (niw + niv + 2)))
+    a = O.sigma_fft(kern, g, maps["irrk_inv"], 5.0, niv, niw)
+    b = O.sigma_direct(kern, g, maps["irrk_inv"], 5.0, niv, niw)
+    assert np.abs(a - b).max() < 1e-12 * np.abs(b).max()
+
+
+def test_sigma_direct_brute_force():
+    """One element by explicit loops: -1/(2 beta N) sum_{q,w,a,d} K_aijd(q,w,v) G_ad(k+q, v-w)."""
+    rng = np.random.default_rng(3)
+    nb, nk, niw, niv, beta = 2, (4, 4, 1), 1, 2, 4.0
+    maps = O.kgrid_irr_maps(nk)
+    nq = len(maps["irrk_ind"])
+    kern = rng.standard_normal((nq,) + (nb,) * 4 + (niw + 1, 2 * niv)) \
+        + 1j * rng.standard_normal((nq,) + (nb,) * 4 + (niw + 1, 2 * niv))
+    gniv = niw + niv + 1
+    g = rng.standard_normal(nk + (nb, nb, 2 * gniv)) + 1j * rng.standard_normal(nk + (nb, nb, 2 * gniv))
+    sig = O.sigma_direct(kern, g, maps["irrk_inv"], beta, niv, niw)
+    kfull = O.to_full_niw(kern[maps["irrk_inv"]], 1)  # [N, nb^4, 2niw+1, 2niv]
+    for (kx, ky, i, j, v) in [(1, 2, 0, 1, 1), (3, 0, 1, 1, 0)]:
+        acc = 0
+        for qx in range(4):
+            for qy in range(4):
+                qf = qx * 4 + qy
+                for iw, w in enumerate(range(-niw, niw + 1)):
+                    for a in range(nb):
+                        for d in range(nb):
+                            acc += kfull[qf, a, i, j, d, iw, niv + v] * \
+                                g[(kx + qx) % 4, (ky + qy) % 4, 0, a, d, gniv + v - w]
+        assert abs(sig[kx, ky, 0, i, j, niv + v] - (-0.5 / beta / 16 * acc)) < 1e-12
+        assert abs(sig[kx, ky, 0, i, j, niv - 1 - v] - np.conj(-0.5 / beta / 16 * acc)) < 1e-12
+
+
+def test_to_full_niw_symmetry():
+    rng = np.random.default_rng(4)
+    x = rng.standard_normal((2, 3, 4)) + 1j * rng.standard_normal((2, 3, 4))  # [., w>=0, v]
+    full = O.to_full_niw(x, 1)
+    assert full.shape == (2, 5, 4)
+    assert np.allclose(full[:, 2:], x)
+    assert np.allclose(full[:, 0, :], np.conj(x[:, 2, ::-1]))
