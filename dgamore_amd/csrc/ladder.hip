@@ -1,5 +1,8 @@
 // Entry point of the batched ladder solves: picks the register-resident
 // kernel for small compound sizes and the blocked workspace LU otherwise.
+#include <cstdlib>
+#include <string>
+
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
@@ -9,11 +12,24 @@ namespace dgami {
 
 int ladder_reg_variant(int n, int nrhs);
 int ladder_reg_launch(int variant, const LadderArgs& a, hipStream_t stream);
+int ladder_col_variant(int n, int nrhs);
+int ladder_col_launch(int variant, const LadderArgs& a, hipStream_t stream);
 int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream);
 
+// Kernel family of the plain FP64 solve: "col" (column-cyclic registers, n <= 128),
+// "reg" (2D register tiles, n <= 128) or "blocked" (workspace LU, any n). The
+// environment variable DGAMI_LADDER_KERNEL forces one family for comparisons.
 int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
-  const int v = ladder_reg_variant(a.n, a.T);
-  if (v >= 0) return ladder_reg_launch(v, a, stream);
+  const char* force = getenv("DGAMI_LADDER_KERNEL");
+  const std::string f = force ? force : "";
+  if (f != "reg" && f != "blocked") {
+    const int v = ladder_col_variant(a.n, a.T);
+    if (v >= 0) return ladder_col_launch(v, a, stream);
+  }
+  if (f != "blocked") {
+    const int v = ladder_reg_variant(a.n, a.T);
+    if (v >= 0) return ladder_reg_launch(v, a, stream);
+  }
   return ladder_blocked_fp64(a, stream);
 }
 

@@ -1,0 +1,51 @@
+"""lambda-correction of the physical susceptibility, FP64 on the host (lambda_correction.py:13-73).
+
+The root find is a short scalar loop over nq*(2niw+1) numbers; it stays in
+double precision on the host as in the reference (its arrays there are complex64).
+"""
+
+import numpy as np
+
+from dgamore_amd import config
+
+
+def get_lambda_start(chi_r):
+    """-min(1/Re chi(w=0)) (lambda_correction.py:13-18)."""
+    w0 = chi_r.shape[-1] // 2
+    return -np.min(1.0 / chi_r[..., w0].real)
+
+
+def apply_lambda(chi_r, lambda_):
+    """1/(1/chi + lambda) (lambda_correction.py:21-25)."""
+    return 1.0 / (1.0 / chi_r + lambda_)
+
+
+def find_lambda(chi_r_mat, chi_r_loc_sum, delta=0.1, eps=1e-7, maxiter=1000):
+    """Newton-like iteration with step halving (lambda_correction.py:28-60), weights irrk_count."""
+    chi_r_mat = np.asarray(chi_r_mat, dtype=np.complex128)
+    counts = np.asarray(config.lattice.q_grid.irrk_count)[:, None]
+    lambda_start = get_lambda_start(chi_r_mat)
+    lambda_ = lambda_start + delta
+    factor = 1.0 / config.sys.beta / config.lattice.q_grid.nk_tot
+    for _ in range(maxiter):
+        chi_lam = apply_lambda(chi_r_mat, lambda_)
+        f_lam = (counts * chi_lam).sum() * factor - chi_r_loc_sum
+        fp_lam = -(counts * chi_lam**2).sum() * factor
+        lambda_new = lambda_ - (f_lam / fp_lam).real
+        if abs(f_lam.real) < eps:
+            return lambda_new
+        if lambda_new < lambda_:
+            delta /= 2
+            lambda_ = lambda_start + delta
+        else:
+            lambda_ = lambda_new
+    return lambda_
+
+
+def perform_single_lambda_correction(chi_r, chi_r_loc_sum):
+    """(corrected chi in the half bosonic range, lambda) for one channel (lambda_correction.py:63-73)."""
+    chi_r = chi_r.to_full_niw_range()
+    chi_mat = chi_r.compress_q_dimension().mat.squeeze()
+    lam = find_lambda(chi_mat, chi_r_loc_sum)
+    chi_r.mat = apply_lambda(chi_mat, lam)[:, None, None, None, None, :]
+    return chi_r.to_half_niw_range(), lam
