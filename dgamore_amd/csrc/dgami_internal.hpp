@@ -52,6 +52,58 @@ __host__ __device__ __forceinline__ double2 cinv(double2 b) {
 }
 __host__ __device__ __forceinline__ double cabs1(double2 a) { return fabs(a.x) + fabs(a.y); }
 
+// complex float (interleaved, numpy.complex64 layout)
+__host__ __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__host__ __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__host__ __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__host__ __device__ __forceinline__ float2 cfms(float2 a, float2 b, float2 c) {
+  float re = fmaf(-b.x, c.x, a.x);
+  re = fmaf(b.y, c.y, re);
+  float im = fmaf(-b.x, c.y, a.y);
+  im = fmaf(-b.y, c.x, im);
+  return make_float2(re, im);
+}
+__host__ __device__ __forceinline__ float2 cfma(float2 a, float2 b, float2 c) {
+  float re = fmaf(b.x, c.x, a.x);
+  re = fmaf(-b.y, c.y, re);
+  float im = fmaf(b.x, c.y, a.y);
+  im = fmaf(b.y, c.x, im);
+  return make_float2(re, im);
+}
+__host__ __device__ __forceinline__ float2 cinv(float2 b) {
+  if (fabsf(b.x) >= fabsf(b.y)) {
+    float r = b.y / b.x;
+    float d = b.x + b.y * r;
+    return make_float2(1.0f / d, -r / d);
+  }
+  float r = b.x / b.y;
+  float d = b.x * r + b.y;
+  return make_float2(r / d, -1.0f / d);
+}
+__host__ __device__ __forceinline__ float cabs1(float2 a) { return fabsf(a.x) + fabsf(a.y); }
+
+// conversions and zero of the two storage types
+template <typename S>
+__host__ __device__ __forceinline__ S czero();
+template <>
+__host__ __device__ __forceinline__ double2 czero<double2>() { return make_double2(0.0, 0.0); }
+template <>
+__host__ __device__ __forceinline__ float2 czero<float2>() { return make_float2(0.0f, 0.0f); }
+template <typename S>
+__host__ __device__ __forceinline__ S from_c128(double2 v);
+template <>
+__host__ __device__ __forceinline__ double2 from_c128<double2>(double2 v) { return v; }
+template <>
+__host__ __device__ __forceinline__ float2 from_c128<float2>(double2 v) {
+  return make_float2(static_cast<float>(v.x), static_cast<float>(v.y));
+}
+__host__ __device__ __forceinline__ double2 to_c128(double2 v) { return v; }
+__host__ __device__ __forceinline__ double2 to_c128(float2 v) {
+  return make_double2(static_cast<double>(v.x), static_cast<double>(v.y));
+}
+
 // ----------------------------------------------------------------------------
 // error reporting (thread-local last error; host side)
 // ----------------------------------------------------------------------------

@@ -17,6 +17,8 @@ enum WorkspaceSlot {
   WS_DC = 5,         // double-counting kernel operands
   WS_PAIR = 6,       // Eliashberg matvec intermediates
   WS_MISC = 7,
+  WS_MIXED = 8,      // mixed-precision factors (slots 8 and 9)
+  WS_MIXED_VEC = 10, // refinement vectors
 };
 
 int launch_fft_grid(const double2* in, double2* out, int64_t batch, int nx, int ny, int nz, int sign, double scale,

@@ -1,33 +1,36 @@
-// Plain-FP64 ladder solves for compound sizes beyond the register kernels.
+// Ladder solves for compound sizes beyond the register kernels (n > 128).
 //
-// Batched, blocked right-looking LU with partial pivoting on the augmented
-// matrices [A | B] held in a device workspace ([batch][n][ld], row major):
-//   assemble -> for each panel of NB columns: panel factorisation (one
-//   workgroup per problem, full-row swaps), TRSM of the block row, GEMM update
-//   of the trailing block (all problems of the batch in one launch)
-//   -> back substitution and output.
-// The right-hand-side columns are part of every row, so the forward
-// elimination of B happens inside the TRSM/GEMM updates.
+// Batched, blocked right-looking LU with partial pivoting on a device
+// workspace W[batch][n][ld] (row major), templated on the storage type:
+// complex128 for the plain FP64 form, complex64 for the mixed form (whose
+// factors are refined in FP64 by ladder_mixed.hip). Per panel of NB columns:
+//   1. panel kernel: one workgroup per problem holds the panel rows in
+//      registers (one thread per row or per few rows), factors the NB columns
+//      with partial pivoting (two barriers per column, no global round trips),
+//      and writes L/U of the panel, the pivots and the row map of the panel's
+//      interchanges;
+//   2. rows kernel: for every column outside the panel, applies the panel's
+//      interchanges as one gather through the row map (independent loads, not
+//      a dependent chain of swaps) and, on trailing columns, the unit-lower
+//      triangular solve U12 = L11^-1 A12;
+//   3. trailing update A22 -= L21 U12 for all problems of the batch in one launch.
+// With the right-hand sides carried as extra columns (plain form), back
+// substitution on them finishes the solve.
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
+#include "ladder_blocked.hpp"
 #include "ladder_common.hpp"
+
+#include <utility>
 
 namespace dgami {
 
-constexpr int BLK_NB = 32;      // panel width
-constexpr int GEMM_TILE = 64;   // trailing-update tile (rows and columns)
-constexpr int BS_CB = 32;       // back-substitution chunk
-
-struct BlockedState {
-  double2* W;     // [batch][n][ld]
-  double2* dinv;  // [batch][n] inverse pivots
-  int* info;      // [batch]
-  int n, ld, ncol;
-};
+constexpr int MAP_STRIDE = 3 * 32 + 1;  // interchange map per problem: count, dst[NB], src[NB], dest[NB] (NB <= 32)
 
 // ---------------------------------------------------------------------------
-__global__ void blk_assemble_kernel(LadderArgs a, BlockedState st, int item0, int batch) {
+template <typename S>
+__global__ void blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0, int batch) {
   const size_t per = static_cast<size_t>(st.n) * st.ld;
   const size_t total = per * batch;
   for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
@@ -46,119 +49,239 @@ __global__ void blk_assemble_kernel(LadderArgs a, BlockedState st, int item0, in
     } else if (j < st.ncol) {
       val = ladder_rhs(a, i, j - st.n);
     }
-    st.W[idx] = val;
+    st.W[idx] = from_c128<S>(val);
   }
   if (blockIdx.x == 0)
     for (int b = threadIdx.x; b < batch; b += blockDim.x) st.info[b] = 0;
 }
 
 // ---------------------------------------------------------------------------
-// panel factorisation of columns [p0, p0+NB) (one workgroup of 256 per problem)
-__global__ void __launch_bounds__(256) blk_panel_kernel(BlockedState st, int p0) {
-  const int b = blockIdx.x;
-  const int n = st.n, ld = st.ld, ncol = st.ncol;
-  double2* W = st.W + static_cast<size_t>(b) * n * ld;
-  const int pend = p0 + BLK_NB < n ? p0 + BLK_NB : n;
-  __shared__ double s_val[4];
-  __shared__ int s_idx[4];
-  __shared__ double2 s_row[BLK_NB];
-  __shared__ double2 s_piv;
-  __shared__ int s_p;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int jc = p0; jc < pend; ++jc) {
-    double best = -1.0;
-    int bi = 0x7fffffff;
-    for (int i = jc + tid; i < n; i += 256) {
-      const double m = cabs1(W[static_cast<size_t>(i) * ld + jc]);
-      if (m > best) {
-        best = m;
-        bi = i;
-      }
-    }
+// Panel factorisation: thread t owns rows p0 + t + blockDim*m (m < RPT) of columns [p0, p0+NB).
+// Each column is one instance of panel_column<C>, expanded at compile time so that every
+// register index is a constant.
+template <typename S, int NB>
+struct PanelShared {
+  double val[2][16];
+  int idx[2][16];
+  S prow[2][NB];
+  S krow[2][NB];
+  int piv[NB];
+  int mdst[2 * NB], msrc[2 * NB];
+};
+
+template <typename S, int NB, int RPT, int C>
+__device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>& sh, BlockedState<S>& st, int b,
+                                             int p0, int nbw) {
+  if (C >= nbw) return;
+  const int n = st.n, TH = blockDim.x;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = TH >> 6;
+  const int col = p0 + C;
+  constexpr int buf = C & 1;
+  double best = -1.0;
+  int bidx = 0x7fffffff;
 #pragma unroll
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ov = __shfl_xor(best, off);
-      const int oi = __shfl_xor(bi, off);
-      if (ov > best || (ov == best && oi < bi)) {
-        best = ov;
-        bi = oi;
+  for (int m = 0; m < RPT; ++m) {
+    const int r = p0 + tid + TH * m;
+    const double v = static_cast<double>(cabs1(P[m][C]));
+    if (r >= col && r < n && v > best) {
+      best = v;
+      bidx = r;
+    }
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    const double ov = __shfl_xor(best, off);
+    const int oi = __shfl_xor(bidx, off);
+    if (ov > best || (ov == best && oi < bidx)) {
+      best = ov;
+      bidx = oi;
+    }
+  }
+  if (lane == 0) {
+    sh.val[buf][wave] = best;
+    sh.idx[buf][wave] = bidx;
+  }
+  __syncthreads();
+  best = sh.val[buf][0];
+  bidx = sh.idx[buf][0];
+  for (int wv = 1; wv < nwave; ++wv) {
+    const double ov = sh.val[buf][wv];
+    const int oi = sh.idx[buf][wv];
+    if (ov > best || (ov == best && oi < bidx)) {
+      best = ov;
+      bidx = oi;
+    }
+  }
+  const int p = bidx;
+  constexpr int mk = 0;  // column C's diagonal row p0 + C lies in the first register row (C < NB <= TH)
+  if constexpr (RPT == 1) {
+    if (tid == p - p0) {
+#pragma unroll
+      for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[0][cc];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < RPT; ++m)
+      if (p0 + tid + TH * m == p) {
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[m][cc];
       }
+  }
+  if (p != col && tid == C) {
+#pragma unroll
+    for (int cc = 0; cc < NB; ++cc) sh.krow[buf][cc] = P[mk][cc];
+  }
+  if (tid == 0) sh.piv[C] = p;
+  __syncthreads();
+  const bool ok = best > 0.0;
+  const S pinv = ok ? cinv(sh.prow[buf][C]) : czero<S>();
+  if (tid == 0) {
+    st.dinv[static_cast<size_t>(b) * n + col] = to_c128(pinv);
+    if (!ok && st.info[b] == 0) st.info[b] = col + 1;
+  }
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const int r = p0 + tid + TH * m;
+    if (r == col) {
+#pragma unroll
+      for (int cc = 0; cc < NB; ++cc) P[m][cc] = sh.prow[buf][cc];
+    } else if (r > col && r < n) {
+      if (r == p) {
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc) P[m][cc] = sh.krow[buf][cc];
+      }
+      const S l = cmul(P[m][C], pinv);
+      P[m][C] = l;
+#pragma unroll
+      for (int cc = C + 1; cc < NB; ++cc) P[m][cc] = cfms(P[m][cc], l, sh.prow[buf][cc]);
     }
-    if (lane == 0) {
-      s_val[wave] = best;
-      s_idx[wave] = bi;
+  }
+}
+
+template <typename S, int NB, int RPT, int... Cs>
+__device__ __forceinline__ void panel_columns(S (&P)[RPT][NB], PanelShared<S, NB>& sh, BlockedState<S>& st, int b,
+                                              int p0, int nbw, std::integer_sequence<int, Cs...>) {
+  (panel_column<S, NB, RPT, Cs>(P, sh, st, b, p0, nbw), ...);
+}
+
+template <typename S, int NB, int RPT, int MAXTH>
+__global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, int p0) {
+  const int b = blockIdx.x;
+  const int n = st.n, ld = st.ld;
+  const int TH = blockDim.x;
+  const int tid = threadIdx.x;
+  S* W = st.W + static_cast<size_t>(b) * n * ld;
+  const int nbw = (n - p0 < NB) ? n - p0 : NB;
+  __shared__ PanelShared<S, NB> sh;
+
+  S P[RPT][NB];
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const int r = p0 + tid + TH * m;
+#pragma unroll
+    for (int c = 0; c < NB; ++c) P[m][c] = (r < n && c < nbw) ? W[static_cast<size_t>(r) * ld + p0 + c] : czero<S>();
+  }
+  panel_columns<S, NB, RPT>(P, sh, st, b, p0, nbw, std::make_integer_sequence<int, NB>{});
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const int r = p0 + tid + TH * m;
+#pragma unroll
+    for (int c = 0; c < NB; ++c)
+      if (r < n && c < nbw) W[static_cast<size_t>(r) * ld + p0 + c] = P[m][c];
+  }
+  // Interchange map of the panel. A row below the panel only ever exchanges its value with a
+  // panel row, so after the NB interchanges every row below the panel holds a former panel
+  // row's value, and the former values of rows below the panel end in panel rows:
+  //   outer[e] = (dst, src): panel row dst receives row src (src below the panel), e < count;
+  //   dest[c]:  panel row p0 + c's former value goes to row dest[c] (-1: stays).
+  if (tid == 0) {
+    int* dst = sh.mdst;
+    int* src = sh.msrc;
+    int cnt = 0;
+    for (int c = 0; c < nbw; ++c) {
+      const int j = p0 + c, p = sh.piv[c];
+      st.piv[static_cast<size_t>(b) * n + j] = p;
+      if (p == j) continue;
+      int ij = -1, ip = -1;
+      for (int e = 0; e < cnt; ++e) {
+        if (dst[e] == j) ij = e;
+        if (dst[e] == p) ip = e;
+      }
+      if (ij < 0) {
+        ij = cnt++;
+        dst[ij] = j;
+        src[ij] = j;
+      }
+      if (ip < 0) {
+        ip = cnt++;
+        dst[ip] = p;
+        src[ip] = p;
+      }
+      const int t = src[ij];
+      src[ij] = src[ip];
+      src[ip] = t;
     }
-    __syncthreads();
-    if (tid == 0) {
-      double bv = s_val[0];
-      int bx = s_idx[0];
-      for (int wv = 1; wv < 4; ++wv)
-        if (s_val[wv] > bv || (s_val[wv] == bv && s_idx[wv] < bx)) {
-          bv = s_val[wv];
-          bx = s_idx[wv];
-        }
-      s_p = bx;
-      const double2 pv = W[static_cast<size_t>(bx) * ld + jc];
-      if (!(bv > 0.0)) {
-        if (st.info[b] == 0) st.info[b] = jc + 1;
-        s_piv = make_double2(0.0, 0.0);
+    int* map = st.rowmap + static_cast<size_t>(b) * MAP_STRIDE;
+    for (int c = 0; c < NB; ++c) map[1 + 2 * NB + c] = -1;
+    int outer = 0;
+    for (int e = 0; e < cnt; ++e) {
+      if (src[e] == dst[e]) continue;
+      if (src[e] < p0 + nbw) {
+        map[1 + 2 * NB + (src[e] - p0)] = dst[e];
       } else {
-        s_piv = cinv(pv);
-      }
-      st.dinv[static_cast<size_t>(b) * n + jc] = s_piv;
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != jc) {
-      for (int c = p0 + tid; c < ncol; c += 256) {
-        const double2 t = W[static_cast<size_t>(jc) * ld + c];
-        W[static_cast<size_t>(jc) * ld + c] = W[static_cast<size_t>(p) * ld + c];
-        W[static_cast<size_t>(p) * ld + c] = t;
+        map[1 + outer] = dst[e];
+        map[1 + NB + outer] = src[e];
+        ++outer;
       }
     }
-    __syncthreads();
-    const double2 dinv = s_piv;
-    // pivot-row segment of the panel into LDS
-    for (int c = jc + 1 + tid; c < pend; c += 256) s_row[c - p0] = W[static_cast<size_t>(jc) * ld + c];
-    __syncthreads();
-    const int width = pend - jc - 1;
-    for (int i = jc + 1 + tid / 4; i < n; i += 64) {
-      double2* row = W + static_cast<size_t>(i) * ld;
-      const double2 l = cmul(row[jc], dinv);
-      for (int c = jc + 1 + (tid & 3); c < jc + 1 + width; c += 4) row[c] = cfms(row[c], l, s_row[c - p0]);
-    }
-    __syncthreads();
-    // store the multipliers after the update used the unscaled column
-    for (int i = jc + 1 + tid; i < n; i += 256) {
-      double2* e = W + static_cast<size_t>(i) * ld + jc;
-      *e = cmul(*e, dinv);
-    }
-    __syncthreads();
+    map[0] = outer;
   }
 }
 
 // ---------------------------------------------------------------------------
-// U12 = L11^-1 A12 for block row [p0, p0+NB), columns [p0+NB, ncol)
-__global__ void __launch_bounds__(64) blk_trsm_kernel(BlockedState st, int p0, int ntile) {
+// Interchanges for every column outside the panel, then U12 = L11^-1 A12 on trailing columns.
+// grid: batch * ntile, 64 threads; tile t covers columns t*64 .. t*64+63 of [0, ncol).
+template <typename S, int NB>
+__global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0, int ntile) {
   const int b = blockIdx.x / ntile;
   const int tile = blockIdx.x - b * ntile;
   const int n = st.n, ld = st.ld;
-  double2* W = st.W + static_cast<size_t>(b) * n * ld;
-  const int kb = (p0 + BLK_NB < n ? BLK_NB : n - p0);
-  __shared__ double2 s_l[BLK_NB][BLK_NB];
-  for (int e = threadIdx.x; e < kb * kb; e += 64) {
-    const int r = e / kb, c = e - (e / kb) * kb;
+  S* W = st.W + static_cast<size_t>(b) * n * ld;
+  const int nbw = (n - p0 < NB) ? n - p0 : NB;
+  __shared__ S s_l[NB][NB + 1];
+  __shared__ int s_dst[NB], s_src[NB], s_dest[NB];
+  const int* map = st.rowmap + static_cast<size_t>(b) * MAP_STRIDE;
+  const int outer = map[0];
+  for (int e = threadIdx.x; e < NB; e += 64) {
+    s_dst[e] = e < outer ? map[1 + e] : 0;
+    s_src[e] = e < outer ? map[1 + NB + e] : 0;
+    s_dest[e] = map[1 + 2 * NB + e];
+  }
+  for (int e = threadIdx.x; e < nbw * nbw; e += 64) {
+    const int r = e / nbw, c = e - (e / nbw) * nbw;
     s_l[r][c] = W[static_cast<size_t>(p0 + r) * ld + p0 + c];
   }
   __syncthreads();
-  const int col = p0 + kb + tile * 64 + threadIdx.x;
-  if (col >= st.ncol) return;
-  double2 x[BLK_NB];
+  const int col = tile * 64 + threadIdx.x;
+  if (col >= st.ncol || (col >= p0 && col < p0 + nbw)) return;
+  if (col < p0 && !st.swap_left) return;
+  // former panel values first (registers), then rows from below the panel into panel rows,
+  // then the former panel values to their destinations
+  S pv[NB];
 #pragma unroll
-  for (int r = 0; r < BLK_NB; ++r) {
-    x[r] = make_double2(0.0, 0.0);
-    if (r < kb) {
-      double2 acc = W[static_cast<size_t>(p0 + r) * ld + col];
+  for (int c = 0; c < NB; ++c)
+    pv[c] = (c < nbw && s_dest[c] >= 0) ? W[static_cast<size_t>(p0 + c) * ld + col] : czero<S>();
+  for (int e = 0; e < outer; ++e) W[static_cast<size_t>(s_dst[e]) * ld + col] = W[static_cast<size_t>(s_src[e]) * ld + col];
+#pragma unroll
+  for (int c = 0; c < NB; ++c)
+    if (c < nbw && s_dest[c] >= 0) W[static_cast<size_t>(s_dest[c]) * ld + col] = pv[c];
+  if (col < p0) return;  // left of the panel: interchanges only
+  S x[NB];
+#pragma unroll
+  for (int r = 0; r < NB; ++r) {
+    x[r] = czero<S>();
+    if (r < nbw) {
+      S acc = W[static_cast<size_t>(p0 + r) * ld + col];
 #pragma unroll
       for (int s = 0; s < r; ++s) acc = cfms(acc, s_l[r][s], x[s]);
       x[r] = acc;
@@ -168,37 +291,39 @@ __global__ void __launch_bounds__(64) blk_trsm_kernel(BlockedState st, int p0, i
 }
 
 // ---------------------------------------------------------------------------
-// A22 -= L21 U12 (rows [p0+NB, n), columns [p0+NB, ncol)), 64x64 tiles, 256 threads
-__global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState st, int p0, int tiles_r, int tiles_c) {
+// A22 -= L21 U12 (rows [p0+NB, n), columns [p0+NB, ncol)), 64x64 tiles, 256 threads.
+template <typename S, int NB>
+__global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c) {
+  constexpr int TILE = 64;
   const int per = tiles_r * tiles_c;
   const int b = blockIdx.x / per;
   const int t = blockIdx.x - b * per;
   const int tr = t / tiles_c, tcl = t - (t / tiles_c) * tiles_c;
   const int n = st.n, ld = st.ld, ncol = st.ncol;
-  const int k0 = p0 + BLK_NB;
-  double2* W = st.W + static_cast<size_t>(b) * n * ld;
-  const int r0 = k0 + tr * GEMM_TILE, c0 = k0 + tcl * GEMM_TILE;
-  __shared__ double2 s_a[GEMM_TILE][BLK_NB + 1];
-  __shared__ double2 s_b[BLK_NB][GEMM_TILE + 1];
+  const int k0 = p0 + NB;
+  S* W = st.W + static_cast<size_t>(b) * n * ld;
+  const int r0 = k0 + tr * TILE, c0 = k0 + tcl * TILE;
+  __shared__ S s_a[TILE][NB + 1];
+  __shared__ S s_b[NB][TILE + 1];
   const int tid = threadIdx.x;
-  for (int e = tid; e < GEMM_TILE * BLK_NB; e += 256) {
-    const int r = e / BLK_NB, s = e - (e / BLK_NB) * BLK_NB;
-    s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : make_double2(0.0, 0.0);
+  for (int e = tid; e < TILE * NB; e += 256) {
+    const int r = e / NB, s = e - (e / NB) * NB;
+    s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : czero<S>();
   }
-  for (int e = tid; e < GEMM_TILE * BLK_NB; e += 256) {
-    const int s = e / GEMM_TILE, c = e - (e / GEMM_TILE) * GEMM_TILE;
-    s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : make_double2(0.0, 0.0);
+  for (int e = tid; e < TILE * NB; e += 256) {
+    const int s = e / TILE, c = e - (e / TILE) * TILE;
+    s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : czero<S>();
   }
   __syncthreads();
   const int ty = tid / 16, tx = tid % 16;
-  double2 acc[4][4];
+  S acc[4][4];
 #pragma unroll
   for (int i = 0; i < 4; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = make_double2(0.0, 0.0);
+    for (int j = 0; j < 4; ++j) acc[i][j] = czero<S>();
 #pragma unroll 4
-  for (int s = 0; s < BLK_NB; ++s) {
-    double2 av[4], bv[4];
+  for (int s = 0; s < NB; ++s) {
+    S av[4], bv[4];
 #pragma unroll
     for (int i = 0; i < 4; ++i) av[i] = s_a[ty + 16 * i][s];
 #pragma unroll
@@ -216,7 +341,7 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState st, int p0, 
     for (int j = 0; j < 4; ++j) {
       const int c = c0 + tx + 16 * j;
       if (c < ncol) {
-        double2* e = W + static_cast<size_t>(r) * ld + c;
+        S* e = W + static_cast<size_t>(r) * ld + c;
         *e = csub(*e, acc[i][j]);
       }
     }
@@ -231,15 +356,16 @@ __device__ __forceinline__ double readlane_dbl(double v, int lane) {
 }
 
 // ---------------------------------------------------------------------------
-// back substitution U x = y on the right-hand-side columns, then output / beta
-__global__ void __launch_bounds__(256) blk_backsub_kernel(LadderArgs a, BlockedState st, int item0) {
+// back substitution U x = y on the carried right-hand-side columns (FP64), then output / beta
+__global__ void __launch_bounds__(256) blk_backsub_kernel(LadderArgs a, BlockedState<double2> st, int item0) {
+  constexpr int CB = 32;
   const int b = blockIdx.x;
   const int n = st.n, ld = st.ld, T = a.T;
   double2* W = st.W + static_cast<size_t>(b) * n * ld;
   const double2* dinv = st.dinv + static_cast<size_t>(b) * n;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int c1 = n; c1 > 0; c1 -= BS_CB) {
-    const int c0 = c1 > BS_CB ? c1 - BS_CB : 0;
+  for (int c1 = n; c1 > 0; c1 -= CB) {
+    const int c0 = c1 > CB ? c1 - CB : 0;
     if (wave == 0) {
       const int row = c0 + lane;
       const bool live = lane < c1 - c0;
@@ -247,7 +373,7 @@ __global__ void __launch_bounds__(256) blk_backsub_kernel(LadderArgs a, BlockedS
       for (int r = 0; r < T; ++r) {
         double2 y = live ? W[static_cast<size_t>(row) * ld + n + r] : make_double2(0.0, 0.0);
 #pragma unroll
-        for (int t = BS_CB - 1; t >= 0; --t) {
+        for (int t = CB - 1; t >= 0; --t) {
           if (c0 + t >= c1) continue;
           if (lane == t) y = cmul(y, di);
           const double2 x = make_double2(readlane_dbl(y.x, t), readlane_dbl(y.y, t));
@@ -275,47 +401,68 @@ __global__ void __launch_bounds__(256) blk_backsub_kernel(LadderArgs a, BlockedS
   if (tid == 0 && a.info) a.info[item] = st.info[b];
 }
 
+// ---------------------------------------------------------------------------
+static unsigned grid_for_total(size_t total) {
+  return ceil_div(total, 256) > 262144u ? 262144u : ceil_div(total, 256);
+}
+
+template <typename S, int NB, int RPT, int MAXTH>
+static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream) {
+  const int n = st.n, ld = st.ld, ncol = st.ncol;
+  const size_t tot = static_cast<size_t>(nb) * n * ld;
+  hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(grid_for_total(tot)), dim3(256), 0, stream, a, st, item0, nb);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  for (int p0 = 0; p0 < n; p0 += NB) {
+    const int rows = n - p0;
+    int th = ((rows + RPT - 1) / RPT + 63) / 64 * 64;
+    DGAMI_ARG_CHECK(th <= MAXTH, "ladder: compound size too large for the panel kernel");
+    hipLaunchKernelGGL((blk_panel_kernel<S, NB, RPT, MAXTH>), dim3(nb), dim3(th), 0, stream, st, p0);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    const int ntile = ceil_div(ncol, 64);
+    hipLaunchKernelGGL((blk_rows_kernel<S, NB>), dim3(nb * ntile), dim3(64), 0, stream, st, p0, ntile);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    const int k1 = p0 + NB;
+    if (k1 < n) {
+      const int tr = ceil_div(n - k1, 64), tc = ceil_div(ncol - k1, 64);
+      const size_t g = static_cast<size_t>(nb) * tr * tc;
+      DGAMI_ARG_CHECK(g < (size_t(1) << 31), "ladder: trailing update grid too large");
+      hipLaunchKernelGGL((blk_gemm_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr,
+                         tc);
+      DGAMI_HIP_CHECK(hipGetLastError());
+    }
+  }
+  return 0;
+}
+
+template <>
+int blocked_factor<double2>(const LadderArgs& a, BlockedState<double2>& st, int item0, int nb, hipStream_t stream) {
+  if (st.n <= 768) return blocked_factor_nb<double2, 32, 1, 768>(a, st, item0, nb, stream);
+  return blocked_factor_nb<double2, 16, 3, 512>(a, st, item0, nb, stream);
+}
+
+template <>
+int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int item0, int nb, hipStream_t stream) {
+  if (st.n <= 1024) return blocked_factor_nb<float2, 32, 1, 1024>(a, st, item0, nb, stream);
+  return blocked_factor_nb<float2, 16, 3, 512>(a, st, item0, nb, stream);
+}
+
 int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream) {
   const int n = a.n, ncol = n + a.T;
   const int ld = (ncol + 3) & ~3;
+  DGAMI_ARG_CHECK(n <= 1536, "ladder: compound size above 1536 is not supported");
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);
-  // batch so that the workspace stays below ~4 GiB
-  size_t batch = (size_t(4) << 30) / per;
+  // MI355X holds 288 GB of HBM: factor up to 48 GiB of problems per pass
+  size_t batch = (size_t(48) << 30) / per;
   if (batch < 1) batch = 1;
   if (batch > items) batch = items;
   if (batch > 65535) batch = 65535;
-  BlockedState st;
-  st.W = static_cast<double2*>(workspace(WS_LADDER, batch * per));
-  st.dinv = static_cast<double2*>(workspace(WS_LADDER_AUX, batch * n * sizeof(double2) + batch * sizeof(int) + 256));
-  if (!st.W || !st.dinv) return -2;
-  st.info = reinterpret_cast<int*>(st.dinv + batch * n);
-  st.n = n;
-  st.ld = ld;
-  st.ncol = ncol;
+  BlockedState<double2> st;
+  if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
   for (size_t item0 = 0; item0 < items; item0 += batch) {
     const int nb = static_cast<int>(items - item0 < batch ? items - item0 : batch);
-    const size_t tot = static_cast<size_t>(nb) * n * ld;
-    const unsigned ablocks = ceil_div(tot, 256) > 262144u ? 262144u : ceil_div(tot, 256);
-    hipLaunchKernelGGL(blk_assemble_kernel, dim3(ablocks), dim3(256), 0, stream, a, st, static_cast<int>(item0), nb);
-    DGAMI_HIP_CHECK(hipGetLastError());
-    for (int p0 = 0; p0 < n; p0 += BLK_NB) {
-      hipLaunchKernelGGL(blk_panel_kernel, dim3(nb), dim3(256), 0, stream, st, p0);
-      DGAMI_HIP_CHECK(hipGetLastError());
-      const int k1 = p0 + (n - p0 < BLK_NB ? n - p0 : BLK_NB);
-      if (k1 < ncol) {
-        const int ntile = ceil_div(ncol - k1, 64);
-        hipLaunchKernelGGL(blk_trsm_kernel, dim3(nb * ntile), dim3(64), 0, stream, st, p0, ntile);
-        DGAMI_HIP_CHECK(hipGetLastError());
-      }
-      if (k1 < n) {  // then k1 == p0 + BLK_NB
-        const int tr = ceil_div(n - k1, GEMM_TILE), tc = ceil_div(ncol - k1, GEMM_TILE);
-        const size_t g = static_cast<size_t>(nb) * tr * tc;
-        DGAMI_ARG_CHECK(g < (size_t(1) << 31), "ladder: trailing update grid too large");
-        hipLaunchKernelGGL(blk_gemm_kernel, dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr, tc);
-        DGAMI_HIP_CHECK(hipGetLastError());
-      }
-    }
+    int rc = blocked_factor<double2>(a, st, static_cast<int>(item0), nb, stream);
+    if (rc) return rc;
     hipLaunchKernelGGL(blk_backsub_kernel, dim3(nb), dim3(256), 0, stream, a, st, static_cast<int>(item0));
     DGAMI_HIP_CHECK(hipGetLastError());
   }
