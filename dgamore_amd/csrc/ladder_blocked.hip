@@ -26,7 +26,7 @@
 
 namespace dgami {
 
-constexpr int MAP_STRIDE = 3 * 32 + 1;  // interchange map per problem: count, dst[NB], src[NB], dest[NB] (NB <= 32)
+constexpr int MAP_STRIDE = BLK_MAP_STRIDE;  // interchange map per problem: count, dst[NB], src[NB], dest[NB]
 
 // ---------------------------------------------------------------------------
 template <typename S>

@@ -19,7 +19,7 @@ struct ZgemmArgs {
 constexpr int ZG_TM = 64, ZG_TN = 64, ZG_TK = 16;
 
 // grid: (ceil(N/64), ceil(M/64), batch); block 256
-__global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g) {
+static __global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g) {
   const int b = blockIdx.z;
   const double2* A = g.A + b * g.sa;
   const double2* B = g.B + b * g.sb;

@@ -145,6 +145,60 @@ def test_ladder_plain_fp64(K, nb, niv, niw, nq, family, monkeypatch):
     assert rel(got[0], ref) < RTOL
 
 
+@pytest.mark.parametrize("nb,niv,niw,nq", [
+    (3, 40, 1, 2),    # n = 720, the 3-orbital box (panel of one thread per row)
+    (3, 48, 1, 1),    # n = 864 (panel of three rows per thread)
+    (1, 500, 1, 1),   # n = 1000, one right-hand side
+])
+def test_ladder_plain_fp64_large(K, nb, niv, niw, nq):
+    say(f"ladder fp64 large nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=nb * 100 + niv)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta)
+    assert int(info.abs().sum()) == 0
+    assert rel(got[0], ref) < RTOL
+
+
+# The mixed form (complex64 LU, FP64 iterative refinement) must agree with the FP64 solve
+# to 1e-8 relative error, the accuracy stated for it.
+MIXED_RTOL = 1e-8
+
+
+@pytest.mark.parametrize("nb,niv,niw,nq", [
+    (1, 20, 3, 5),    # n = 40
+    (1, 60, 2, 3),    # n = 120
+    (2, 15, 2, 4),    # n = 60, 4 right-hand sides
+    (3, 10, 2, 3),    # n = 180, 9 right-hand sides
+    (3, 40, 1, 2),    # n = 720, the 3-orbital box
+])
+def test_ladder_mixed_fp32_ir(K, nb, niv, niw, nq):
+    say(f"ladder mixed fp32+IR nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=nb * 100 + niv + 1)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=1)
+    assert int(info.abs().sum()) == 0
+    assert rel(got[0], ref) < MIXED_RTOL
+
+
+def test_ladder_mixed_two_channels_realistic_vertex(K):
+    say("ladder mixed srvo3 Gamma dens+magn nb=3 niv=10 n=180")
+    gd = golden("srvo3_gamma_dens_loc.npy")[:, :, :, :, :3]
+    gm = golden("srvo3_gamma_magn_loc.npy")[:, :, :, :, :3]
+    chi0_inv, _, u, v, beta = _ladder_case(3, 10, 2, 3, seed=7, gamma=gd)
+    ud, um = O.u_as_channel(u, "dens"), O.u_as_channel(u, "magn")
+    vd, vm = O.v_as_channel(v, "dens"), O.v_as_channel(v, "magn")
+    got, _ = _run_ladder(K, chi0_inv, [gd, gm], [ud, um], [vd, vm], beta, precision=1)
+    assert rel(got[0], O.ladder_sum_solve(chi0_inv, gd, ud, vd, beta)) < MIXED_RTOL
+    assert rel(got[1], O.ladder_sum_solve(chi0_inv, gm, um, vm, beta)) < MIXED_RTOL
+
+
+def test_ladder_split_bf16_mode_is_refused(K):
+    say("ladder bf16x3 mode -> error")
+    chi0_inv, gamma, u, v, beta = _ladder_case(1, 4, 1, 2, seed=5)
+    with pytest.raises(ValueError):
+        _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=2)
+
+
 def test_ladder_two_channels_realistic_vertex(K):
     """SrVO3's local irreducible vertices (reference fixture, 3 orbitals, n=180) with synthetic chi0."""
     say("ladder fp64 srvo3 Gamma dens+magn nb=3 niv=10 n=180")
@@ -298,6 +352,11 @@ def test_solve_sum_dense(K, nb, niv):
     got = out.reshape(3, 2, nb, nb, 2 * niv, nb, nb).cpu().numpy()  # [q,w,o1,o2,v,o4,o3]
     got = got.transpose(0, 2, 3, 6, 5, 1, 4)
     assert rel(got, ref) < RTOL
+    _lib.call("dgami_solve_sum_dense", 6, nb, niv, float(beta), K._p(d), 1, K._p(out), K._p(info, torch.int32),
+              K._stream())
+    got = out.reshape(3, 2, nb, nb, 2 * niv, nb, nb).cpu().numpy().transpose(0, 2, 3, 6, 5, 1, 4)
+    assert int(info.abs().sum()) == 0
+    assert rel(got, ref) < 1e-8
 
 
 # ----------------------------------------------------------------------------- whole SDE iteration
