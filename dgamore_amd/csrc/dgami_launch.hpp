@@ -13,12 +13,14 @@ enum WorkspaceSlot {
   WS_GR = 1,         // its momentum-space FFT
   WS_SIGMA_PART = 2, // per-frequency-chunk partial self-energies
   WS_LADDER = 3,     // ladder matrices / factors (large-n path)
-  WS_LADDER_AUX = 4, // ladder refinement vectors
+  WS_LADDER_AUX = 4, // pivots, inverse pivots and interchange maps of WS_LADDER
   WS_DC = 5,         // double-counting kernel operands
   WS_PAIR = 6,       // Eliashberg matvec intermediates
   WS_MISC = 7,
   WS_MIXED = 8,      // mixed-precision factors (slots 8 and 9)
   WS_MIXED_VEC = 10, // refinement vectors
+  WS_TRI = 11,       // inverse diagonal blocks of the triangular factors
+  WS_TRI_VEC = 12,   // complex64 right-hand sides of the triangular sweeps
 };
 
 int launch_fft_grid(const double2* in, double2* out, int64_t batch, int nx, int ny, int nz, int sign, double scale,

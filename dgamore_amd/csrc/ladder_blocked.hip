@@ -21,7 +21,10 @@
 #include "dgami_launch.hpp"
 #include "ladder_blocked.hpp"
 #include "ladder_common.hpp"
+#include "tri_solve.hpp"
 
+#include <cstdlib>
+#include <string>
 #include <utility>
 
 namespace dgami {
@@ -29,30 +32,30 @@ namespace dgami {
 constexpr int MAP_STRIDE = BLK_MAP_STRIDE;  // interchange map per problem: count, dst[NB], src[NB], dest[NB]
 
 // ---------------------------------------------------------------------------
+// grid (ceil(n / ASM_ROWS), batch), 256 threads: ASM_ROWS rows of one problem per workgroup.
+constexpr int ASM_ROWS = 4;
+
 template <typename S>
-__global__ void blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0, int batch) {
-  const size_t per = static_cast<size_t>(st.n) * st.ld;
-  const size_t total = per * batch;
-  for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
-       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
-    const int b = static_cast<int>(idx / per);
-    const int rem = static_cast<int>(idx - static_cast<size_t>(b) * per);
-    const int i = rem / st.ld, j = rem - (rem / st.ld) * st.ld;
-    const int item = item0 + b;
-    const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
-    double2 val = make_double2(0.0, 0.0);
-    if (j < st.n) {
-      const double2* G = ladder_matrix_ptr(a, chan, w, q);
-      const double2* U = a.dense ? nullptr : a.u_r + chan * a.T * a.T;
-      const double2* V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
-      val = ladder_elem(a, G, U, V, w, q, i, j);
-    } else if (j < st.ncol) {
-      val = ladder_rhs(a, i, j - st.n);
+__global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0) {
+  const int b = blockIdx.y;
+  const int item = item0 + b;
+  const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
+  const double2* G = ladder_matrix_ptr(a, chan, w, q);
+  const double2* U = a.dense ? nullptr : a.u_r + chan * a.T * a.T;
+  const double2* V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
+  S* Wb = st.W + static_cast<size_t>(b) * st.n * st.ld;
+  const int r1 = min(st.n, (blockIdx.x + 1) * ASM_ROWS);
+  for (int i = blockIdx.x * ASM_ROWS; i < r1; ++i) {
+    for (int j = threadIdx.x; j < st.ld; j += 256) {
+      double2 val = make_double2(0.0, 0.0);
+      if (j < st.n)
+        val = ladder_elem(a, G, U, V, w, q, i, j);
+      else if (j < st.ncol)
+        val = ladder_rhs(a, i, j - st.n);
+      Wb[static_cast<size_t>(i) * st.ld + j] = from_c128<S>(val);
     }
-    st.W[idx] = from_c128<S>(val);
   }
-  if (blockIdx.x == 0)
-    for (int b = threadIdx.x; b < batch; b += blockDim.x) st.info[b] = 0;
+  if (blockIdx.x == 0 && threadIdx.x == 0) st.info[b] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -348,69 +351,131 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
   }
 }
 
-__device__ __forceinline__ double readlane_dbl(double v, int lane) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane(static_cast<int>(bits & 0xffffffffLL), lane);
-  const int hi = __builtin_amdgcn_readlane(static_cast<int>(bits >> 32), lane);
-  return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
-}
+// Same update on the matrix cores: each wavefront owns a 32x32 quarter of the tile as 2x2
+// blocks of 16x16, and a complex product becomes two real MFMA chains over the interleaved
+// k index (re/im): C_re += [a_re, -a_im] . [b_re; b_im], C_im += [a_im, a_re] . [b_re; b_im].
+// Operand layout of the 16x16x4 MFMA (lane l): a = A[l%16][l/16], b = B[l/16][l%16],
+// d[e] = D[4*(l/16)+e][l%16].
+template <typename S>
+struct Mfma16x16x4;
+template <>
+struct Mfma16x16x4<double2> {
+  using R = double;
+  typedef double V4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V4 op(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+};
+template <>
+struct Mfma16x16x4<float2> {
+  using R = float;
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  static __device__ __forceinline__ V4 op(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+};
 
-// ---------------------------------------------------------------------------
-// back substitution U x = y on the carried right-hand-side columns (FP64), then output / beta
-__global__ void __launch_bounds__(256) blk_backsub_kernel(LadderArgs a, BlockedState<double2> st, int item0) {
-  constexpr int CB = 32;
-  const int b = blockIdx.x;
-  const int n = st.n, ld = st.ld, T = a.T;
-  double2* W = st.W + static_cast<size_t>(b) * n * ld;
-  const double2* dinv = st.dinv + static_cast<size_t>(b) * n;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  for (int c1 = n; c1 > 0; c1 -= CB) {
-    const int c0 = c1 > CB ? c1 - CB : 0;
-    if (wave == 0) {
-      const int row = c0 + lane;
-      const bool live = lane < c1 - c0;
-      const double2 di = live ? dinv[row] : make_double2(0.0, 0.0);
-      for (int r = 0; r < T; ++r) {
-        double2 y = live ? W[static_cast<size_t>(row) * ld + n + r] : make_double2(0.0, 0.0);
+template <typename S, int NB>
+__global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c) {
+  using M = Mfma16x16x4<S>;
+  using R = typename M::R;
+  using V4 = typename M::V4;
+  constexpr int TILE = 64;
+  const int per = tiles_r * tiles_c;
+  const int b = blockIdx.x / per;
+  const int t = blockIdx.x - b * per;
+  const int tr = t / tiles_c, tcl = t - (t / tiles_c) * tiles_c;
+  const int n = st.n, ld = st.ld, ncol = st.ncol;
+  const int k0 = p0 + NB;
+  S* W = st.W + static_cast<size_t>(b) * n * ld;
+  const int r0 = k0 + tr * TILE, c0 = k0 + tcl * TILE;
+  __shared__ S s_a[TILE][NB + 1];
+  __shared__ S s_b[NB][TILE + 1];
+  const int tid = threadIdx.x;
+  for (int e = tid; e < TILE * NB; e += 256) {
+    const int r = e / NB, s = e - (e / NB) * NB;
+    s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : czero<S>();
+  }
+  for (int e = tid; e < TILE * NB; e += 256) {
+    const int s = e / TILE, c = e - (e / TILE) * TILE;
+    s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : czero<S>();
+  }
+  __syncthreads();
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int l16 = lane & 15, kq = lane >> 4, kc = kq >> 1;
+  const bool odd = (kq & 1) != 0;
+  V4 accr[2][2], acci[2][2];
 #pragma unroll
-        for (int t = CB - 1; t >= 0; --t) {
-          if (c0 + t >= c1) continue;
-          if (lane == t) y = cmul(y, di);
-          const double2 x = make_double2(readlane_dbl(y.x, t), readlane_dbl(y.y, t));
-          if (lane < t) y = cfms(y, W[static_cast<size_t>(row) * ld + c0 + t], x);
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accr[i][j] = V4{0, 0, 0, 0};
+      acci[i][j] = V4{0, 0, 0, 0};
+    }
+#pragma unroll
+  for (int kk = 0; kk < NB; kk += 2) {
+    const int k = kk + kc;
+    R ar[2], ai[2], bv[2];
+#pragma unroll
+    for (int i = 0; i < 2; ++i) {
+      const S v = s_a[wr + 16 * i + l16][k];
+      ar[i] = odd ? -v.y : v.x;
+      ai[i] = odd ? v.x : v.y;
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const S v = s_b[k][wc + 16 * j + l16];
+      bv[j] = odd ? v.y : v.x;
+    }
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        accr[i][j] = M::op(ar[i], bv[j], accr[i][j]);
+        acci[i][j] = M::op(ai[i], bv[j], acci[i][j]);
+      }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = c0 + wc + 16 * j + l16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = r0 + wr + 16 * i + 4 * kq + e;
+        if (r < n && c < ncol) {
+          S* p = W + static_cast<size_t>(r) * ld + c;
+          S v = *p;
+          v.x -= accr[i][j][e];
+          v.y -= acci[i][j][e];
+          *p = v;
         }
-        if (live) W[static_cast<size_t>(row) * ld + n + r] = y;
       }
     }
-    __syncthreads();
-    for (int idx = tid; idx < c0 * T; idx += 256) {
-      const int i = idx / T, r = idx - (idx / T) * T;
-      const double2* urow = W + static_cast<size_t>(i) * ld;
-      double2 acc = urow[n + r];
-      for (int t = c0; t < c1; ++t) acc = cfms(acc, urow[t], W[static_cast<size_t>(t) * ld + n + r]);
-      W[static_cast<size_t>(i) * ld + n + r] = acc;
-    }
-    __syncthreads();
-  }
-  const int item = item0 + b;
-  const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
-  for (int idx = tid; idx < n * T; idx += 256) {
-    const int i = idx / T, r = idx - (idx / T) * T;
-    a.out[ladder_out_index(a, chan, w, q, i, r)] = cscale(W[static_cast<size_t>(i) * ld + n + r], a.inv_beta);
-  }
-  if (tid == 0 && a.info) a.info[item] = st.info[b];
+}
+
+__global__ void blk_info_kernel(const int* __restrict__ st_info, int32_t* __restrict__ info, int item0, int nitems) {
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < nitems) info[item0 + b] = st_info[b];
+}
+
+int blocked_copy_info(const int* st_info, int32_t* info, int item0, int nitems, hipStream_t stream) {
+  if (!info) return 0;
+  hipLaunchKernelGGL(blk_info_kernel, dim3(ceil_div(nitems, 256)), dim3(256), 0, stream, st_info, info, item0, nitems);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 // ---------------------------------------------------------------------------
-static unsigned grid_for_total(size_t total) {
-  return ceil_div(total, 256) > 262144u ? 262144u : ceil_div(total, 256);
+// Trailing updates run on the matrix cores; DGAMI_BLOCKED_GEMM=vector selects the
+// vector-FMA tile kernel instead (for comparisons).
+static bool use_vector_gemm() {
+  const char* v = getenv("DGAMI_BLOCKED_GEMM");
+  return v && std::string(v) == "vector";
 }
 
 template <typename S, int NB, int RPT, int MAXTH>
 static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream) {
   const int n = st.n, ld = st.ld, ncol = st.ncol;
-  const size_t tot = static_cast<size_t>(nb) * n * ld;
-  hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(grid_for_total(tot)), dim3(256), 0, stream, a, st, item0, nb);
+  DGAMI_ARG_CHECK(nb <= 65535, "ladder: batch too large");
+  hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(ceil_div(n, ASM_ROWS), nb), dim3(256), 0, stream, a, st, item0);
   DGAMI_HIP_CHECK(hipGetLastError());
   for (int p0 = 0; p0 < n; p0 += NB) {
     const int rows = n - p0;
@@ -426,8 +491,12 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
       const int tr = ceil_div(n - k1, 64), tc = ceil_div(ncol - k1, 64);
       const size_t g = static_cast<size_t>(nb) * tr * tc;
       DGAMI_ARG_CHECK(g < (size_t(1) << 31), "ladder: trailing update grid too large");
-      hipLaunchKernelGGL((blk_gemm_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr,
-                         tc);
+      if (use_vector_gemm())
+        hipLaunchKernelGGL((blk_gemm_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr,
+                           tc);
+      else
+        hipLaunchKernelGGL((blk_gemm_mfma_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0,
+                           tr, tc);
       DGAMI_HIP_CHECK(hipGetLastError());
     }
   }
@@ -459,12 +528,22 @@ int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream) {
   if (batch > 65535) batch = 65535;
   BlockedState<double2> st;
   if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
+  const int nblk = static_cast<int>(ceil_div(n, TRI_B));
+  double2* tri = static_cast<double2*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(double2)));
+  if (!tri) return -2;
   for (size_t item0 = 0; item0 < items; item0 += batch) {
     const int nb = static_cast<int>(items - item0 < batch ? items - item0 : batch);
     int rc = blocked_factor<double2>(a, st, static_cast<int>(item0), nb, stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(blk_backsub_kernel, dim3(nb), dim3(256), 0, stream, a, st, static_cast<int>(item0));
-    DGAMI_HIP_CHECK(hipGetLastError());
+    // back substitution on the carried right-hand sides, x / beta straight to the output
+    TriFactors<double2> f{st.W, tri, st.dinv, n, ld, nblk};
+    rc = tri_diag_inverse(f, nb, false, stream);
+    if (rc) return rc;
+    TriVec<double2> y{st.W + n, static_cast<size_t>(n) * ld, ld};
+    rc = tri_backward(f, nb, a.T, y, TriVec<double2>{nullptr, 0, 0}, &a, static_cast<int>(item0), stream);
+    if (rc) return rc;
+    rc = blocked_copy_info(st.info, a.info, static_cast<int>(item0), nb, stream);
+    if (rc) return rc;
   }
   return 0;
 }

@@ -42,4 +42,7 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
 template <typename S>
 int blocked_factor(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream);
 
+// info[item0 + b] = first zero pivot of problem b of the batch (no-op for info == nullptr).
+int blocked_copy_info(const int* st_info, int32_t* info, int item0, int nitems, hipStream_t stream);
+
 }  // namespace dgami

@@ -8,7 +8,7 @@
 //                    formed for all q of one (chan, w) as a single complex128 GEMM
 //                    (Gamma is shared by the q's; the chi0^-1 blocks and the rank-nb^2
 //                    U_r + V_r term are applied per item);
-//   D = (LU)^-1 P R  in FP32 (triangular solves on LDS-resident right-hand sides);
+//   D = (LU)^-1 P R  in FP32 (blocked triangular sweeps with inverse diagonal blocks);
 //   X += D           in FP64.
 // Starting from X = 0 (R = B), MIXED_SOLVES solves are made. Each refinement step
 // multiplies the error by about cond(A) * 2^-24, so two refinements bring the result
@@ -20,13 +20,12 @@
 #include "dgami_launch.hpp"
 #include "ladder_blocked.hpp"
 #include "ladder_common.hpp"
+#include "tri_solve.hpp"
 #include "zgemm.hpp"
 
 namespace dgami {
 
 constexpr int MIXED_SOLVES = 3;
-constexpr int TRI_CB = 32;        // rows per triangular block of the substitutions
-constexpr int SOLVE_THREADS = 256;
 
 // Item layout of the refinement vectors: group g = (chan, w) (or the item itself in
 // dense mode) holds gq items; X and R are [group][n][gq][T] so that Gamma(g) X(g) is
@@ -113,128 +112,41 @@ __global__ void __launch_bounds__(256) mix_residual_kernel(MixedArgs m) {
 }
 
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ float readlane_f(float v, int lane) {
-  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), lane));
-}
-
-// D = U^-1 L^-1 P R in FP32 with R[item] staged in LDS; X += D. On the last solve the
-// result is also written to the output (divided by beta). One workgroup per item.
-template <int T>
-__global__ void __launch_bounds__(SOLVE_THREADS) mix_solve_kernel(MixedArgs m, int first, int last) {
-  extern __shared__ float2 s_y[];  // [n][T]
+// Y = P R in complex64 (R = B on the first solve). grid (ceil(n*T/256), batch).
+__global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __restrict__ Y, int first) {
   const LadderArgs& a = m.a;
-  const int lb = blockIdx.x;
+  const int lb = blockIdx.y;
   const int gl = lb / m.gq, q = lb - gl * m.gq;
-  const int item = m.item0 + lb;
-  const int n = a.n, gq = m.gq, ld = m.ld;
-  const size_t gbase = static_cast<size_t>(gl) * n * gq * T;
-  double2* X = m.X + gbase;
-  const double2* R = m.R + gbase;
-  const float2* W = m.W + static_cast<size_t>(lb) * n * ld;
-  const int* perm = m.perm + static_cast<size_t>(lb) * n;
-  const double2* dinv = m.dinv + static_cast<size_t>(lb) * n;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-
-  for (int idx = tid; idx < n * T; idx += SOLVE_THREADS) {
-    const int i = idx / T, t = idx - (idx / T) * T;
-    const int src = perm[i];
-    s_y[idx] = first ? from_c128<float2>(ladder_rhs(a, src, t))
-                     : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
-  }
-  __syncthreads();
-
-  // forward substitution with the unit lower factor
-  for (int c0 = 0; c0 < n; c0 += TRI_CB) {
-    const int cb = min(TRI_CB, n - c0);
-    if (wave == 0) {
-      const int row = c0 + lane;
-      const bool live = lane < cb;
-      float2 lrow[TRI_CB];
-#pragma unroll
-      for (int s = 0; s < TRI_CB; ++s)
-        lrow[s] = (live && s < lane) ? W[static_cast<size_t>(row) * ld + c0 + s] : make_float2(0.f, 0.f);
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        float2 y = live ? s_y[row * T + t] : make_float2(0.f, 0.f);
-#pragma unroll
-        for (int s = 0; s < TRI_CB; ++s) {
-          const float2 xs = make_float2(readlane_f(y.x, s), readlane_f(y.y, s));
-          if (lane > s) y = cfms(y, lrow[s], xs);
-        }
-        if (live) s_y[row * T + t] = y;
-      }
-    }
-    __syncthreads();
-    for (int i = c0 + cb + tid; i < n; i += SOLVE_THREADS) {
-      float2 acc[T];
-#pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = s_y[i * T + t];
-      const float2* lr = W + static_cast<size_t>(i) * ld + c0;
-      for (int s = 0; s < cb; ++s) {
-        const float2 l = lr[s];
-#pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = cfms(acc[t], l, s_y[(c0 + s) * T + t]);
-      }
-#pragma unroll
-      for (int t = 0; t < T; ++t) s_y[i * T + t] = acc[t];
-    }
-    __syncthreads();
-  }
-
-  // back substitution with the upper factor
-  for (int c1 = n; c1 > 0; c1 -= TRI_CB) {
-    const int c0 = c1 > TRI_CB ? c1 - TRI_CB : 0;
-    const int cb = c1 - c0;
-    if (wave == 0) {
-      const int row = c0 + lane;
-      const bool live = lane < cb;
-      float2 urow[TRI_CB];
-#pragma unroll
-      for (int s = 0; s < TRI_CB; ++s)
-        urow[s] = (live && s > lane && s < cb) ? W[static_cast<size_t>(row) * ld + c0 + s] : make_float2(0.f, 0.f);
-      const float2 di = live ? from_c128<float2>(dinv[row]) : make_float2(0.f, 0.f);
-#pragma unroll
-      for (int t = 0; t < T; ++t) {
-        float2 y = live ? s_y[row * T + t] : make_float2(0.f, 0.f);
-#pragma unroll
-        for (int s = TRI_CB - 1; s >= 0; --s) {
-          if (lane == s) y = cmul(y, di);
-          const float2 xs = make_float2(readlane_f(y.x, s), readlane_f(y.y, s));
-          if (lane < s) y = cfms(y, urow[s], xs);
-        }
-        if (live) s_y[row * T + t] = y;
-      }
-    }
-    __syncthreads();
-    for (int i = tid; i < c0; i += SOLVE_THREADS) {
-      float2 acc[T];
-#pragma unroll
-      for (int t = 0; t < T; ++t) acc[t] = s_y[i * T + t];
-      const float2* ur = W + static_cast<size_t>(i) * ld + c0;
-      for (int s = 0; s < cb; ++s) {
-        const float2 u = ur[s];
-#pragma unroll
-        for (int t = 0; t < T; ++t) acc[t] = cfms(acc[t], u, s_y[(c0 + s) * T + t]);
-      }
-#pragma unroll
-      for (int t = 0; t < T; ++t) s_y[i * T + t] = acc[t];
-    }
-    __syncthreads();
-  }
-
-  const int qt = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
-  for (int idx = tid; idx < n * T; idx += SOLVE_THREADS) {
-    const int i = idx / T, t = idx - (idx / T) * T;
-    const size_t o = (static_cast<size_t>(i) * gq + q) * T + t;
-    const double2 x = first ? to_c128(s_y[idx]) : cadd(X[o], to_c128(s_y[idx]));
-    X[o] = x;
-    if (last) a.out[ladder_out_index(a, chan, w, qt, i, t)] = cscale(x, a.inv_beta);
-  }
+  const int n = a.n, T = a.T, gq = m.gq;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n * T) return;
+  const int i = idx / T, t = idx - (idx / T) * T;
+  const int src = m.perm[static_cast<size_t>(lb) * n + i];
+  const double2* R = m.R + static_cast<size_t>(gl) * n * gq * T;
+  Y[static_cast<size_t>(lb) * n * T + idx] =
+      first ? from_c128<float2>(ladder_rhs(a, src, t)) : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
 }
 
-__global__ void mix_info_kernel(const int* __restrict__ st_info, int32_t* __restrict__ info, int item0, int nitems) {
-  const int b = blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < nitems) info[item0 + b] = st_info[b];
+// X += D (FP64); on the last solve also out = X / beta.
+__global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const float2* __restrict__ D, int first,
+                                                             int last) {
+  const LadderArgs& a = m.a;
+  const int lb = blockIdx.y;
+  const int gl = lb / m.gq, q = lb - gl * m.gq;
+  const int n = a.n, T = a.T, gq = m.gq;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n * T) return;
+  const int i = idx / T, t = idx - (idx / T) * T;
+  double2* X = m.X + static_cast<size_t>(gl) * n * gq * T;
+  const size_t o = (static_cast<size_t>(i) * gq + q) * T + t;
+  const double2 d = to_c128(D[static_cast<size_t>(lb) * n * T + idx]);
+  const double2 x = first ? d : cadd(X[o], d);
+  X[o] = x;
+  if (last) {
+    const int item = m.item0 + lb;
+    const int qt = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
+    a.out[ladder_out_index(a, chan, w, qt, i, t)] = cscale(x, a.inv_beta);
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -244,14 +156,15 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   const int n = a.n, T = a.T;
   DGAMI_ARG_CHECK(T <= 16, "ladder: at most 4 orbitals in the mixed-precision form");
   DGAMI_ARG_CHECK(n <= 1536, "ladder: compound size above 1536 is not supported");
-  const size_t lds = static_cast<size_t>(n) * T * sizeof(float2);
-  DGAMI_ARG_CHECK(lds <= 128 * 1024, "ladder: compound size times right-hand sides too large for the LDS solve");
   const int ld = (n + 7) & ~7;
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const int gq = a.dense ? 1 : a.nq;
   const size_t groups = items / gq;
-  // per item: FP32 factors + FP64 X and R + permutation
-  const size_t per_item = static_cast<size_t>(n) * ld * sizeof(float2) + 2 * static_cast<size_t>(n) * T * sizeof(double2) +
+  const int nblk = static_cast<int>(ceil_div(n, TRI_B));
+  // per item: FP32 factors, their inverse diagonal blocks, FP64 X and R, FP32 sweep vectors, permutation
+  const size_t per_item = static_cast<size_t>(n) * ld * sizeof(float2) +
+                          static_cast<size_t>(nblk) * 2 * TRI_B * TRI_B * sizeof(float2) +
+                          2 * static_cast<size_t>(n) * T * sizeof(double2) + 2 * static_cast<size_t>(n) * T * sizeof(float2) +
                           static_cast<size_t>(n) * sizeof(int);
   size_t gb = (size_t(48) << 30) / (per_item * gq);
   gb = std::max<size_t>(1, std::min(gb, groups));
@@ -274,15 +187,11 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   m.dinv = st.dinv;
   m.ld = ld;
   m.gq = gq;
-  void (*solve)(MixedArgs, int, int) = nullptr;
-  switch (T) {
-    case 1: solve = mix_solve_kernel<1>; break;
-    case 4: solve = mix_solve_kernel<4>; break;
-    case 9: solve = mix_solve_kernel<9>; break;
-    case 16: solve = mix_solve_kernel<16>; break;
-    default: DGAMI_ARG_CHECK(false, "ladder: nb^2 must be 1, 4, 9 or 16");
-  }
-  allow_lds(solve, lds);
+  float2* tri = static_cast<float2*>(workspace(WS_TRI, bitems * nblk * 2 * TRI_B * TRI_B * sizeof(float2)));
+  float2* yv = static_cast<float2*>(workspace(WS_TRI_VEC, 2 * bitems * n * T * sizeof(float2)));
+  if (!tri || !yv) return -2;
+  float2* xv = yv + bitems * n * T;
+  const TriVec<float2> Yv{yv, static_cast<size_t>(n) * T, T}, Xv{xv, static_cast<size_t>(n) * T, T};
 
   for (size_t g0 = 0; g0 < groups; g0 += gb) {
     const int ng = static_cast<int>(std::min(gb, groups - g0));
@@ -294,6 +203,10 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     if (rc) return rc;
     hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), stream, st.piv, perm, n);
     DGAMI_HIP_CHECK(hipGetLastError());
+    const TriFactors<float2> f{st.W, tri, st.dinv, n, ld, nblk};
+    rc = tri_diag_inverse(f, nit, true, stream);
+    if (rc) return rc;
+    const dim3 vgrid(ceil_div(static_cast<size_t>(n) * T, 256), nit);
     for (int s = 0; s < MIXED_SOLVES; ++s) {
       if (s > 0) {
         ZgemmArgs g;
@@ -316,14 +229,19 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, stream, m);
         DGAMI_HIP_CHECK(hipGetLastError());
       }
-      hipLaunchKernelGGL(solve, dim3(nit), dim3(SOLVE_THREADS), lds, stream, m, s == 0 ? 1 : 0,
+      // D = U^-1 L^-1 P R in complex64, X += D
+      hipLaunchKernelGGL(mix_gather_kernel, vgrid, dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0);
+      DGAMI_HIP_CHECK(hipGetLastError());
+      rc = tri_forward(f, nit, T, Yv, Xv, stream);
+      if (rc) return rc;
+      rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
+      if (rc) return rc;
+      hipLaunchKernelGGL(mix_accumulate_kernel, vgrid, dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0,
                          s == MIXED_SOLVES - 1 ? 1 : 0);
       DGAMI_HIP_CHECK(hipGetLastError());
     }
-    if (a.info) {
-      hipLaunchKernelGGL(mix_info_kernel, dim3(ceil_div(nit, 256)), dim3(256), 0, stream, st.info, a.info, item0, nit);
-      DGAMI_HIP_CHECK(hipGetLastError());
-    }
+    rc = blocked_copy_info(st.info, a.info, item0, nit, stream);
+    if (rc) return rc;
   }
   return 0;
 }

@@ -133,10 +133,13 @@ def _run_ladder(K, chi0_inv, gammas, us, vs, beta, precision=0):
     (2, 20, 2, 3),    # n = 160 (blocked LU path)
     (3, 12, 1, 2),    # n = 216 (blocked)
 ])
-@pytest.mark.parametrize("family", ["auto", "reg", "blocked"])
+@pytest.mark.parametrize("family", ["auto", "reg", "blocked", "blocked_vector"])
 def test_ladder_plain_fp64(K, nb, niv, niw, nq, family, monkeypatch):
     say(f"ladder fp64 nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv} kernel={family}")
-    if family != "auto":
+    if family == "blocked_vector":
+        monkeypatch.setenv("DGAMI_LADDER_KERNEL", "blocked")
+        monkeypatch.setenv("DGAMI_BLOCKED_GEMM", "vector")
+    elif family != "auto":
         monkeypatch.setenv("DGAMI_LADDER_KERNEL", family)
     chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=nb * 100 + niv)
     ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
