@@ -42,7 +42,25 @@ def parse():
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a 1 GiB device-to-device copy (3 times), the reference kernel for scaling "
+                         "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
+    ap.add_argument("--ladder-only", action="store_true",
+                    help="after set-up and one warm-up step, run only the batched ladder solve (for counter runs)")
     return ap.parse_args()
+
+
+def copy_calibration():
+    import torch
+
+    n = (1 << 30) // 16
+    a = torch.ones(n, dtype=torch.complex128, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": n * 16, "copy_bytes_written": n * 16, "copies": 3}), flush=True)
 
 
 def cpu_baseline(model, target_s):
@@ -120,6 +138,9 @@ def main():
         print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); the multi-GPU sharding of the q-work "
               "list is described in the design notes but not run.", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
     import torch
 
     from dgamore_amd import _lib
@@ -134,6 +155,12 @@ def main():
     print(f"[bench] config {args.config} ({model.name}) nq_irr={pipe.nq} niw={model.niw} niv={model.niv} "
           f"n={n} set-up {time.perf_counter() - t_setup:.1f}s", file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()
+    if args.ladder_only:
+        pipe.step(keep=True)
+        torch.cuda.synchronize()
+        ms = time_ladder(pipe, 1, stream)
+        print(json.dumps({"ladder_ms": ms, "solves": pipe.solves_per_step(), "precision": args.precision}), flush=True)
+        return
 
     ms_step = time_steps(pipe, args.steps, args.warmup, stream)
     print(f"[bench] {args.precision}: {args.steps} steps, {ms_step:.3f} ms/step", file=sys.stderr, flush=True)

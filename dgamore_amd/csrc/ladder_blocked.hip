@@ -62,11 +62,16 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
 // Panel factorisation: thread t owns rows p0 + t + blockDim*m (m < RPT) of columns [p0, p0+NB).
 // Each column is one instance of panel_column<C>, expanded at compile time so that every
 // register index is a constant.
+// One barrier per column: before it, every wavefront publishes its local pivot candidate
+// (value, row index and the whole candidate row) and the owner of the diagonal row
+// publishes that row; after it, every thread picks the global winner from the nwave
+// candidates and applies the interchange and the rank-1 update from LDS. The LDS
+// slots alternate with the column parity, so one barrier per column suffices.
 template <typename S, int NB>
 struct PanelShared {
   double val[2][16];
   int idx[2][16];
-  S prow[2][NB];
+  S cand[2][16][NB];
   S krow[2][NB];
   int piv[NB];
   int mdst[2 * NB], msrc[2 * NB];
@@ -100,44 +105,46 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
       bidx = oi;
     }
   }
+  // publish this wavefront's candidate row
+  if constexpr (RPT == 1) {
+    if (p0 + tid == bidx) {
+#pragma unroll
+      for (int cc = 0; cc < NB; ++cc) sh.cand[buf][wave][cc] = P[0][cc];
+    }
+  } else {
+#pragma unroll
+    for (int m = 0; m < RPT; ++m)
+      if (p0 + tid + TH * m == bidx) {
+#pragma unroll
+        for (int cc = 0; cc < NB; ++cc) sh.cand[buf][wave][cc] = P[m][cc];
+      }
+  }
   if (lane == 0) {
     sh.val[buf][wave] = best;
     sh.idx[buf][wave] = bidx;
   }
+  if (tid == C) {  // row p0 + C is the first register row of thread C (C < NB <= 64)
+#pragma unroll
+    for (int cc = 0; cc < NB; ++cc) sh.krow[buf][cc] = P[0][cc];
+  }
   __syncthreads();
   best = sh.val[buf][0];
   bidx = sh.idx[buf][0];
+  int bw = 0;
   for (int wv = 1; wv < nwave; ++wv) {
     const double ov = sh.val[buf][wv];
     const int oi = sh.idx[buf][wv];
     if (ov > best || (ov == best && oi < bidx)) {
       best = ov;
       bidx = oi;
+      bw = wv;
     }
   }
   const int p = bidx;
-  constexpr int mk = 0;  // column C's diagonal row p0 + C lies in the first register row (C < NB <= TH)
-  if constexpr (RPT == 1) {
-    if (tid == p - p0) {
-#pragma unroll
-      for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[0][cc];
-    }
-  } else {
-#pragma unroll
-    for (int m = 0; m < RPT; ++m)
-      if (p0 + tid + TH * m == p) {
-#pragma unroll
-        for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[m][cc];
-      }
-  }
-  if (p != col && tid == C) {
-#pragma unroll
-    for (int cc = 0; cc < NB; ++cc) sh.krow[buf][cc] = P[mk][cc];
-  }
+  const S* prow = sh.cand[buf][bw];
   if (tid == 0) sh.piv[C] = p;
-  __syncthreads();
   const bool ok = best > 0.0;
-  const S pinv = ok ? cinv(sh.prow[buf][C]) : czero<S>();
+  const S pinv = ok ? cinv(prow[C]) : czero<S>();
   if (tid == 0) {
     st.dinv[static_cast<size_t>(b) * n + col] = to_c128(pinv);
     if (!ok && st.info[b] == 0) st.info[b] = col + 1;
@@ -147,7 +154,7 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
     const int r = p0 + tid + TH * m;
     if (r == col) {
 #pragma unroll
-      for (int cc = 0; cc < NB; ++cc) P[m][cc] = sh.prow[buf][cc];
+      for (int cc = 0; cc < NB; ++cc) P[m][cc] = prow[cc];
     } else if (r > col && r < n) {
       if (r == p) {
 #pragma unroll
@@ -156,7 +163,7 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
       const S l = cmul(P[m][C], pinv);
       P[m][C] = l;
 #pragma unroll
-      for (int cc = C + 1; cc < NB; ++cc) P[m][cc] = cfms(P[m][cc], l, sh.prow[buf][cc]);
+      for (int cc = C + 1; cc < NB; ++cc) P[m][cc] = cfms(P[m][cc], l, prow[cc]);
     }
   }
 }
