@@ -25,6 +25,7 @@
 
 #include <cstdlib>
 #include <string>
+#include <type_traits>
 #include <utility>
 
 namespace dgami {
@@ -366,12 +367,6 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
 template <typename S>
 struct Mfma16x16x4;
 template <>
-struct Mfma16x16x4<double2> {
-  using R = double;
-  typedef double V4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ V4 op(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
-};
-template <>
 struct Mfma16x16x4<float2> {
   using R = float;
   typedef float V4 __attribute__((ext_vector_type(4)));
@@ -471,8 +466,11 @@ int blocked_copy_info(const int* st_info, int32_t* info, int item0, int nitems, 
 }
 
 // ---------------------------------------------------------------------------
-// Trailing updates run on the matrix cores; DGAMI_BLOCKED_GEMM=vector selects the
-// vector-FMA tile kernel instead (for comparisons).
+// Complex64 trailing updates run on the FP32 matrix cores; DGAMI_BLOCKED_GEMM=vector selects
+// the vector-FMA tile kernel instead (for comparisons). Complex128 updates always use the
+// vector-FMA kernel: the FP64 MFMA (v_mfma_f64_16x16x4_f64) returned wrong products on
+// gfx950 when the compiler rewrote its source registers right after issue, which source
+// code cannot prevent.
 static bool use_vector_gemm() {
   const char* v = getenv("DGAMI_BLOCKED_GEMM");
   return v && std::string(v) == "vector";
@@ -498,12 +496,15 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
       const int tr = ceil_div(n - k1, 64), tc = ceil_div(ncol - k1, 64);
       const size_t g = static_cast<size_t>(nb) * tr * tc;
       DGAMI_ARG_CHECK(g < (size_t(1) << 31), "ladder: trailing update grid too large");
-      if (use_vector_gemm())
+      bool vector = true;
+      if constexpr (std::is_same<S, float2>::value) vector = use_vector_gemm();
+      if (vector) {
         hipLaunchKernelGGL((blk_gemm_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr,
                            tc);
-      else
+      } else if constexpr (std::is_same<S, float2>::value) {
         hipLaunchKernelGGL((blk_gemm_mfma_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0,
                            tr, tc);
+      }
       DGAMI_HIP_CHECK(hipGetLastError());
     }
   }

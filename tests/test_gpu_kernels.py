@@ -174,8 +174,11 @@ MIXED_RTOL = 1e-8
     (3, 10, 2, 3),    # n = 180, 9 right-hand sides
     (3, 40, 1, 2),    # n = 720, the 3-orbital box
 ])
-def test_ladder_mixed_fp32_ir(K, nb, niv, niw, nq):
-    say(f"ladder mixed fp32+IR nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv}")
+@pytest.mark.parametrize("gemm", ["mfma", "vector"])
+def test_ladder_mixed_fp32_ir(K, nb, niv, niw, nq, gemm, monkeypatch):
+    say(f"ladder mixed fp32+IR nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv} trailing update={gemm}")
+    if gemm == "vector":
+        monkeypatch.setenv("DGAMI_BLOCKED_GEMM", "vector")
     chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=nb * 100 + niv + 1)
     ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
     got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=1)
