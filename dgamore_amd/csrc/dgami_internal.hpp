@@ -105,6 +105,24 @@ __host__ __device__ __forceinline__ double2 to_c128(float2 v) {
 }
 
 // ----------------------------------------------------------------------------
+// XCD-aware tile order. Workgroups are dispatched round-robin over MI355X's 8 XCDs, each
+// with its own L2. With a grid of xcd_grid(nprob, per) workgroups, xcd_tile maps the
+// linear id so that all `per` tiles of one problem run on the same XCD and share its L2
+// copy of the problem's operand panels. Returns false for the padding ids.
+// ----------------------------------------------------------------------------
+constexpr int NUM_XCD = 8;
+
+inline size_t xcd_grid(size_t nprob, size_t per) { return (nprob + NUM_XCD - 1) / NUM_XCD * NUM_XCD * per; }
+
+__device__ __forceinline__ bool xcd_tile(unsigned id, int nprob, int per, int& prob, int& tile) {
+  const unsigned x = id % NUM_XCD, slot = id / NUM_XCD;
+  const unsigned j = slot / per;
+  tile = static_cast<int>(slot - j * per);
+  prob = static_cast<int>(j * NUM_XCD + x);
+  return prob < nprob;
+}
+
+// ----------------------------------------------------------------------------
 // error reporting (thread-local last error; host side)
 // ----------------------------------------------------------------------------
 void set_error(const std::string& msg);

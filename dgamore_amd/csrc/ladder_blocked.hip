@@ -251,11 +251,11 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
 
 // ---------------------------------------------------------------------------
 // Interchanges for every column outside the panel, then U12 = L11^-1 A12 on trailing columns.
-// grid: batch * ntile, 64 threads; tile t covers columns t*64 .. t*64+63 of [0, ncol).
+// grid: xcd_grid(batch, ntile), 64 threads; tile t covers columns t*64 .. t*64+63 of [0, ncol).
 template <typename S, int NB>
-__global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0, int ntile) {
-  const int b = blockIdx.x / ntile;
-  const int tile = blockIdx.x - b * ntile;
+__global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0, int ntile, int nprob) {
+  int b, tile;
+  if (!xcd_tile(blockIdx.x, nprob, ntile, b, tile)) return;
   const int n = st.n, ld = st.ld;
   S* W = st.W + static_cast<size_t>(b) * n * ld;
   const int nbw = (n - p0 < NB) ? n - p0 : NB;
@@ -304,11 +304,12 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
 // ---------------------------------------------------------------------------
 // A22 -= L21 U12 (rows [p0+NB, n), columns [p0+NB, ncol)), 64x64 tiles, 256 threads.
 template <typename S, int NB>
-__global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c) {
+__global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c,
+                                                       int nprob) {
   constexpr int TILE = 64;
   const int per = tiles_r * tiles_c;
-  const int b = blockIdx.x / per;
-  const int t = blockIdx.x - b * per;
+  int b, t;
+  if (!xcd_tile(blockIdx.x, nprob, per, b, t)) return;
   const int tr = t / tiles_c, tcl = t - (t / tiles_c) * tiles_c;
   const int n = st.n, ld = st.ld, ncol = st.ncol;
   const int k0 = p0 + NB;
@@ -317,6 +318,16 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
   __shared__ S s_a[TILE][NB + 1];
   __shared__ S s_b[NB][TILE + 1];
   const int tid = threadIdx.x;
+  const int ty = tid / 16, tx = tid % 16;
+  // the C tile is loaded first so that its latency overlaps the operand staging
+  S acc[4][4];
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int r = r0 + ty + 16 * i, c = c0 + tx + 16 * j;
+      acc[i][j] = (r < n && c < ncol) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
+    }
   for (int e = tid; e < TILE * NB; e += 256) {
     const int r = e / NB, s = e - (e / NB) * NB;
     s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : czero<S>();
@@ -326,12 +337,6 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
     s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : czero<S>();
   }
   __syncthreads();
-  const int ty = tid / 16, tx = tid % 16;
-  S acc[4][4];
-#pragma unroll
-  for (int i = 0; i < 4; ++i)
-#pragma unroll
-    for (int j = 0; j < 4; ++j) acc[i][j] = czero<S>();
 #pragma unroll 4
   for (int s = 0; s < NB; ++s) {
     S av[4], bv[4];
@@ -342,7 +347,7 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
 #pragma unroll
     for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = cfma(acc[i][j], av[i], bv[j]);
+      for (int j = 0; j < 4; ++j) acc[i][j] = cfms(acc[i][j], av[i], bv[j]);
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
@@ -351,17 +356,14 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int c = c0 + tx + 16 * j;
-      if (c < ncol) {
-        S* e = W + static_cast<size_t>(r) * ld + c;
-        *e = csub(*e, acc[i][j]);
-      }
+      if (c < ncol) W[static_cast<size_t>(r) * ld + c] = acc[i][j];
     }
   }
 }
 
 // Same update on the matrix cores: each wavefront owns a 32x32 quarter of the tile as 2x2
 // blocks of 16x16, and a complex product becomes two real MFMA chains over the interleaved
-// k index (re/im): C_re += [a_re, -a_im] . [b_re; b_im], C_im += [a_im, a_re] . [b_re; b_im].
+// k index (re/im): C_re -= [a_re, -a_im] . [b_re; b_im], C_im -= [a_im, a_re] . [b_re; b_im].
 // Operand layout of the 16x16x4 MFMA (lane l): a = A[l%16][l/16], b = B[l/16][l%16],
 // d[e] = D[4*(l/16)+e][l%16].
 template <typename S>
@@ -374,14 +376,15 @@ struct Mfma16x16x4<float2> {
 };
 
 template <typename S, int NB>
-__global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c) {
+__global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c,
+                                                            int nprob) {
   using M = Mfma16x16x4<S>;
   using R = typename M::R;
   using V4 = typename M::V4;
   constexpr int TILE = 64;
   const int per = tiles_r * tiles_c;
-  const int b = blockIdx.x / per;
-  const int t = blockIdx.x - b * per;
+  int b, t;
+  if (!xcd_tile(blockIdx.x, nprob, per, b, t)) return;
   const int tr = t / tiles_c, tcl = t - (t / tiles_c) * tiles_c;
   const int n = st.n, ld = st.ld, ncol = st.ncol;
   const int k0 = p0 + NB;
@@ -390,6 +393,26 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, 
   __shared__ S s_a[TILE][NB + 1];
   __shared__ S s_b[NB][TILE + 1];
   const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int l16 = lane & 15, kq = lane >> 4, kc = kq >> 1;
+  const bool odd = (kq & 1) != 0;
+  // accumulators start from the C tile (loaded first, overlapping the operand staging);
+  // the A operand enters negated, so the chains compute C - A B
+  V4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = c0 + wc + 16 * j + l16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = r0 + wr + 16 * i + 4 * kq + e;
+        const S v = (r < n && c < ncol) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
+        accr[i][j][e] = v.x;
+        acci[i][j][e] = v.y;
+      }
+    }
   for (int e = tid; e < TILE * NB; e += 256) {
     const int r = e / NB, s = e - (e / NB) * NB;
     s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : czero<S>();
@@ -399,18 +422,6 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, 
     s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : czero<S>();
   }
   __syncthreads();
-  const int wave = tid >> 6, lane = tid & 63;
-  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int l16 = lane & 15, kq = lane >> 4, kc = kq >> 1;
-  const bool odd = (kq & 1) != 0;
-  V4 accr[2][2], acci[2][2];
-#pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      accr[i][j] = V4{0, 0, 0, 0};
-      acci[i][j] = V4{0, 0, 0, 0};
-    }
 #pragma unroll
   for (int kk = 0; kk < NB; kk += 2) {
     const int k = kk + kc;
@@ -418,8 +429,8 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, 
 #pragma unroll
     for (int i = 0; i < 2; ++i) {
       const S v = s_a[wr + 16 * i + l16][k];
-      ar[i] = odd ? -v.y : v.x;
-      ai[i] = odd ? v.x : v.y;
+      ar[i] = odd ? v.y : -v.x;
+      ai[i] = odd ? -v.x : -v.y;
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
@@ -443,11 +454,10 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, 
       for (int e = 0; e < 4; ++e) {
         const int r = r0 + wr + 16 * i + 4 * kq + e;
         if (r < n && c < ncol) {
-          S* p = W + static_cast<size_t>(r) * ld + c;
-          S v = *p;
-          v.x -= accr[i][j][e];
-          v.y -= acci[i][j][e];
-          *p = v;
+          S v;
+          v.x = accr[i][j][e];
+          v.y = acci[i][j][e];
+          W[static_cast<size_t>(r) * ld + c] = v;
         }
       }
     }
@@ -489,21 +499,22 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
     hipLaunchKernelGGL((blk_panel_kernel<S, NB, RPT, MAXTH>), dim3(nb), dim3(th), 0, stream, st, p0);
     DGAMI_HIP_CHECK(hipGetLastError());
     const int ntile = ceil_div(ncol, 64);
-    hipLaunchKernelGGL((blk_rows_kernel<S, NB>), dim3(nb * ntile), dim3(64), 0, stream, st, p0, ntile);
+    hipLaunchKernelGGL((blk_rows_kernel<S, NB>), dim3(static_cast<unsigned>(xcd_grid(nb, ntile))), dim3(64), 0, stream,
+                       st, p0, ntile, nb);
     DGAMI_HIP_CHECK(hipGetLastError());
     const int k1 = p0 + NB;
     if (k1 < n) {
       const int tr = ceil_div(n - k1, 64), tc = ceil_div(ncol - k1, 64);
-      const size_t g = static_cast<size_t>(nb) * tr * tc;
+      const size_t g = xcd_grid(nb, static_cast<size_t>(tr) * tc);
       DGAMI_ARG_CHECK(g < (size_t(1) << 31), "ladder: trailing update grid too large");
       bool vector = true;
       if constexpr (std::is_same<S, float2>::value) vector = use_vector_gemm();
       if (vector) {
         hipLaunchKernelGGL((blk_gemm_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr,
-                           tc);
+                           tc, nb);
       } else if constexpr (std::is_same<S, float2>::value) {
         hipLaunchKernelGGL((blk_gemm_mfma_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0,
-                           tr, tc);
+                           tr, tc, nb);
       }
       DGAMI_HIP_CHECK(hipGetLastError());
     }

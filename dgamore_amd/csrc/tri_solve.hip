@@ -66,69 +66,86 @@ __global__ void __launch_bounds__(64) tri_diag_inverse_kernel(TriFactors<S> f, i
 }
 
 // ---------------------------------------------------------------------------
-// One block step. grid (1 + ntiles, batch), 256 threads. Workgroup 0 writes x_b; workgroup
-// t >= 1 updates row tile t-1 of the rows below (LOWER) or above (upper) block kb.
-template <typename S, int T, bool LOWER, bool TO_LADDER>
-__global__ void __launch_bounds__(256) tri_step_kernel(TriFactors<S> f, TriVec<S> y, TriVec<S> out, LadderArgs la,
-                                                       int item0, int kb) {
-  const int b = blockIdx.y;
-  const int c0 = kb * TRI_B;
-  const int cb = min(TRI_B, f.n - c0);
+// One whole sweep per problem: grid (batch), 256 threads. Thread t keeps rows t + 256 m
+// (m < RPT) of the right-hand sides in registers. For every block b (forward: first to
+// last, backward: last to first): the owners stage y_b in LDS, x_b = D_b^-1 y_b is formed
+// in LDS, the owners of block b keep x_b, and every thread updates its rows below (L) or
+// above (U) block b with its own row of the factor's block column.
+template <typename S, int T, bool LOWER, bool TO_LADDER, int RPT>
+__global__ void __launch_bounds__(256) tri_sweep_kernel(TriFactors<S> f, TriVec<S> y, TriVec<S> out, LadderArgs la,
+                                                        int item0) {
+  const int b = blockIdx.x;
   const int tid = threadIdx.x;
-  __shared__ S s_dinv[TRI_B][TRI_B + 1];
+  const int n = f.n;
+  __shared__ S s_d[TRI_B][TRI_B + 1];
   __shared__ S s_y[TRI_B][T];
   __shared__ S s_x[TRI_B][T];
-  __shared__ S s_f[TRI_TILE][TRI_B + 1];
-  const S* Db = f.dinvblk + (static_cast<size_t>(b) * f.nblk + kb) * 2 * TRI_B * TRI_B + (LOWER ? 0 : TRI_B * TRI_B);
-  S* Y = y.p + static_cast<size_t>(b) * y.item_stride;
-  for (int e = tid; e < TRI_B * TRI_B; e += 256) {
-    const int r = e / TRI_B, c = e - (e / TRI_B) * TRI_B;
-    s_dinv[r][c] = Db[e];
+  const S* F = f.W + static_cast<size_t>(b) * n * f.ld;
+  const S* Y = y.p + static_cast<size_t>(b) * y.item_stride;
+  S yr[RPT][T];
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const int r = tid + 256 * m;
+#pragma unroll
+    for (int t = 0; t < T; ++t) yr[m][t] = r < n ? Y[static_cast<size_t>(r) * y.row_stride + t] : czero<S>();
   }
-  for (int e = tid; e < TRI_B * T; e += 256) {
-    const int i = e / T, t = e - (e / T) * T;
-    s_y[i][t] = i < cb ? Y[static_cast<size_t>(c0 + i) * y.row_stride + t] : czero<S>();
-  }
-  __syncthreads();
-  for (int e = tid; e < TRI_B * T; e += 256) {
-    const int i = e / T, t = e - (e / T) * T;
-    S acc = czero<S>();
-#pragma unroll 8
-    for (int k = 0; k < TRI_B; ++k) acc = cfma(acc, s_dinv[i][k], s_y[k][t]);
-    s_x[i][t] = acc;
-  }
-  __syncthreads();
-  if (blockIdx.x == 0) {
-    for (int e = tid; e < cb * T; e += 256) {
+  for (int s = 0; s < f.nblk; ++s) {
+    const int kb = LOWER ? s : f.nblk - 1 - s;
+    const int c0 = kb * TRI_B;
+    const int cb = min(TRI_B, n - c0);
+    const S* Db = f.dinvblk + (static_cast<size_t>(b) * f.nblk + kb) * 2 * TRI_B * TRI_B + (LOWER ? 0 : TRI_B * TRI_B);
+    for (int e = tid; e < TRI_B * TRI_B; e += 256) s_d[e / TRI_B][e % TRI_B] = Db[e];
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+      const int r = tid + 256 * m;
+      if (r >= c0 && r < c0 + cb) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) s_y[r - c0][t] = yr[m][t];
+      }
+    }
+    if (tid < TRI_B && tid >= cb) {
+#pragma unroll
+      for (int t = 0; t < T; ++t) s_y[tid][t] = czero<S>();
+    }
+    __syncthreads();
+    for (int e = tid; e < TRI_B * T; e += 256) {
       const int i = e / T, t = e - (e / T) * T;
+      S acc = czero<S>();
+#pragma unroll 8
+      for (int k = 0; k < TRI_B; ++k) acc = cfma(acc, s_d[i][k], s_y[k][t]);
+      s_x[i][t] = acc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int m = 0; m < RPT; ++m) {
+      const int r = tid + 256 * m;
+      if (r >= c0 && r < c0 + cb) {
+#pragma unroll
+        for (int t = 0; t < T; ++t) yr[m][t] = s_x[r - c0][t];
+      } else if (r < n && (LOWER ? r >= c0 + cb : r < c0)) {
+        const S* frow = F + static_cast<size_t>(r) * f.ld + c0;
+        for (int k = 0; k < cb; ++k) {
+          const S l = frow[k];
+#pragma unroll
+          for (int t = 0; t < T; ++t) yr[m][t] = cfms(yr[m][t], l, s_x[k][t]);
+        }
+      }
+    }
+  }
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const int r = tid + 256 * m;
+    if (r >= n) continue;
+#pragma unroll
+    for (int t = 0; t < T; ++t) {
       if constexpr (TO_LADDER) {
         const int item = item0 + b;
         const int q = item % la.nq, w = (item / la.nq) % la.nw, chan = item / (la.nq * la.nw);
-        la.out[ladder_out_index(la, chan, w, q, c0 + i, t)] = cscale(to_c128(s_x[i][t]), la.inv_beta);
+        la.out[ladder_out_index(la, chan, w, q, r, t)] = cscale(to_c128(yr[m][t]), la.inv_beta);
       } else {
-        out.p[static_cast<size_t>(b) * out.item_stride + static_cast<size_t>(c0 + i) * out.row_stride + t] = s_x[i][t];
+        out.p[static_cast<size_t>(b) * out.item_stride + static_cast<size_t>(r) * out.row_stride + t] = yr[m][t];
       }
     }
-    return;
-  }
-  const int r0 = LOWER ? c0 + cb + (blockIdx.x - 1) * TRI_TILE : (blockIdx.x - 1) * TRI_TILE;
-  const int r1 = LOWER ? f.n : c0;
-  const S* F = f.W + static_cast<size_t>(b) * f.n * f.ld;
-  for (int e = tid; e < TRI_TILE * TRI_B; e += 256) {
-    const int r = e / TRI_B, c = e - (e / TRI_B) * TRI_B;
-    s_f[r][c] = (r0 + r < r1 && c < cb) ? F[static_cast<size_t>(r0 + r) * f.ld + c0 + c] : czero<S>();
-  }
-  __syncthreads();
-  const int rr = tid >> 2, tq = tid & 3;
-  const int r = r0 + rr;
-  if (r >= r1) return;
-#pragma unroll
-  for (int t = tq; t < T; t += 4) {
-    S* yp = Y + static_cast<size_t>(r) * y.row_stride + t;
-    S acc = *yp;
-#pragma unroll 8
-    for (int k = 0; k < TRI_B; ++k) acc = cfms(acc, s_f[rr][k], s_x[k][t]);
-    *yp = acc;
   }
 }
 
@@ -146,16 +163,21 @@ static int tri_sweep_t(const TriFactors<S>& f, int batch, TriVec<S> y, TriVec<S>
                        int item0, hipStream_t stream) {
   LadderArgs args{};
   if (la) args = *la;
-  for (int s = 0; s < f.nblk; ++s) {
-    const int kb = LOWER ? s : f.nblk - 1 - s;
-    const int c0 = kb * TRI_B;
-    const int cb = min(TRI_B, f.n - c0);
-    const int rows = LOWER ? f.n - c0 - cb : c0;
-    const int tiles = 1 + static_cast<int>(ceil_div(rows, TRI_TILE));
-    hipLaunchKernelGGL((tri_step_kernel<S, T, LOWER, TO_LADDER>), dim3(tiles, batch), dim3(256), 0, stream, f, y,
-                       out, args, item0, kb);
-    DGAMI_HIP_CHECK(hipGetLastError());
+  const int rpt = static_cast<int>(ceil_div(f.n, 256));
+#define DGAMI_TRI_LAUNCH(R)                                                                                   \
+  hipLaunchKernelGGL((tri_sweep_kernel<S, T, LOWER, TO_LADDER, R>), dim3(batch), dim3(256), 0, stream, f, y, out, \
+                     args, item0)
+  switch (rpt) {
+    case 1: DGAMI_TRI_LAUNCH(1); break;
+    case 2: DGAMI_TRI_LAUNCH(2); break;
+    case 3: DGAMI_TRI_LAUNCH(3); break;
+    case 4: DGAMI_TRI_LAUNCH(4); break;
+    case 5:
+    case 6: DGAMI_TRI_LAUNCH(6); break;
+    default: DGAMI_ARG_CHECK(false, "triangular solve: compound size above 1536");
   }
+#undef DGAMI_TRI_LAUNCH
+  DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
 }
 

@@ -3,12 +3,11 @@
 // The factors of every problem are stored row major in W[item][n][ld] (unit lower L
 // below the diagonal, U on and above it). Solves proceed in blocks of TRI_B = 32 rows:
 // the inverses of the 32x32 diagonal blocks of L and U are formed once per
-// factorisation (tri_diag_inverse), after which every block step of a sweep is one
-// fully parallel launch:
+// factorisation (tri_diag_inverse), after which a sweep is one launch with one workgroup
+// per problem, the right-hand sides in registers, and per block b
 //   x_b = D_b^-1 y_b                (D_b^-1 the stored diagonal-block inverse)
 //   y_rows -= F[rows, b] x_b        (rows below b for L, above b for U)
-// Each workgroup of a step recomputes x_b (a 32x32 by 32xT product) and updates one
-// 64-row tile, so no step needs more than one launch and no workgroup waits on another.
+// so that every factor element is read once per sweep.
 #pragma once
 
 #include "dgami_internal.hpp"
@@ -17,7 +16,6 @@
 namespace dgami {
 
 constexpr int TRI_B = 32;      // rows per diagonal block
-constexpr int TRI_TILE = 64;   // rows per workgroup of a block step
 
 template <typename S>
 struct TriFactors {

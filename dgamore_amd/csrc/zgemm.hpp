@@ -18,13 +18,15 @@ struct ZgemmArgs {
 
 constexpr int ZG_TM = 64, ZG_TN = 64, ZG_TK = 16;
 
-// grid: (ceil(N/64), ceil(M/64), batch); block 256
-static __global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g) {
-  const int b = blockIdx.z;
+// grid: xcd_grid(batch, tiles_m * tiles_n); block 256 (all tiles of one batch entry on one XCD)
+static __global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g, int batch, int tiles_n, int tiles_m) {
+  int b, t;
+  if (!xcd_tile(blockIdx.x, batch, tiles_m * tiles_n, b, t)) return;
   const double2* A = g.A + b * g.sa;
   const double2* B = g.B + b * g.sb;
   double2* C = g.C + b * g.sc;
-  const int m0 = blockIdx.y * ZG_TM, n0 = blockIdx.x * ZG_TN;
+  const int tm = t / tiles_n, tn = t - (t / tiles_n) * tiles_n;
+  const int m0 = tm * ZG_TM, n0 = tn * ZG_TN;
   __shared__ double2 s_a[ZG_TM][ZG_TK + 1];
   __shared__ double2 s_b[ZG_TK][ZG_TN + 1];
   const int tid = threadIdx.x;
@@ -78,8 +80,10 @@ static __global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g) {
 
 inline int launch_zgemm(const ZgemmArgs& g, int batch, hipStream_t stream) {
   if (g.M <= 0 || g.N <= 0 || batch <= 0) return 0;
-  dim3 grid(ceil_div(g.N, ZG_TN), ceil_div(g.M, ZG_TM), batch);
-  hipLaunchKernelGGL(zgemm_kernel, grid, dim3(256), 0, stream, g);
+  const int tn = ceil_div(g.N, ZG_TN), tm = ceil_div(g.M, ZG_TM);
+  const size_t grid = xcd_grid(batch, static_cast<size_t>(tn) * tm);
+  DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "zgemm: grid too large");
+  hipLaunchKernelGGL(zgemm_kernel, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, g, batch, tn, tm);
   DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
 }
