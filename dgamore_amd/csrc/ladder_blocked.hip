@@ -63,16 +63,26 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
 // Panel factorisation: thread t owns rows p0 + t + blockDim*m (m < RPT) of columns [p0, p0+NB).
 // Each column is one instance of panel_column<C>, expanded at compile time so that every
 // register index is a constant.
-// One barrier per column: before it, every wavefront publishes its local pivot candidate
-// (value, row index and the whole candidate row) and the owner of the diagonal row
-// publishes that row; after it, every thread picks the global winner from the nwave
-// candidates and applies the interchange and the rank-1 update from LDS. The LDS
-// slots alternate with the column parity, so one barrier per column suffices.
+// Pivot search per column: every thread takes the largest |Re|+|Im| of its rows, a
+// wavefront reduces the keys with DPP row shifts/broadcasts (the key is the upper word of
+// the FP64 magnitude plus one, so any valid row beats the invalid ones; candidates equal
+// in that word are interchangeable pivots), the wavefront leaders meet in LDS, and the
+// owner of the winning row publishes it for the rank-1 update (two barriers per column).
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false)));
+  return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
 template <typename S, int NB>
 struct PanelShared {
-  double val[2][16];
+  unsigned key[2][16];
   int idx[2][16];
-  S cand[2][16][NB];
+  S prow[2][NB];
   S krow[2][NB];
   int piv[NB];
   int mdst[2 * NB], msrc[2 * NB];
@@ -97,55 +107,52 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
       bidx = r;
     }
   }
-#pragma unroll
-  for (int off = 32; off > 0; off >>= 1) {
-    const double ov = __shfl_xor(best, off);
-    const int oi = __shfl_xor(bidx, off);
-    if (ov > best || (ov == best && oi < bidx)) {
-      best = ov;
+  {
+    const unsigned key = best >= 0.0 ? static_cast<unsigned>(__double_as_longlong(best) >> 32) + 1u : 0u;
+    const unsigned wkey = wave_max_u32(key);
+    const unsigned long long hit = __ballot(key == wkey);
+    const int wlane = __ffsll(static_cast<long long>(hit)) - 1;
+    const int widx = __builtin_amdgcn_readlane(bidx, wlane);
+    if (lane == 0) {
+      sh.key[buf][wave] = wkey;
+      sh.idx[buf][wave] = widx;
+    }
+  }
+  __syncthreads();
+  unsigned gkey = sh.key[buf][0];
+  bidx = sh.idx[buf][0];
+  for (int wv = 1; wv < nwave; ++wv) {
+    const unsigned k2 = sh.key[buf][wv];
+    const int oi = sh.idx[buf][wv];
+    if (k2 > gkey || (k2 == gkey && oi < bidx)) {
+      gkey = k2;
       bidx = oi;
     }
   }
-  // publish this wavefront's candidate row
+  best = gkey > 1u ? 1.0 : 0.0;  // nonzero pivot magnitude
+  const int p = bidx;
+  constexpr int mk = 0;  // column C's diagonal row p0 + C lies in the first register row (C < NB <= TH)
   if constexpr (RPT == 1) {
-    if (p0 + tid == bidx) {
+    if (tid == p - p0) {
 #pragma unroll
-      for (int cc = 0; cc < NB; ++cc) sh.cand[buf][wave][cc] = P[0][cc];
+      for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[0][cc];
     }
   } else {
 #pragma unroll
     for (int m = 0; m < RPT; ++m)
-      if (p0 + tid + TH * m == bidx) {
+      if (p0 + tid + TH * m == p) {
 #pragma unroll
-        for (int cc = 0; cc < NB; ++cc) sh.cand[buf][wave][cc] = P[m][cc];
+        for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[m][cc];
       }
   }
-  if (lane == 0) {
-    sh.val[buf][wave] = best;
-    sh.idx[buf][wave] = bidx;
-  }
-  if (tid == C) {  // row p0 + C is the first register row of thread C (C < NB <= 64)
+  if (p != col && tid == C) {
 #pragma unroll
-    for (int cc = 0; cc < NB; ++cc) sh.krow[buf][cc] = P[0][cc];
+    for (int cc = 0; cc < NB; ++cc) sh.krow[buf][cc] = P[mk][cc];
   }
-  __syncthreads();
-  best = sh.val[buf][0];
-  bidx = sh.idx[buf][0];
-  int bw = 0;
-  for (int wv = 1; wv < nwave; ++wv) {
-    const double ov = sh.val[buf][wv];
-    const int oi = sh.idx[buf][wv];
-    if (ov > best || (ov == best && oi < bidx)) {
-      best = ov;
-      bidx = oi;
-      bw = wv;
-    }
-  }
-  const int p = bidx;
-  const S* prow = sh.cand[buf][bw];
   if (tid == 0) sh.piv[C] = p;
+  __syncthreads();
   const bool ok = best > 0.0;
-  const S pinv = ok ? cinv(prow[C]) : czero<S>();
+  const S pinv = ok ? cinv(sh.prow[buf][C]) : czero<S>();
   if (tid == 0) {
     st.dinv[static_cast<size_t>(b) * n + col] = to_c128(pinv);
     if (!ok && st.info[b] == 0) st.info[b] = col + 1;
@@ -155,7 +162,7 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
     const int r = p0 + tid + TH * m;
     if (r == col) {
 #pragma unroll
-      for (int cc = 0; cc < NB; ++cc) P[m][cc] = prow[cc];
+      for (int cc = 0; cc < NB; ++cc) P[m][cc] = sh.prow[buf][cc];
     } else if (r > col && r < n) {
       if (r == p) {
 #pragma unroll
@@ -164,7 +171,7 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
       const S l = cmul(P[m][C], pinv);
       P[m][C] = l;
 #pragma unroll
-      for (int cc = C + 1; cc < NB; ++cc) P[m][cc] = cfms(P[m][cc], l, prow[cc]);
+      for (int cc = C + 1; cc < NB; ++cc) P[m][cc] = cfms(P[m][cc], l, sh.prow[buf][cc]);
     }
   }
 }
