@@ -183,7 +183,7 @@ __device__ __forceinline__ void panel_columns(S (&P)[RPT][NB], PanelShared<S, NB
 }
 
 template <typename S, int NB, int RPT, int MAXTH>
-__global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, int p0) {
+__global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, int p0, int slot) {
   const int b = blockIdx.x;
   const int n = st.n, ld = st.ld;
   const int TH = blockDim.x;
@@ -239,7 +239,7 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
       src[ij] = src[ip];
       src[ip] = t;
     }
-    int* map = st.rowmap + static_cast<size_t>(b) * MAP_STRIDE;
+    int* map = st.rowmap + (static_cast<size_t>(b) * 2 + slot) * MAP_STRIDE;
     for (int c = 0; c < NB; ++c) map[1 + 2 * NB + c] = -1;
     int outer = 0;
     for (int e = 0; e < cnt; ++e) {
@@ -257,10 +257,15 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
 }
 
 // ---------------------------------------------------------------------------
-// Interchanges for every column outside the panel, then U12 = L11^-1 A12 on trailing columns.
-// grid: xcd_grid(batch, ntile), 64 threads; tile t covers columns t*64 .. t*64+63 of [0, ncol).
+// Interchanges of panel p0 (its map in `slot`) and/or U12 = L11^-1 A12 for the columns of
+// [c_lo, c_hi) and [0, l_hi), the panel's own columns excluded; the triangular solve only
+// touches columns right of the panel. grid: xcd_grid(batch, ntile), 64 threads; the first
+// ntile_a tiles of 64 columns cover [c_lo, c_hi), the rest [0, l_hi).
+constexpr int ROWS_SWAP = 1, ROWS_TRSM = 2;
+
 template <typename S, int NB>
-__global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0, int ntile, int nprob) {
+__global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0, int slot, int c_lo, int c_hi,
+                                                      int l_hi, int flags, int ntile_a, int ntile, int nprob) {
   int b, tile;
   if (!xcd_tile(blockIdx.x, nprob, ntile, b, tile)) return;
   const int n = st.n, ld = st.ld;
@@ -268,32 +273,36 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
   const int nbw = (n - p0 < NB) ? n - p0 : NB;
   __shared__ S s_l[NB][NB + 1];
   __shared__ int s_dst[NB], s_src[NB], s_dest[NB];
-  const int* map = st.rowmap + static_cast<size_t>(b) * MAP_STRIDE;
+  const int* map = st.rowmap + (static_cast<size_t>(b) * 2 + slot) * MAP_STRIDE;
   const int outer = map[0];
   for (int e = threadIdx.x; e < NB; e += 64) {
     s_dst[e] = e < outer ? map[1 + e] : 0;
     s_src[e] = e < outer ? map[1 + NB + e] : 0;
     s_dest[e] = map[1 + 2 * NB + e];
   }
-  for (int e = threadIdx.x; e < nbw * nbw; e += 64) {
-    const int r = e / nbw, c = e - (e / nbw) * nbw;
-    s_l[r][c] = W[static_cast<size_t>(p0 + r) * ld + p0 + c];
+  if (flags & ROWS_TRSM) {
+    for (int e = threadIdx.x; e < nbw * nbw; e += 64) {
+      const int r = e / nbw, c = e - (e / nbw) * nbw;
+      s_l[r][c] = W[static_cast<size_t>(p0 + r) * ld + p0 + c];
+    }
   }
   __syncthreads();
-  const int col = tile * 64 + threadIdx.x;
-  if (col >= st.ncol || (col >= p0 && col < p0 + nbw)) return;
-  if (col < p0 && !st.swap_left) return;
-  // former panel values first (registers), then rows from below the panel into panel rows,
-  // then the former panel values to their destinations
-  S pv[NB];
+  const int col = tile < ntile_a ? c_lo + tile * 64 + threadIdx.x : (tile - ntile_a) * 64 + threadIdx.x;
+  if (col >= (tile < ntile_a ? c_hi : l_hi) || (col >= p0 && col < p0 + nbw)) return;
+  if (flags & ROWS_SWAP) {
+    // former panel values first (registers), then rows from below the panel into panel rows,
+    // then the former panel values to their destinations
+    S pv[NB];
 #pragma unroll
-  for (int c = 0; c < NB; ++c)
-    pv[c] = (c < nbw && s_dest[c] >= 0) ? W[static_cast<size_t>(p0 + c) * ld + col] : czero<S>();
-  for (int e = 0; e < outer; ++e) W[static_cast<size_t>(s_dst[e]) * ld + col] = W[static_cast<size_t>(s_src[e]) * ld + col];
+    for (int c = 0; c < NB; ++c)
+      pv[c] = (c < nbw && s_dest[c] >= 0) ? W[static_cast<size_t>(p0 + c) * ld + col] : czero<S>();
+    for (int e = 0; e < outer; ++e)
+      W[static_cast<size_t>(s_dst[e]) * ld + col] = W[static_cast<size_t>(s_src[e]) * ld + col];
 #pragma unroll
-  for (int c = 0; c < NB; ++c)
-    if (c < nbw && s_dest[c] >= 0) W[static_cast<size_t>(s_dest[c]) * ld + col] = pv[c];
-  if (col < p0) return;  // left of the panel: interchanges only
+    for (int c = 0; c < NB; ++c)
+      if (c < nbw && s_dest[c] >= 0) W[static_cast<size_t>(s_dest[c]) * ld + col] = pv[c];
+  }
+  if (!(flags & ROWS_TRSM) || col < p0) return;
   S x[NB];
 #pragma unroll
   for (int r = 0; r < NB; ++r) {
@@ -301,7 +310,7 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
     if (r < nbw) {
       S acc = W[static_cast<size_t>(p0 + r) * ld + col];
 #pragma unroll
-      for (int s = 0; s < r; ++s) acc = cfms(acc, s_l[r][s], x[s]);
+      for (int s2 = 0; s2 < r; ++s2) acc = cfms(acc, s_l[r][s2], x[s2]);
       x[r] = acc;
       W[static_cast<size_t>(p0 + r) * ld + col] = acc;
     }
@@ -309,21 +318,40 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
 }
 
 // ---------------------------------------------------------------------------
-// A22 -= L21 U12 (rows [p0+NB, n), columns [p0+NB, ncol)), 64x64 tiles, 256 threads.
-template <typename S, int NB>
-__global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c,
-                                                       int nprob) {
-  constexpr int TILE = 64;
-  const int per = tiles_r * tiles_c;
+// W[r0:r1, c0:c1] -= W[r0:r1, k0:k0+kw] W[k0:k0+kw, c0:c1] for every problem: 64x64 tiles,
+// 256 threads, the k range in LDS chunks of KC columns with the C tile held in registers.
+struct GemmRange {
+  int r0, r1, c0, c1, k0, kw;
+  int tiles_r, tiles_c;
+};
+
+inline GemmRange gemm_range(int r0, int r1, int c0, int c1, int k0, int kw) {
+  return GemmRange{r0, r1, c0, c1, k0, kw, static_cast<int>(ceil_div(r1 - r0, 64)), static_cast<int>(ceil_div(c1 - c0, 64))};
+}
+
+template <typename S, int KC>
+__device__ __forceinline__ void gemm_stage(const S* W, int ld, const GemmRange& g, int rt, int ct, int kc, int kn,
+                                           S (&s_a)[64][KC + 1], S (&s_b)[KC][65]) {
+  for (int e = threadIdx.x; e < 64 * KC; e += 256) {
+    const int r = e / KC, s = e - (e / KC) * KC;
+    s_a[r][s] = (rt + r < g.r1 && s < kn) ? W[static_cast<size_t>(rt + r) * ld + g.k0 + kc + s] : czero<S>();
+  }
+  for (int e = threadIdx.x; e < 64 * KC; e += 256) {
+    const int s = e / 64, c = e - (e / 64) * 64;
+    s_b[s][c] = (ct + c < g.c1 && s < kn) ? W[static_cast<size_t>(g.k0 + kc + s) * ld + ct + c] : czero<S>();
+  }
+}
+
+template <typename S, int KC>
+__global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, GemmRange g, int nprob) {
   int b, t;
-  if (!xcd_tile(blockIdx.x, nprob, per, b, t)) return;
-  const int tr = t / tiles_c, tcl = t - (t / tiles_c) * tiles_c;
-  const int n = st.n, ld = st.ld, ncol = st.ncol;
-  const int k0 = p0 + NB;
-  S* W = st.W + static_cast<size_t>(b) * n * ld;
-  const int r0 = k0 + tr * TILE, c0 = k0 + tcl * TILE;
-  __shared__ S s_a[TILE][NB + 1];
-  __shared__ S s_b[NB][TILE + 1];
+  if (!xcd_tile(blockIdx.x, nprob, g.tiles_r * g.tiles_c, b, t)) return;
+  const int tr = t / g.tiles_c, tcl = t - (t / g.tiles_c) * g.tiles_c;
+  const int ld = st.ld;
+  S* W = st.W + static_cast<size_t>(b) * st.n * ld;
+  const int rt = g.r0 + tr * 64, ct = g.c0 + tcl * 64;
+  __shared__ S s_a[64][KC + 1];
+  __shared__ S s_b[KC][65];
   const int tid = threadIdx.x;
   const int ty = tid / 16, tx = tid % 16;
   // the C tile is loaded first so that its latency overlaps the operand staging
@@ -332,140 +360,113 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, int p
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int r = r0 + ty + 16 * i, c = c0 + tx + 16 * j;
-      acc[i][j] = (r < n && c < ncol) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
+      const int r = rt + ty + 16 * i, c = ct + tx + 16 * j;
+      acc[i][j] = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
     }
-  for (int e = tid; e < TILE * NB; e += 256) {
-    const int r = e / NB, s = e - (e / NB) * NB;
-    s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : czero<S>();
-  }
-  for (int e = tid; e < TILE * NB; e += 256) {
-    const int s = e / TILE, c = e - (e / TILE) * TILE;
-    s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : czero<S>();
-  }
-  __syncthreads();
+  for (int kc = 0; kc < g.kw; kc += KC) {
+    const int kn = min(KC, g.kw - kc);
+    if (kc > 0) __syncthreads();
+    gemm_stage<S, KC>(W, ld, g, rt, ct, kc, kn, s_a, s_b);
+    __syncthreads();
 #pragma unroll 4
-  for (int s = 0; s < NB; ++s) {
-    S av[4], bv[4];
+    for (int s2 = 0; s2 < KC; ++s2) {
+      S av[4], bv[4];
 #pragma unroll
-    for (int i = 0; i < 4; ++i) av[i] = s_a[ty + 16 * i][s];
+      for (int i = 0; i < 4; ++i) av[i] = s_a[ty + 16 * i][s2];
 #pragma unroll
-    for (int j = 0; j < 4; ++j) bv[j] = s_b[s][tx + 16 * j];
+      for (int j = 0; j < 4; ++j) bv[j] = s_b[s2][tx + 16 * j];
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < 4; ++i)
 #pragma unroll
-      for (int j = 0; j < 4; ++j) acc[i][j] = cfms(acc[i][j], av[i], bv[j]);
+        for (int j = 0; j < 4; ++j) acc[i][j] = cfms(acc[i][j], av[i], bv[j]);
+    }
   }
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int r = r0 + ty + 16 * i;
-    if (r >= n) continue;
+    const int r = rt + ty + 16 * i;
+    if (r >= g.r1) continue;
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
-      const int c = c0 + tx + 16 * j;
-      if (c < ncol) W[static_cast<size_t>(r) * ld + c] = acc[i][j];
+      const int c = ct + tx + 16 * j;
+      if (c < g.c1) W[static_cast<size_t>(r) * ld + c] = acc[i][j];
     }
   }
 }
 
-// Same update on the matrix cores: each wavefront owns a 32x32 quarter of the tile as 2x2
-// blocks of 16x16, and a complex product becomes two real MFMA chains over the interleaved
-// k index (re/im): C_re -= [a_re, -a_im] . [b_re; b_im], C_im -= [a_im, a_re] . [b_re; b_im].
-// Operand layout of the 16x16x4 MFMA (lane l): a = A[l%16][l/16], b = B[l/16][l%16],
-// d[e] = D[4*(l/16)+e][l%16].
-template <typename S>
-struct Mfma16x16x4;
-template <>
-struct Mfma16x16x4<float2> {
-  using R = float;
+// Same update on the FP32 matrix cores: each wavefront owns a 32x32 quarter of the tile as
+// 2x2 blocks of 16x16, and a complex product becomes two real MFMA chains over the
+// interleaved k index (re/im): C_re -= [a_re, -a_im] . [b_re; b_im],
+// C_im -= [a_im, a_re] . [b_re; b_im]. Operand layout of the 16x16x4 MFMA (lane l):
+// a = A[l%16][l/16], b = B[l/16][l%16], d[e] = D[4*(l/16)+e][l%16]. The accumulators start
+// from the C tile and the A operand enters negated.
+template <int KC>
+__global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<float2> st, GemmRange g, int nprob) {
   typedef float V4 __attribute__((ext_vector_type(4)));
-  static __device__ __forceinline__ V4 op(R a, R b, V4 c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
-};
-
-template <typename S, int NB>
-__global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<S> st, int p0, int tiles_r, int tiles_c,
-                                                            int nprob) {
-  using M = Mfma16x16x4<S>;
-  using R = typename M::R;
-  using V4 = typename M::V4;
-  constexpr int TILE = 64;
-  const int per = tiles_r * tiles_c;
   int b, t;
-  if (!xcd_tile(blockIdx.x, nprob, per, b, t)) return;
-  const int tr = t / tiles_c, tcl = t - (t / tiles_c) * tiles_c;
-  const int n = st.n, ld = st.ld, ncol = st.ncol;
-  const int k0 = p0 + NB;
-  S* W = st.W + static_cast<size_t>(b) * n * ld;
-  const int r0 = k0 + tr * TILE, c0 = k0 + tcl * TILE;
-  __shared__ S s_a[TILE][NB + 1];
-  __shared__ S s_b[NB][TILE + 1];
+  if (!xcd_tile(blockIdx.x, nprob, g.tiles_r * g.tiles_c, b, t)) return;
+  const int tr = t / g.tiles_c, tcl = t - (t / g.tiles_c) * g.tiles_c;
+  const int ld = st.ld;
+  float2* W = st.W + static_cast<size_t>(b) * st.n * ld;
+  const int rt = g.r0 + tr * 64, ct = g.c0 + tcl * 64;
+  __shared__ float2 s_a[64][KC + 1];
+  __shared__ float2 s_b[KC][65];
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
-  const int l16 = lane & 15, kq = lane >> 4, kc = kq >> 1;
+  const int l16 = lane & 15, kq = lane >> 4, kcl = kq >> 1;
   const bool odd = (kq & 1) != 0;
-  // accumulators start from the C tile (loaded first, overlapping the operand staging);
-  // the A operand enters negated, so the chains compute C - A B
   V4 accr[2][2], acci[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int c = c0 + wc + 16 * j + l16;
+      const int c = ct + wc + 16 * j + l16;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = r0 + wr + 16 * i + 4 * kq + e;
-        const S v = (r < n && c < ncol) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
+        const int r = rt + wr + 16 * i + 4 * kq + e;
+        const float2 v = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : make_float2(0.f, 0.f);
         accr[i][j][e] = v.x;
         acci[i][j][e] = v.y;
       }
     }
-  for (int e = tid; e < TILE * NB; e += 256) {
-    const int r = e / NB, s = e - (e / NB) * NB;
-    s_a[r][s] = (r0 + r < n) ? W[static_cast<size_t>(r0 + r) * ld + p0 + s] : czero<S>();
-  }
-  for (int e = tid; e < TILE * NB; e += 256) {
-    const int s = e / TILE, c = e - (e / TILE) * TILE;
-    s_b[s][c] = (c0 + c < ncol) ? W[static_cast<size_t>(p0 + s) * ld + c0 + c] : czero<S>();
-  }
-  __syncthreads();
+  for (int kc = 0; kc < g.kw; kc += KC) {
+    const int kn = min(KC, g.kw - kc);
+    if (kc > 0) __syncthreads();
+    gemm_stage<float2, KC>(W, ld, g, rt, ct, kc, kn, s_a, s_b);
+    __syncthreads();
 #pragma unroll
-  for (int kk = 0; kk < NB; kk += 2) {
-    const int k = kk + kc;
-    R ar[2], ai[2], bv[2];
+    for (int kk = 0; kk < KC; kk += 2) {
+      const int k = kk + kcl;
+      float ar[2], ai[2], bv[2];
 #pragma unroll
-    for (int i = 0; i < 2; ++i) {
-      const S v = s_a[wr + 16 * i + l16][k];
-      ar[i] = odd ? v.y : -v.x;
-      ai[i] = odd ? -v.x : -v.y;
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const S v = s_b[k][wc + 16 * j + l16];
-      bv[j] = odd ? v.y : v.x;
-    }
-#pragma unroll
-    for (int i = 0; i < 2; ++i)
+      for (int i = 0; i < 2; ++i) {
+        const float2 v = s_a[wr + 16 * i + l16][k];
+        ar[i] = odd ? v.y : -v.x;
+        ai[i] = odd ? -v.x : -v.y;
+      }
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
-        accr[i][j] = M::op(ar[i], bv[j], accr[i][j]);
-        acci[i][j] = M::op(ai[i], bv[j], acci[i][j]);
+        const float2 v = s_b[k][wc + 16 * j + l16];
+        bv[j] = odd ? v.y : v.x;
       }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ar[i], bv[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(ai[i], bv[j], acci[i][j], 0, 0, 0);
+        }
+    }
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
     for (int j = 0; j < 2; ++j) {
-      const int c = c0 + wc + 16 * j + l16;
+      const int c = ct + wc + 16 * j + l16;
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
-        const int r = r0 + wr + 16 * i + 4 * kq + e;
-        if (r < n && c < ncol) {
-          S v;
-          v.x = accr[i][j][e];
-          v.y = acci[i][j][e];
-          W[static_cast<size_t>(r) * ld + c] = v;
-        }
+        const int r = rt + wr + 16 * i + 4 * kq + e;
+        if (r < g.r1 && c < g.c1) W[static_cast<size_t>(r) * ld + c] = make_float2(accr[i][j][e], acci[i][j][e]);
       }
     }
 }
@@ -494,38 +495,87 @@ static bool use_vector_gemm() {
 }
 
 template <typename S, int NB, int RPT, int MAXTH>
+struct BlockedSteps {
+  const LadderArgs& a;
+  BlockedState<S>& st;
+  int nb;
+  hipStream_t stream;
+
+  int panel(int p0, int slot) {
+    const int rows = st.n - p0;
+    const int th = ((rows + RPT - 1) / RPT + 63) / 64 * 64;
+    DGAMI_ARG_CHECK(th <= MAXTH, "ladder: compound size too large for the panel kernel");
+    hipLaunchKernelGGL((blk_panel_kernel<S, NB, RPT, MAXTH>), dim3(nb), dim3(th), 0, stream, st, p0, slot);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  // interchanges / triangular solve of panel p0 on [c_lo, c_hi) and, with swap_left, [0, l_hi)
+  int rows(int p0, int slot, int c_lo, int c_hi, int l_hi, int flags) {
+    if (!st.swap_left) l_hi = 0;
+    const int ta = c_hi > c_lo ? static_cast<int>(ceil_div(c_hi - c_lo, 64)) : 0;
+    const int ntile = ta + (l_hi > 0 ? static_cast<int>(ceil_div(l_hi, 64)) : 0);
+    if (ntile == 0) return 0;
+    hipLaunchKernelGGL((blk_rows_kernel<S, NB>), dim3(static_cast<unsigned>(xcd_grid(nb, ntile))), dim3(64), 0, stream,
+                       st, p0, slot, c_lo, c_hi, l_hi, flags, ta, ntile, nb);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+  int gemm(int r0, int r1, int c0, int c1, int k0, int kw) {
+    if (r1 <= r0 || c1 <= c0 || kw <= 0) return 0;
+    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw);
+    const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
+    DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
+    bool vector = true;
+    if constexpr (std::is_same<S, float2>::value) vector = use_vector_gemm();
+    if (vector) {
+      hipLaunchKernelGGL((blk_gemm_kernel<S, 32>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g, nb);
+    } else if constexpr (std::is_same<S, float2>::value) {
+      hipLaunchKernelGGL(blk_gemm_mfma_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g, nb);
+    }
+    DGAMI_HIP_CHECK(hipGetLastError());
+    return 0;
+  }
+};
+
+// Right-looking LU in outer blocks of 2*NB columns, each factored as two register panels
+// of NB columns, so that the trailing matrix is read and written once per 2*NB columns:
+//   panel 1; its interchanges + U12 on panel 2's columns, panel 2's columns -= L21 U12;
+//   panel 2; its interchanges on panel 1's columns;
+//   trailing columns: interchanges 1 + U12 (rows of panel 1), interchanges 2,
+//     rows of panel 2 -= L(panel 2 rows, panel 1) U12, U12 of panel 2;
+//   rows below the block -= L21 (2*NB columns) U12 (2*NB rows).
+template <typename S, int NB, int RPT, int MAXTH>
 static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream) {
-  const int n = st.n, ld = st.ld, ncol = st.ncol;
+  const int n = st.n, ncol = st.ncol;
   DGAMI_ARG_CHECK(nb <= 65535, "ladder: batch too large");
   hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(ceil_div(n, ASM_ROWS), nb), dim3(256), 0, stream, a, st, item0);
   DGAMI_HIP_CHECK(hipGetLastError());
-  for (int p0 = 0; p0 < n; p0 += NB) {
-    const int rows = n - p0;
-    int th = ((rows + RPT - 1) / RPT + 63) / 64 * 64;
-    DGAMI_ARG_CHECK(th <= MAXTH, "ladder: compound size too large for the panel kernel");
-    hipLaunchKernelGGL((blk_panel_kernel<S, NB, RPT, MAXTH>), dim3(nb), dim3(th), 0, stream, st, p0);
-    DGAMI_HIP_CHECK(hipGetLastError());
-    const int ntile = ceil_div(ncol, 64);
-    hipLaunchKernelGGL((blk_rows_kernel<S, NB>), dim3(static_cast<unsigned>(xcd_grid(nb, ntile))), dim3(64), 0, stream,
-                       st, p0, ntile, nb);
-    DGAMI_HIP_CHECK(hipGetLastError());
-    const int k1 = p0 + NB;
-    if (k1 < n) {
-      const int tr = ceil_div(n - k1, 64), tc = ceil_div(ncol - k1, 64);
-      const size_t g = xcd_grid(nb, static_cast<size_t>(tr) * tc);
-      DGAMI_ARG_CHECK(g < (size_t(1) << 31), "ladder: trailing update grid too large");
-      bool vector = true;
-      if constexpr (std::is_same<S, float2>::value) vector = use_vector_gemm();
-      if (vector) {
-        hipLaunchKernelGGL((blk_gemm_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0, tr,
-                           tc, nb);
-      } else if constexpr (std::is_same<S, float2>::value) {
-        hipLaunchKernelGGL((blk_gemm_mfma_kernel<S, NB>), dim3(static_cast<unsigned>(g)), dim3(256), 0, stream, st, p0,
-                           tr, tc, nb);
-      }
-      DGAMI_HIP_CHECK(hipGetLastError());
+  BlockedSteps<S, NB, RPT, MAXTH> k{a, st, nb, stream};
+  const int BOTH = ROWS_SWAP | ROWS_TRSM;
+  int rc = 0;
+#define DGAMI_STEP(x) \
+  if ((rc = (x)) != 0) return rc
+  for (int P = 0; P < n; P += 2 * NB) {
+    const int w1 = min(NB, n - P);
+    DGAMI_STEP(k.panel(P, 0));
+    const int P2 = P + NB;
+    if (P2 >= n) {  // last, single panel: only the right-hand-side columns remain
+      DGAMI_STEP(k.rows(P, 0, P + w1, ncol, P, BOTH));
+      break;
     }
+    const int w2 = min(NB, n - P2);
+    const int E = P2 + w2;  // end of the outer block
+    DGAMI_STEP(k.rows(P, 0, P2, E, 0, BOTH));
+    DGAMI_STEP(k.gemm(P2, n, P2, E, P, NB));
+    DGAMI_STEP(k.panel(P2, 1));
+    DGAMI_STEP(k.rows(P2, 1, P, P2, 0, ROWS_SWAP));
+    DGAMI_STEP(k.rows(P, 0, E, ncol, P, BOTH));
+    DGAMI_STEP(k.rows(P2, 1, E, ncol, P, ROWS_SWAP));
+    DGAMI_STEP(k.gemm(P2, E, E, ncol, P, NB));
+    DGAMI_STEP(k.rows(P2, 1, E, ncol, 0, ROWS_TRSM));
+    DGAMI_STEP(k.gemm(E, n, E, ncol, P, E - P));
   }
+#undef DGAMI_STEP
   return 0;
 }
 

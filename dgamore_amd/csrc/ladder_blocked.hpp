@@ -14,7 +14,7 @@ struct BlockedState {
   S* W;            // [batch][n][ld] matrices (row major), factored in place
   double2* dinv;   // [batch][n] inverse pivots (FP64 copy)
   int* piv;        // [batch][n] pivot row of every elimination step
-  int* rowmap;     // [batch][BLK_MAP_STRIDE] interchange map of the current panel
+  int* rowmap;     // [batch][2][BLK_MAP_STRIDE] interchange maps of the two panels of an outer block
   int* info;       // [batch] first zero pivot (1-based) or 0
   int n, ld, ncol;
   int swap_left;   // also apply interchanges left of the panel (needed when L is reused)
@@ -24,14 +24,14 @@ struct BlockedState {
 template <typename S>
 inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t batch, int ws) {
   st.W = static_cast<S*>(workspace(ws, batch * n * ld * sizeof(S)));
-  const size_t aux = batch * n * sizeof(double2) + batch * n * sizeof(int) + batch * BLK_MAP_STRIDE * sizeof(int) +
+  const size_t aux = batch * n * sizeof(double2) + batch * n * sizeof(int) + batch * 2 * BLK_MAP_STRIDE * sizeof(int) +
                      batch * sizeof(int) + 1024;
   char* p = static_cast<char*>(workspace(ws + 1, aux));
   if (!st.W || !p) return false;
   st.dinv = reinterpret_cast<double2*>(p);
   st.piv = reinterpret_cast<int*>(st.dinv + batch * n);
   st.rowmap = st.piv + batch * n;
-  st.info = st.rowmap + batch * BLK_MAP_STRIDE;
+  st.info = st.rowmap + batch * 2 * BLK_MAP_STRIDE;
   st.n = n;
   st.ld = ld;
   st.ncol = ncol;
