@@ -37,8 +37,9 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--precision", default="mixed", choices=["plain", "mixed"],
-                    help="form whose throughput is the headline value (mixed: complex64 LU + FP64 refinement)")
+    ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "bf16x3"],
+                    help="form whose throughput is the headline value (mixed: complex64 LU on the FP32 matrix "
+                         "cores + FP64 refinement; bf16x3: the same with split-BF16 MFMA trailing updates)")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
@@ -147,7 +148,7 @@ def main():
     from dgamore_amd import synthetic as S
     from dgamore_amd.pipeline import SDEPipeline
 
-    prec = {"plain": _lib.PLAIN_FP64, "mixed": _lib.MIXED_FP32_IR}[args.precision]
+    prec = {"plain": _lib.PLAIN_FP64, "mixed": _lib.MIXED_FP32_IR, "bf16x3": _lib.MIXED_BF16X3_IR}[args.precision]
     t_setup = time.perf_counter()
     model = S.make_model(args.config)
     pipe = SDEPipeline(model, prec)
@@ -204,7 +205,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "c128" if prec == _lib.PLAIN_FP64 else "c64-LU+c128-refinement",
+        "dtype": {0: "c128", 1: "c64-LU+c128-refinement", 2: "bf16x3-LU+c128-refinement"}[prec],
         "data": "synthetic (seeded DMFT-like model, random-perturbation local vertex)",
         "config": {"workload": f"{model.name}: {model.nb} orbitals, k-grid {'x'.join(map(str, model.nk))}, "
                                f"{pipe.nq} irreducible q, niw={model.niw}, niv={model.niv}, n={n}",

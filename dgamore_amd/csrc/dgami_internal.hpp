@@ -123,6 +123,25 @@ __device__ __forceinline__ bool xcd_tile(unsigned id, int nprob, int per, int& p
 }
 
 // ----------------------------------------------------------------------------
+// Wavefront maximum of a 32-bit key with DPP row shifts and row broadcasts (the result
+// collects in lane 63 and is returned uniform). Pivot searches reduce the key
+// (upper word of the FP64 magnitude) + 1, then pick the lowest lane holding the maximum.
+// ----------------------------------------------------------------------------
+__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false)));
+  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false)));
+  return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
+__device__ __forceinline__ unsigned pivot_key(double mag, bool valid) {
+  return valid ? static_cast<unsigned>(__double_as_longlong(mag) >> 32) + 1u : 0u;
+}
+
+// ----------------------------------------------------------------------------
 // error reporting (thread-local last error; host side)
 // ----------------------------------------------------------------------------
 void set_error(const std::string& msg);

@@ -68,15 +68,6 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
 // the FP64 magnitude plus one, so any valid row beats the invalid ones; candidates equal
 // in that word are interchangeable pivots), the wavefront leaders meet in LDS, and the
 // owner of the winning row publishes it for the rank-1 update (two barriers per column).
-__device__ __forceinline__ unsigned wave_max_u32(unsigned v) {
-  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xf, 0xf, false)));
-  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xf, 0xf, false)));
-  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xf, 0xf, false)));
-  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xf, 0xf, false)));
-  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xa, 0xf, false)));
-  v = max(v, static_cast<unsigned>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xc, 0xf, false)));
-  return static_cast<unsigned>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
-}
 
 template <typename S, int NB>
 struct PanelShared {
@@ -471,6 +462,126 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<float2>
     }
 }
 
+// Split-BF16 form of the complex64 update (three MFMA products per chain): every FP32
+// operand x is split as x = hi + lo with hi = bf16(x), lo = bf16(x - hi) (round to
+// nearest even), and a b ~ hi_a hi_b + hi_a lo_b + lo_a hi_b with FP32 accumulation
+// (relative error ~2^-16 per product; the FP64 refinement of the mixed solve removes it).
+// The interleaved real operands are staged in LDS as bf16: A' = [-a_re, a_im] and
+// A'' = [-a_im, -a_re] (negated for the subtraction) by rows, B' = [b_re; b_im] by columns.
+// Operand layout of the 16x16x32 bf16 MFMA (lane l): a[i] = A[l%16][8*(l/16)+i],
+// b[i] = B[8*(l/16)+i][l%16], d[e] = D[4*(l/16)+e][l%16].
+struct Bf16Split {
+  __bf16 hi, lo;
+};
+
+__device__ __forceinline__ Bf16Split bf16_split(float x) {
+  const __bf16 h = static_cast<__bf16>(x);
+  return Bf16Split{h, static_cast<__bf16>(x - static_cast<float>(h))};
+}
+
+template <int KC>
+__global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float2> st, GemmRange g, int nprob) {
+  typedef float V4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 B8 __attribute__((ext_vector_type(8)));
+  constexpr int KR = 2 * KC + 8;  // bf16 per staged row (interleaved re/im, padded)
+  int b, t;
+  if (!xcd_tile(blockIdx.x, nprob, g.tiles_r * g.tiles_c, b, t)) return;
+  const int tr = t / g.tiles_c, tcl = t - (t / g.tiles_c) * g.tiles_c;
+  const int ld = st.ld;
+  float2* W = st.W + static_cast<size_t>(b) * st.n * ld;
+  const int rt = g.r0 + tr * 64, ct = g.c0 + tcl * 64;
+  __shared__ __attribute__((aligned(16))) __bf16 s_a[4][64][KR];  // A' hi, A' lo, A'' hi, A'' lo
+  __shared__ __attribute__((aligned(16))) __bf16 s_b[2][64][KR];  // B' hi, B' lo, [column][k]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
+  const int l16 = lane & 15, kg = lane >> 4;
+  V4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = ct + wc + 16 * j + l16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = rt + wr + 16 * i + 4 * kg + e;
+        const float2 v = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : make_float2(0.f, 0.f);
+        accr[i][j][e] = v.x;
+        acci[i][j][e] = v.y;
+      }
+    }
+  for (int kc = 0; kc < g.kw; kc += KC) {
+    const int kn = min(KC, g.kw - kc);
+    if (kc > 0) __syncthreads();
+    for (int e = tid; e < 64 * KC; e += 256) {
+      const int r = e / KC, s2 = e - (e / KC) * KC;
+      const float2 v = (rt + r < g.r1 && s2 < kn) ? W[static_cast<size_t>(rt + r) * ld + g.k0 + kc + s2]
+                                                  : make_float2(0.f, 0.f);
+      const Bf16Split re = bf16_split(v.x), im = bf16_split(v.y);
+      s_a[0][r][2 * s2] = -re.hi;
+      s_a[1][r][2 * s2] = -re.lo;
+      s_a[0][r][2 * s2 + 1] = im.hi;
+      s_a[1][r][2 * s2 + 1] = im.lo;
+      s_a[2][r][2 * s2] = -im.hi;
+      s_a[3][r][2 * s2] = -im.lo;
+      s_a[2][r][2 * s2 + 1] = -re.hi;
+      s_a[3][r][2 * s2 + 1] = -re.lo;
+    }
+    for (int e = tid; e < 64 * KC; e += 256) {
+      const int s2 = e / 64, c = e - (e / 64) * 64;
+      const float2 v = (ct + c < g.c1 && s2 < kn) ? W[static_cast<size_t>(g.k0 + kc + s2) * ld + ct + c]
+                                                  : make_float2(0.f, 0.f);
+      const Bf16Split re = bf16_split(v.x), im = bf16_split(v.y);
+      s_b[0][c][2 * s2] = re.hi;
+      s_b[1][c][2 * s2] = re.lo;
+      s_b[0][c][2 * s2 + 1] = im.hi;
+      s_b[1][c][2 * s2 + 1] = im.lo;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < 2 * KC; ks += 32) {
+      const int k = ks + 8 * kg;
+      B8 ah[2], al[2], a2h[2], a2l[2], bh[2], bl[2];
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        const int row = wr + 16 * i + l16;
+        ah[i] = *reinterpret_cast<const B8*>(&s_a[0][row][k]);
+        al[i] = *reinterpret_cast<const B8*>(&s_a[1][row][k]);
+        a2h[i] = *reinterpret_cast<const B8*>(&s_a[2][row][k]);
+        a2l[i] = *reinterpret_cast<const B8*>(&s_a[3][row][k]);
+      }
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int col = wc + 16 * j + l16;
+        bh[j] = *reinterpret_cast<const B8*>(&s_b[0][col][k]);
+        bl[j] = *reinterpret_cast<const B8*>(&s_b[1][col][k]);
+      }
+#pragma unroll
+      for (int i = 0; i < 2; ++i)
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bh[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ah[i], bl[j], accr[i][j], 0, 0, 0);
+          accr[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(al[i], bh[j], accr[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[i], bh[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2h[i], bl[j], acci[i][j], 0, 0, 0);
+          acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l[i], bh[j], acci[i][j], 0, 0, 0);
+        }
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const int c = ct + wc + 16 * j + l16;
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        const int r = rt + wr + 16 * i + 4 * kg + e;
+        if (r < g.r1 && c < g.c1) W[static_cast<size_t>(r) * ld + c] = make_float2(accr[i][j][e], acci[i][j][e]);
+      }
+    }
+}
+
 __global__ void blk_info_kernel(const int* __restrict__ st_info, int32_t* __restrict__ info, int item0, int nitems) {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
   if (b < nitems) info[item0 + b] = st_info[b];
@@ -525,12 +636,17 @@ struct BlockedSteps {
     const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw);
     const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
     DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
-    bool vector = true;
-    if constexpr (std::is_same<S, float2>::value) vector = use_vector_gemm();
-    if (vector) {
+    int mode = BLK_GEMM_VECTOR;
+    if constexpr (std::is_same<S, float2>::value) mode = use_vector_gemm() ? BLK_GEMM_VECTOR : st.gemm_mode;
+    if (mode == BLK_GEMM_VECTOR) {
       hipLaunchKernelGGL((blk_gemm_kernel<S, 32>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g, nb);
     } else if constexpr (std::is_same<S, float2>::value) {
-      hipLaunchKernelGGL(blk_gemm_mfma_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g, nb);
+      if (mode == BLK_GEMM_BF16X3)
+        hipLaunchKernelGGL(blk_gemm_bf16x3_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
+                           nb);
+      else
+        hipLaunchKernelGGL(blk_gemm_mfma_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
+                           nb);
     }
     DGAMI_HIP_CHECK(hipGetLastError());
     return 0;

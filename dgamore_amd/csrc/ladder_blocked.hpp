@@ -9,6 +9,9 @@ namespace dgami {
 constexpr int BLK_MAX_NB = 64;                      // widest panel
 constexpr int BLK_MAP_STRIDE = 3 * BLK_MAX_NB + 1;  // ints of one problem's interchange map
 
+// Trailing-update kernels of the complex64 factorisation (complex128 always uses vector FMA).
+enum BlockedGemm { BLK_GEMM_VECTOR = 0, BLK_GEMM_F32_MFMA = 1, BLK_GEMM_BF16X3 = 2 };
+
 template <typename S>
 struct BlockedState {
   S* W;            // [batch][n][ld] matrices (row major), factored in place
@@ -18,6 +21,7 @@ struct BlockedState {
   int* info;       // [batch] first zero pivot (1-based) or 0
   int n, ld, ncol;
   int swap_left;   // also apply interchanges left of the panel (needed when L is reused)
+  int gemm_mode;   // BlockedGemm (complex64 only)
 };
 
 // Allocate the workspace of `batch` problems in slots ws and ws+1; returns false on failure.
@@ -36,6 +40,7 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
   st.ld = ld;
   st.ncol = ncol;
   st.swap_left = 0;
+  st.gemm_mode = BLK_GEMM_F32_MFMA;
   return true;
 }
 

@@ -3,8 +3,9 @@
 // One workgroup of NW wavefronts solves one (chan, w, q) problem A X = B.
 // Wavefront w owns the columns j = w + NW*jj of the augmented matrix [A | B]
 // and lane l owns the rows i = l + 64*m, all in registers (RS*CS complex per
-// lane). Gaussian elimination with partial pivoting (largest |Re|+|Im|, first
-// index on ties) then needs only one workgroup barrier per step:
+// lane). Gaussian elimination with partial pivoting (largest |Re|+|Im| by the
+// upper word of its FP64 value, lowest lane on ties) then needs only one
+// workgroup barrier per step:
 //   * the wavefront owning column k finds the pivot with a wavefront
 //     reduction and publishes the pivot row index and the multipliers of
 //     column k through LDS;            -- barrier --
@@ -95,14 +96,12 @@ __global__ void __launch_bounds__(NW * 64) ladder_col_kernel(LadderArgs a) {
           bidx = i;
         }
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_xor(best, off);
-        const int oi = __shfl_xor(bidx, off);
-        if (ov > best || (ov == best && oi < bidx)) {
-          best = ov;
-          bidx = oi;
-        }
+      {
+        const unsigned key = pivot_key(best, best >= 0.0);
+        const unsigned wkey = wave_max_u32(key);
+        const unsigned long long hit = __ballot(key == wkey);
+        bidx = __builtin_amdgcn_readlane(bidx, __ffsll(static_cast<long long>(hit)) - 1);
+        best = wkey > 1u ? 1.0 : 0.0;  // nonzero pivot magnitude
       }
       const int p = bidx;
       const int pl = p & 63, pm = p >> 6;

@@ -1,7 +1,8 @@
 // Mixed-precision ladder solves: complex64 LU factorisation, FP64 iterative refinement.
 //
 // Every item's matrix A is factored once in FP32 (PA = LU, batched blocked LU of
-// ladder_blocked.hip with the interchanges also applied left of each panel). The
+// ladder_blocked.hip with the interchanges also applied left of each panel; trailing
+// updates on the FP32 matrix cores, or as three split-BF16 MFMA products). The
 // solution X of A X = B (B = indicator right-hand sides, the nu' sum of A^-1)
 // is then refined in FP64:
 //   R = B - A X      in FP64 from the FP64 operands, with the vertex term Gamma(chan,w) X
@@ -151,8 +152,11 @@ __global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const 
 
 // ---------------------------------------------------------------------------
 int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
-  DGAMI_ARG_CHECK(precision == DGAMI_MIXED_FP32_IR,
-                  "ladder: the split-BF16 mixed mode is not available in this build; use DGAMI_MIXED_FP32_IR");
+  DGAMI_ARG_CHECK(precision == DGAMI_MIXED_FP32_IR || precision == DGAMI_MIXED_BF16X3_IR,
+                  "ladder: unknown mixed-precision mode");
+  // the split-BF16 factors carry ~2^-16 relative error per product instead of 2^-24, so
+  // each refinement step gains fewer digits: two more solves reach the same accuracy
+  const int solves = precision == DGAMI_MIXED_BF16X3_IR ? MIXED_SOLVES + 2 : MIXED_SOLVES;
   const int n = a.n, T = a.T;
   DGAMI_ARG_CHECK(T <= 16, "ladder: at most 4 orbitals in the mixed-precision form");
   DGAMI_ARG_CHECK(n <= 1536, "ladder: compound size above 1536 is not supported");
@@ -175,6 +179,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   BlockedState<float2> st;
   if (!blocked_alloc(st, n, ld, n, bitems, WS_MIXED)) return -2;
   st.swap_left = 1;
+  st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
   char* vec = static_cast<char*>(workspace(WS_MIXED_VEC, 2 * bitems * n * T * sizeof(double2) + bitems * n * sizeof(int)));
   if (!vec) return -2;
   MixedArgs m;
@@ -207,7 +212,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     rc = tri_diag_inverse(f, nit, true, stream);
     if (rc) return rc;
     const dim3 vgrid(ceil_div(static_cast<size_t>(n) * T, 256), nit);
-    for (int s = 0; s < MIXED_SOLVES; ++s) {
+    for (int s = 0; s < solves; ++s) {
       if (s > 0) {
         ZgemmArgs g;
         g.A = a.gamma + static_cast<size_t>(g0) * n * n;
@@ -237,7 +242,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
       rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
       if (rc) return rc;
       hipLaunchKernelGGL(mix_accumulate_kernel, vgrid, dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0,
-                         s == MIXED_SOLVES - 1 ? 1 : 0);
+                         s == solves - 1 ? 1 : 0);
       DGAMI_HIP_CHECK(hipGetLastError());
     }
     rc = blocked_copy_info(st.info, a.info, item0, nit, stream);

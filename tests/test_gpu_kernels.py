@@ -198,11 +198,25 @@ def test_ladder_mixed_two_channels_realistic_vertex(K):
     assert rel(got[1], O.ladder_sum_solve(chi0_inv, gm, um, vm, beta)) < MIXED_RTOL
 
 
-def test_ladder_split_bf16_mode_is_refused(K):
-    say("ladder bf16x3 mode -> error")
+@pytest.mark.parametrize("nb,niv,niw,nq", [
+    (1, 60, 2, 3),    # n = 120
+    (3, 10, 2, 3),    # n = 180
+    (3, 40, 1, 2),    # n = 720, the 3-orbital box
+])
+def test_ladder_mixed_split_bf16_ir(K, nb, niv, niw, nq):
+    say(f"ladder mixed split-bf16+IR nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=nb * 100 + niv + 2)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=2)
+    assert int(info.abs().sum()) == 0
+    assert rel(got[0], ref) < MIXED_RTOL
+
+
+def test_ladder_unknown_precision_is_refused(K):
+    say("ladder precision 7 -> error")
     chi0_inv, gamma, u, v, beta = _ladder_case(1, 4, 1, 2, seed=5)
     with pytest.raises(ValueError):
-        _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=2)
+        _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=7)
 
 
 def test_ladder_two_channels_realistic_vertex(K):
