@@ -57,11 +57,17 @@ def copy_calibration():
     n = (1 << 30) // 16
     a = torch.ones(n, dtype=torch.complex128, device="cuda")
     b = torch.empty_like(a)
+    b.copy_(a)
     torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
     for _ in range(3):
         b.copy_(a)
+    e1.record()
     torch.cuda.synchronize()
-    print(json.dumps({"copy_bytes_read": n * 16, "copy_bytes_written": n * 16, "copies": 3}), flush=True)
+    ms = e0.elapsed_time(e1) / 3
+    print(json.dumps({"copy_bytes_read": n * 16, "copy_bytes_written": n * 16, "copies": 4,
+                      "copy_ms": ms, "copy_GBps": 2 * n * 16 / (ms * 1e-3) / 1e9}), flush=True)
 
 
 def cpu_baseline(model, target_s):

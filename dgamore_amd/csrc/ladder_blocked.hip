@@ -316,14 +316,14 @@ struct GemmRange {
   int tiles_r, tiles_c;
 };
 
-inline GemmRange gemm_range(int r0, int r1, int c0, int c1, int k0, int kw) {
-  return GemmRange{r0, r1, c0, c1, k0, kw, static_cast<int>(ceil_div(r1 - r0, 64)), static_cast<int>(ceil_div(c1 - c0, 64))};
+inline GemmRange gemm_range(int r0, int r1, int c0, int c1, int k0, int kw, int tm = 64) {
+  return GemmRange{r0, r1, c0, c1, k0, kw, static_cast<int>(ceil_div(r1 - r0, tm)), static_cast<int>(ceil_div(c1 - c0, 64))};
 }
 
-template <typename S, int KC>
+template <typename S, int KC, int TM = 64>
 __device__ __forceinline__ void gemm_stage(const S* W, int ld, const GemmRange& g, int rt, int ct, int kc, int kn,
-                                           S (&s_a)[64][KC + 1], S (&s_b)[KC][65]) {
-  for (int e = threadIdx.x; e < 64 * KC; e += 256) {
+                                           S (&s_a)[TM][KC + 1], S (&s_b)[KC][65]) {
+  for (int e = threadIdx.x; e < TM * KC; e += 256) {
     const int r = e / KC, s = e - (e / KC) * KC;
     s_a[r][s] = (rt + r < g.r1 && s < kn) ? W[static_cast<size_t>(rt + r) * ld + g.k0 + kc + s] : czero<S>();
   }
@@ -333,22 +333,26 @@ __device__ __forceinline__ void gemm_stage(const S* W, int ld, const GemmRange& 
   }
 }
 
-template <typename S, int KC>
+// Thread (ty, tx) of the 16x16 layout owns rows ty + 16 i (i < RI) and columns tx + 16 j
+// (j < 4) of a (16 RI) x 64 tile; RI = 8 gives 12 LDS reads per 32 complex FMAs, enough
+// to keep the FP64 pipes busy.
+template <typename S, int KC, int RI>
 __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, GemmRange g, int nprob) {
+  constexpr int TM = 16 * RI;
   int b, t;
   if (!xcd_tile(blockIdx.x, nprob, g.tiles_r * g.tiles_c, b, t)) return;
   const int tr = t / g.tiles_c, tcl = t - (t / g.tiles_c) * g.tiles_c;
   const int ld = st.ld;
   S* W = st.W + static_cast<size_t>(b) * st.n * ld;
-  const int rt = g.r0 + tr * 64, ct = g.c0 + tcl * 64;
-  __shared__ S s_a[64][KC + 1];
+  const int rt = g.r0 + tr * TM, ct = g.c0 + tcl * 64;
+  __shared__ S s_a[TM][KC + 1];
   __shared__ S s_b[KC][65];
   const int tid = threadIdx.x;
   const int ty = tid / 16, tx = tid % 16;
   // the C tile is loaded first so that its latency overlaps the operand staging
-  S acc[4][4];
+  S acc[RI][4];
 #pragma unroll
-  for (int i = 0; i < 4; ++i)
+  for (int i = 0; i < RI; ++i)
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const int r = rt + ty + 16 * i, c = ct + tx + 16 * j;
@@ -357,23 +361,23 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, GemmR
   for (int kc = 0; kc < g.kw; kc += KC) {
     const int kn = min(KC, g.kw - kc);
     if (kc > 0) __syncthreads();
-    gemm_stage<S, KC>(W, ld, g, rt, ct, kc, kn, s_a, s_b);
+    gemm_stage<S, KC, TM>(W, ld, g, rt, ct, kc, kn, s_a, s_b);
     __syncthreads();
-#pragma unroll 4
+#pragma unroll 2
     for (int s2 = 0; s2 < KC; ++s2) {
-      S av[4], bv[4];
+      S av[RI], bv[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) av[i] = s_a[ty + 16 * i][s2];
+      for (int i = 0; i < RI; ++i) av[i] = s_a[ty + 16 * i][s2];
 #pragma unroll
       for (int j = 0; j < 4; ++j) bv[j] = s_b[s2][tx + 16 * j];
 #pragma unroll
-      for (int i = 0; i < 4; ++i)
+      for (int i = 0; i < RI; ++i)
 #pragma unroll
         for (int j = 0; j < 4; ++j) acc[i][j] = cfms(acc[i][j], av[i], bv[j]);
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
+  for (int i = 0; i < RI; ++i) {
     const int r = rt + ty + 16 * i;
     if (r >= g.r1) continue;
 #pragma unroll
@@ -633,13 +637,20 @@ struct BlockedSteps {
   }
   int gemm(int r0, int r1, int c0, int c1, int k0, int kw) {
     if (r1 <= r0 || c1 <= c0 || kw <= 0) return 0;
-    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw);
-    const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
-    DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
     int mode = BLK_GEMM_VECTOR;
     if constexpr (std::is_same<S, float2>::value) mode = use_vector_gemm() ? BLK_GEMM_VECTOR : st.gemm_mode;
+    // complex128 vector tiles are 128 x 64 (8 rows per thread), the others 64 x 64
+    constexpr bool tall = std::is_same<S, double2>::value;
+    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw, tall && mode == BLK_GEMM_VECTOR ? 128 : 64);
+    const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
+    DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
     if (mode == BLK_GEMM_VECTOR) {
-      hipLaunchKernelGGL((blk_gemm_kernel<S, 32>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g, nb);
+      if constexpr (tall)
+        hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
+                           nb);
+      else
+        hipLaunchKernelGGL((blk_gemm_kernel<S, 32, 4>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
+                           nb);
     } else if constexpr (std::is_same<S, float2>::value) {
       if (mode == BLK_GEMM_BF16X3)
         hipLaunchKernelGGL(blk_gemm_bf16x3_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
