@@ -500,7 +500,8 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
   const int wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int l16 = lane & 15, kg = lane >> 4;
-  V4 accr[2][2], acci[2][2];
+  // the C tile is held apart from the accumulators, so its loads overlap staging and compute
+  float2 cin[2][2][4];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
 #pragma unroll
@@ -509,37 +510,45 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = rt + wr + 16 * i + 4 * kg + e;
-        const float2 v = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : make_float2(0.f, 0.f);
-        accr[i][j][e] = v.x;
-        acci[i][j][e] = v.y;
+        cin[i][j][e] = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : make_float2(0.f, 0.f);
       }
+    }
+  V4 accr[2][2], acci[2][2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      accr[i][j] = V4{0.f, 0.f, 0.f, 0.f};
+      acci[i][j] = V4{0.f, 0.f, 0.f, 0.f};
     }
   for (int kc = 0; kc < g.kw; kc += KC) {
     const int kn = min(KC, g.kw - kc);
     if (kc > 0) __syncthreads();
-    for (int e = tid; e < 64 * KC; e += 256) {
-      const int r = e / KC, s2 = e - (e / KC) * KC;
-      const float2 v = (rt + r < g.r1 && s2 < kn) ? W[static_cast<size_t>(rt + r) * ld + g.k0 + kc + s2]
-                                                  : make_float2(0.f, 0.f);
-      const Bf16Split re = bf16_split(v.x), im = bf16_split(v.y);
-      s_a[0][r][2 * s2] = -re.hi;
-      s_a[1][r][2 * s2] = -re.lo;
-      s_a[0][r][2 * s2 + 1] = im.hi;
-      s_a[1][r][2 * s2 + 1] = im.lo;
-      s_a[2][r][2 * s2] = -im.hi;
-      s_a[3][r][2 * s2] = -im.lo;
-      s_a[2][r][2 * s2 + 1] = -re.hi;
-      s_a[3][r][2 * s2 + 1] = -re.lo;
+    // each thread splits a pair of consecutive k of one row (A) or one column (B) and writes
+    // 4 packed bf16 per array
+    for (int e = tid; e < 64 * (KC / 2); e += 256) {
+      const int r = e / (KC / 2), s2 = 2 * (e - (e / (KC / 2)) * (KC / 2));
+      const bool in = rt + r < g.r1;
+      const float2* src = W + static_cast<size_t>(rt + r) * ld + g.k0 + kc + s2;
+      const float2 v0 = (in && s2 < kn) ? src[0] : make_float2(0.f, 0.f);
+      const float2 v1 = (in && s2 + 1 < kn) ? src[1] : make_float2(0.f, 0.f);
+      const Bf16Split r0 = bf16_split(v0.x), i0 = bf16_split(v0.y), r1 = bf16_split(v1.x), i1 = bf16_split(v1.y);
+      typedef __bf16 B4 __attribute__((ext_vector_type(4)));
+      *reinterpret_cast<B4*>(&s_a[0][r][2 * s2]) = B4{-r0.hi, i0.hi, -r1.hi, i1.hi};
+      *reinterpret_cast<B4*>(&s_a[1][r][2 * s2]) = B4{-r0.lo, i0.lo, -r1.lo, i1.lo};
+      *reinterpret_cast<B4*>(&s_a[2][r][2 * s2]) = B4{-i0.hi, -r0.hi, -i1.hi, -r1.hi};
+      *reinterpret_cast<B4*>(&s_a[3][r][2 * s2]) = B4{-i0.lo, -r0.lo, -i1.lo, -r1.lo};
     }
-    for (int e = tid; e < 64 * KC; e += 256) {
-      const int s2 = e / 64, c = e - (e / 64) * 64;
-      const float2 v = (ct + c < g.c1 && s2 < kn) ? W[static_cast<size_t>(g.k0 + kc + s2) * ld + ct + c]
-                                                  : make_float2(0.f, 0.f);
-      const Bf16Split re = bf16_split(v.x), im = bf16_split(v.y);
-      s_b[0][c][2 * s2] = re.hi;
-      s_b[1][c][2 * s2] = re.lo;
-      s_b[0][c][2 * s2 + 1] = im.hi;
-      s_b[1][c][2 * s2 + 1] = im.lo;
+    for (int e = tid; e < 64 * (KC / 2); e += 256) {
+      const int s2 = 2 * (e / 64), c = e - (e / 64) * 64;
+      const bool in = ct + c < g.c1;
+      const float2 v0 = (in && s2 < kn) ? W[static_cast<size_t>(g.k0 + kc + s2) * ld + ct + c] : make_float2(0.f, 0.f);
+      const float2 v1 =
+          (in && s2 + 1 < kn) ? W[static_cast<size_t>(g.k0 + kc + s2 + 1) * ld + ct + c] : make_float2(0.f, 0.f);
+      const Bf16Split r0 = bf16_split(v0.x), i0 = bf16_split(v0.y), r1 = bf16_split(v1.x), i1 = bf16_split(v1.y);
+      typedef __bf16 B4 __attribute__((ext_vector_type(4)));
+      *reinterpret_cast<B4*>(&s_b[0][c][2 * s2]) = B4{r0.hi, i0.hi, r1.hi, i1.hi};
+      *reinterpret_cast<B4*>(&s_b[1][c][2 * s2]) = B4{r0.lo, i0.lo, r1.lo, i1.lo};
     }
     __syncthreads();
 #pragma unroll
@@ -581,7 +590,8 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = rt + wr + 16 * i + 4 * kg + e;
-        if (r < g.r1 && c < g.c1) W[static_cast<size_t>(r) * ld + c] = make_float2(accr[i][j][e], acci[i][j][e]);
+        if (r < g.r1 && c < g.c1)
+          W[static_cast<size_t>(r) * ld + c] = make_float2(cin[i][j][e].x + accr[i][j][e], cin[i][j][e].y + acci[i][j][e]);
       }
     }
 }
