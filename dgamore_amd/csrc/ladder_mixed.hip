@@ -11,9 +11,10 @@
 //                    U_r + V_r term are applied per item);
 //   D = (LU)^-1 P R  in FP32 (blocked triangular sweeps with inverse diagonal blocks);
 //   X += D           in FP64.
-// Starting from X = 0 (R = B), MIXED_SOLVES solves are made. Each refinement step
-// multiplies the error by about cond(A) * 2^-24, so two refinements bring the result
-// to FP64 accuracy for the condition numbers of ladder matrices.
+// Starting from X = 0 (R = B), solves continue until the predicted relative error is below
+// 1e-10 (see MIXED_TOL). Each refinement step multiplies the error by about cond(A) * 2^-24
+// (FP32 factors) or cond(A) * 2^-16 (split-BF16 updates); the number of solves is data
+// dependent, so the host reads one float per solve.
 #include <algorithm>
 
 #include "dgami.h"
@@ -26,7 +27,12 @@
 
 namespace dgami {
 
-constexpr int MIXED_SOLVES = 3;
+// Refinement stops when the predicted relative error of X, |D_s| (|D_s| / |D_s-1|) / |X|,
+// falls below MIXED_TOL (the contraction |D_s| / |D_s-1| estimates the error factor of one
+// refinement step), or after the maximum number of solves.
+constexpr int MIXED_MAX_SOLVES_FP32 = 4;
+constexpr int MIXED_MAX_SOLVES_BF16 = 8;
+constexpr double MIXED_TOL = 1e-10;
 
 // Item layout of the refinement vectors: group g = (chan, w) (or the item itself in
 // dense mode) holds gq items; X and R are [group][n][gq][T] so that Gamma(g) X(g) is
@@ -128,26 +134,55 @@ __global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __
       first ? from_c128<float2>(ladder_rhs(a, src, t)) : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
 }
 
-// X += D (FP64); on the last solve also out = X / beta.
+// X += D (FP64) and out = X / beta; per item, the largest |D| and |X| (|Re|+|Im|) go to
+// norms[2*item], norms[2*item+1]. One workgroup per item.
 __global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const float2* __restrict__ D, int first,
-                                                             int last) {
+                                                             float* __restrict__ norms) {
   const LadderArgs& a = m.a;
-  const int lb = blockIdx.y;
+  const int lb = blockIdx.x;
   const int gl = lb / m.gq, q = lb - gl * m.gq;
   const int n = a.n, T = a.T, gq = m.gq;
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n * T) return;
-  const int i = idx / T, t = idx - (idx / T) * T;
+  const int item = m.item0 + lb;
+  const int qt = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
   double2* X = m.X + static_cast<size_t>(gl) * n * gq * T;
-  const size_t o = (static_cast<size_t>(i) * gq + q) * T + t;
-  const double2 d = to_c128(D[static_cast<size_t>(lb) * n * T + idx]);
-  const double2 x = first ? d : cadd(X[o], d);
-  X[o] = x;
-  if (last) {
-    const int item = m.item0 + lb;
-    const int qt = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
+  const float2* Db = D + static_cast<size_t>(lb) * n * T;
+  float dmax = 0.f, xmax = 0.f;
+  for (int idx = threadIdx.x; idx < n * T; idx += 256) {
+    const int i = idx / T, t = idx - (idx / T) * T;
+    const size_t o = (static_cast<size_t>(i) * gq + q) * T + t;
+    const double2 d = to_c128(Db[idx]);
+    const double2 x = first ? d : cadd(X[o], d);
+    X[o] = x;
     a.out[ladder_out_index(a, chan, w, qt, i, t)] = cscale(x, a.inv_beta);
+    dmax = fmaxf(dmax, static_cast<float>(cabs1(d)));
+    xmax = fmaxf(xmax, static_cast<float>(cabs1(x)));
   }
+  __shared__ float s_d[4], s_x[4];
+  const unsigned dk = wave_max_u32(__float_as_uint(dmax)), xk = wave_max_u32(__float_as_uint(xmax));
+  if ((threadIdx.x & 63) == 0) {
+    s_d[threadIdx.x >> 6] = __uint_as_float(dk);
+    s_x[threadIdx.x >> 6] = __uint_as_float(xk);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    norms[2 * static_cast<size_t>(lb)] = fmaxf(fmaxf(s_d[0], s_d[1]), fmaxf(s_d[2], s_d[3]));
+    norms[2 * static_cast<size_t>(lb) + 1] = fmaxf(fmaxf(s_x[0], s_x[1]), fmaxf(s_x[2], s_x[3]));
+  }
+}
+
+// Largest relative correction max_item(|D|_max / |X|_max) of the batch into *ratio.
+__global__ void __launch_bounds__(256) mix_ratio_kernel(const float* __restrict__ norms, int nitems,
+                                                        float* __restrict__ ratio) {
+  float r = 0.f;
+  for (int b = threadIdx.x; b < nitems; b += 256) {
+    const float d = norms[2 * b], x = norms[2 * b + 1];
+    r = fmaxf(r, x > 0.f ? d / x : (d > 0.f ? 1.f : 0.f));
+  }
+  __shared__ float s_r[4];
+  const unsigned k = wave_max_u32(__float_as_uint(r));
+  if ((threadIdx.x & 63) == 0) s_r[threadIdx.x >> 6] = __uint_as_float(k);
+  __syncthreads();
+  if (threadIdx.x == 0) *ratio = fmaxf(fmaxf(s_r[0], s_r[1]), fmaxf(s_r[2], s_r[3]));
 }
 
 // ---------------------------------------------------------------------------
@@ -155,8 +190,8 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   DGAMI_ARG_CHECK(precision == DGAMI_MIXED_FP32_IR || precision == DGAMI_MIXED_BF16X3_IR,
                   "ladder: unknown mixed-precision mode");
   // the split-BF16 factors carry ~2^-16 relative error per product instead of 2^-24, so
-  // each refinement step gains fewer digits: two more solves reach the same accuracy
-  const int solves = precision == DGAMI_MIXED_BF16X3_IR ? MIXED_SOLVES + 2 : MIXED_SOLVES;
+  // each refinement step gains fewer digits and more steps may be needed
+  const int max_solves = precision == DGAMI_MIXED_BF16X3_IR ? MIXED_MAX_SOLVES_BF16 : MIXED_MAX_SOLVES_FP32;
   const int n = a.n, T = a.T;
   DGAMI_ARG_CHECK(T <= 16, "ladder: at most 4 orbitals in the mixed-precision form");
   DGAMI_ARG_CHECK(n <= 1536, "ladder: compound size above 1536 is not supported");
@@ -180,7 +215,8 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   if (!blocked_alloc(st, n, ld, n, bitems, WS_MIXED)) return -2;
   st.swap_left = 1;
   st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
-  char* vec = static_cast<char*>(workspace(WS_MIXED_VEC, 2 * bitems * n * T * sizeof(double2) + bitems * n * sizeof(int)));
+  char* vec = static_cast<char*>(workspace(WS_MIXED_VEC, 2 * bitems * n * T * sizeof(double2) + bitems * n * sizeof(int) +
+                                                             (2 * bitems + 4) * sizeof(float)));
   if (!vec) return -2;
   MixedArgs m;
   m.a = a;
@@ -189,6 +225,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   m.R = m.X + bitems * n * T;
   int* perm = reinterpret_cast<int*>(m.R + bitems * n * T);
   m.perm = perm;
+  float* norms = reinterpret_cast<float*>(perm + bitems * n);  // [bitems][2] + the batch ratio
   m.dinv = st.dinv;
   m.ld = ld;
   m.gq = gq;
@@ -212,7 +249,8 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     rc = tri_diag_inverse(f, nit, true, stream);
     if (rc) return rc;
     const dim3 vgrid(ceil_div(static_cast<size_t>(n) * T, 256), nit);
-    for (int s = 0; s < solves; ++s) {
+    double prev = 0.0;
+    for (int s = 0; s < max_solves; ++s) {
       if (s > 0) {
         ZgemmArgs g;
         g.A = a.gamma + static_cast<size_t>(g0) * n * n;
@@ -241,9 +279,18 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
       if (rc) return rc;
       rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
       if (rc) return rc;
-      hipLaunchKernelGGL(mix_accumulate_kernel, vgrid, dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0,
-                         s == solves - 1 ? 1 : 0);
+      hipLaunchKernelGGL(mix_accumulate_kernel, dim3(nit), dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0, norms);
       DGAMI_HIP_CHECK(hipGetLastError());
+      if (s == 0) continue;
+      hipLaunchKernelGGL(mix_ratio_kernel, dim3(1), dim3(256), 0, stream, norms, nit, norms + 2 * bitems);
+      DGAMI_HIP_CHECK(hipGetLastError());
+      float ratio = 0.f;
+      DGAMI_HIP_CHECK(hipMemcpyAsync(&ratio, norms + 2 * bitems, sizeof(float), hipMemcpyDeviceToHost, stream));
+      DGAMI_HIP_CHECK(hipStreamSynchronize(stream));
+      const double r = ratio;
+      const double contraction = (s >= 2 && prev > 0.0) ? r / prev : r;
+      prev = r;
+      if (r * contraction < MIXED_TOL || !(r == r)) break;
     }
     rc = blocked_copy_info(st.info, a.info, item0, nit, stream);
     if (rc) return rc;
