@@ -27,8 +27,12 @@ if ROOT not in sys.path:
     sys.path.insert(0, ROOT)
 
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
-PEAK_FP32_TFLOPS = 157.3
+PEAK_FP32_TFLOPS = 157.3  # dense FP32 matrix cores
+PEAK_BF16_TFLOPS = 2500.0  # dense BF16 matrix cores; the split-BF16 form needs three products
 PEAK_HBM_GBS = 8000.0
+# HBM bytes per trailing-update launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
+# (scaled by the copy-kernel factors), one "precision workload GB_per_launch" line each
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_traffic.txt")
 
 
 def parse():
@@ -71,8 +75,11 @@ def copy_calibration():
 
 
 def cpu_baseline(model, target_s):
-    """Time the reference's per-q solve path (NumPy/LAPACK, complex128) on a bounded sample."""
+    """Time the reference's per-q solve path (NumPy/LAPACK, complex128) on a bounded sample,
+    with all host threads and with one thread (the reference runs one MPI rank per core with
+    OMP_NUM_THREADS=1)."""
     import numpy as np
+    from threadpoolctl import threadpool_limits
 
     from dgamore_amd.brillouin_zone import KGrid
     from oracle import dga_oracle as O
@@ -85,20 +92,28 @@ def cpu_baseline(model, target_s):
     chi0 = O.bubble_direct(g, niw, niv, q_list, beta)
     inv = O.block_invert_vn1(chi0)
     zero_v = np.zeros((1,) + (nb,) * 4)
-    solves = 0
-    t0 = time.perf_counter()
-    while True:
-        for ch in ("dens", "magn"):
-            ur = O.u_as_channel(model.u, ch)
-            O.ladder_sum_solve(inv, model.gamma[ch], ur, zero_v, beta)
-            solves += niw + 1
-        el = time.perf_counter() - t0
-        if el > target_s:
-            break
+
+    def timed(seconds):
+        solves = 0
+        t0 = time.perf_counter()
+        while True:
+            for ch in ("dens", "magn"):
+                ur = O.u_as_channel(model.u, ch)
+                O.ladder_sum_solve(inv, model.gamma[ch], ur, zero_v, beta)
+                solves += niw + 1
+            el = time.perf_counter() - t0
+            if el > seconds:
+                return solves, el
+
     threads = len(os.sched_getaffinity(0))
+    solves, el = timed(target_s)
+    with threadpool_limits(limits=1):
+        solves1, el1 = timed(target_s / 3)
     return {"value": solves / el, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"{solves} (q,w,channel) ladder solves of n={nb * nb * 2 * niv} via numpy.linalg.solve "
-                      f"(complex128, the reference's solve-based path) in {el:.1f} s, BLAS threads <= {threads}"}
+                      f"(complex128, the reference's solve-based path) in {el:.1f} s with BLAS threads <= {threads}; "
+                      f"one thread: {solves1} solves in {el1:.1f} s",
+            "value_1_thread": solves1 / el1}
 
 
 def time_steps(pipe, steps, warmup, stream):
@@ -131,6 +146,73 @@ def time_ladder(pipe, nrep, stream):
     lb.record(stream)
     torch.cuda.synchronize()
     return la.elapsed_time(lb) / nrep
+
+
+def time_sigma(pipe, nrep, stream):
+    """The Sigma sum (irr -> FBZ kernel, k-convolution with G, nu sum) alone."""
+    import torch
+
+    from dgamore_amd import kernels as K
+
+    last, m = pipe.last, pipe.m
+    la, lb = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    la.record(stream)
+    for _ in range(nrep):
+        K.sigma_from_kernel(last["kernel"], pipe.irrk_inv, last["g"], m.nk, m.niw, m.beta)
+    lb.record(stream)
+    torch.cuda.synchronize()
+    return la.elapsed_time(lb) / nrep
+
+
+def dominant_kernel(pipe, stream, ms_ladder, precision_name):
+    """Roofline of the dominant kernel of the ladder solve, timed with HIP events on its stream.
+
+    Compound sizes above 128 run the blocked LU, whose trailing-matrix update dominates: the
+    library brackets each of its launches with events (dgami_profile_*) and returns the
+    algorithmic bytes (C read + written, A and B panels read once) and flops (8 m n k).
+    Smaller sizes run one register-resident kernel per call, timed as the whole ladder call.
+    """
+    import torch
+
+    from dgamore_amd import _lib
+    from dgamore_amd import kernels as K
+
+    m = pipe.m
+    peaks = {_lib.PLAIN_FP64: PEAK_FP64_TFLOPS, _lib.MIXED_FP32_IR: PEAK_FP32_TFLOPS,
+             _lib.MIXED_BF16X3_IR: PEAK_BF16_TFLOPS / 3}
+    peak_tf = peaks[pipe.precision]
+    _lib.profile_enable(True)
+    K.ladder_solve_sum(pipe.last["chi0_inv"], pipe.gamma, pipe.u_r, None, m.beta, pipe.precision, check_info=False)
+    torch.cuda.synchronize()
+    prof = _lib.profile_read()
+    _lib.profile_enable(False)
+    if prof["launches"] == 0:
+        flops = pipe.solves_per_step() * pipe.ladder_flops_per_solve()
+        tf = flops / (ms_ladder * 1e-3) / 1e12
+        return {"bound": "mfma", "kernel": "register-resident ladder kernel (one launch per call)",
+                "achieved": tf, "peak": peak_tf, "unit": "TFLOP/s", "frac": tf / peak_tf, "traffic": None,
+                "launches": 1, "avg_launch_ms": ms_ladder}
+    avg_ms = prof["ms"] / prof["launches"]
+    tf = prof["flops"] / (prof["ms"] * 1e-3) / 1e12
+    gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
+    # the bound is the resource whose ideal time for the algorithmic work is longer
+    mfma_bound = prof["flops"] / (peak_tf * 1e12) >= prof["bytes"] / (PEAK_HBM_GBS * 1e9)
+    traffic = None
+    if os.path.exists(TRAFFIC_FILE):
+        for line in open(TRAFFIC_FILE):
+            f = line.split()
+            if len(f) == 3 and f[0] == precision_name and f[1] == m.name:
+                traffic = float(f[2]) * 1e9
+    out = {"kernel": "trailing-matrix update of the blocked LU (blk_gemm_*)", "launches": prof["launches"],
+           "avg_launch_ms": avg_ms, "traffic": traffic,
+           "algorithmic_bytes_per_launch": prof["bytes"] / prof["launches"],
+           "algorithmic_flops_per_launch": prof["flops"] / prof["launches"],
+           "achieved_TFLOPs": tf, "achieved_GBps": gbs}
+    if mfma_bound:
+        out.update(bound="mfma", achieved=tf, peak=peak_tf, unit="TFLOP/s", frac=tf / peak_tf)
+    else:
+        out.update(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s", frac=gbs / PEAK_HBM_GBS)
+    return out
 
 
 def rel_err(a, b):
@@ -176,8 +258,14 @@ def main():
     solves = pipe.solves_per_step()
     flops = solves * pipe.ladder_flops_per_solve()
     achieved = flops / (ms_ladder * 1e-3) / 1e12
-    peak = PEAK_FP64_TFLOPS if prec == _lib.PLAIN_FP64 else PEAK_FP32_TFLOPS
+    peak = {_lib.PLAIN_FP64: PEAK_FP64_TFLOPS, _lib.MIXED_FP32_IR: PEAK_FP32_TFLOPS,
+            _lib.MIXED_BF16X3_IR: PEAK_BF16_TFLOPS / 3}[prec]
     info_bad = int(torch.count_nonzero(pipe.last["info"]).item())
+    ms_sigma = time_sigma(pipe, max(2, args.steps), stream)
+    roof = dominant_kernel(pipe, stream, ms_ladder, args.precision)
+    print(f"[bench] {args.precision}: Sigma sum {ms_sigma:.3f} ms; dominant kernel {roof['kernel']}: "
+          f"{roof['launches']} launches, {roof['avg_launch_ms']:.3f} ms avg, {roof['frac']:.3f} of {roof['bound']} peak",
+          file=sys.stderr, flush=True)
 
     compare = None
     if not args.no_compare and prec != _lib.PLAIN_FP64:
@@ -218,13 +306,14 @@ def main():
                    "nq_irr": pipe.nq, "niw": model.niw, "niv": model.niv, "n_compound": n,
                    "solves_per_step": solves, "precision": args.precision},
         "sigma_wall_ms": ms_step,
+        "sigma_sum_ms": ms_sigma,
         "ladder_ms": ms_ladder,
         "ladder_solves_per_s": solves / (ms_ladder * 1e-3),
+        "ladder_roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                            "frac": achieved / peak, "flops_per_solve": pipe.ladder_flops_per_solve()},
         "singular_solves": info_bad,
         "compare_fp64": compare,
-        "roofline": {"bound": "mfma", "kernel": "ladder solve launch sequence (dgami_ladder_solve_sum)",
-                     "achieved": achieved, "peak": peak, "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": None, "flops_per_solve": pipe.ladder_flops_per_solve()},
+        "roofline": roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

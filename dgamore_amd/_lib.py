@@ -30,6 +30,8 @@ _i64 = ctypes.c_int64
 SIGNATURES = {
     "dgami_last_error": ([], ctypes.c_char_p),
     "dgami_version": ([], _i),
+    "dgami_profile_enable": ([_i], _i),
+    "dgami_profile_read": ([_c, _c, _c, _c], _i),
     "dgami_fft_grid": ([_c, _c, _i64, _i, _i, _i, _i, _d, _c], _i),
     "dgami_bubble_irrq": ([_i, _i, _i, _i, _i, _i, _i, _d, _c, _c, _i, _c, _c], _i),
     "dgami_chi0_prepare": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _c], _i),
@@ -82,3 +84,15 @@ def check(rc, what):
 
 def call(name, *args):
     check(getattr(load(), name)(*args), name)
+
+
+def profile_enable(on=True):
+    """Start (and reset) or stop the library's trailing-update kernel timing."""
+    call("dgami_profile_enable", int(bool(on)))
+
+
+def profile_read():
+    """Totals since profile_enable(True): {'ms', 'bytes', 'flops', 'launches'} (waits for the events)."""
+    ms, by, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
+    call("dgami_profile_read", ctypes.addressof(ms), ctypes.addressof(by), ctypes.addressof(fl), ctypes.addressof(n))
+    return {"ms": ms.value, "bytes": by.value, "flops": fl.value, "launches": n.value}

@@ -20,6 +20,44 @@ std::vector<Slot> g_slots;
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+namespace {
+struct Profile {
+  bool on = false;
+  std::vector<std::pair<hipEvent_t, hipEvent_t>> events;
+  double bytes = 0, flops = 0;
+};
+std::mutex g_prof_mutex;
+Profile g_prof;
+}  // namespace
+
+bool profile_on() { return g_prof.on; }
+
+int profile_begin(hipStream_t stream, hipEvent_t* start) {
+  DGAMI_HIP_CHECK(hipEventCreate(start));
+  DGAMI_HIP_CHECK(hipEventRecord(*start, stream));
+  return 0;
+}
+
+int profile_end(hipStream_t stream, hipEvent_t start, double bytes, double flops) {
+  hipEvent_t stop;
+  DGAMI_HIP_CHECK(hipEventCreate(&stop));
+  DGAMI_HIP_CHECK(hipEventRecord(stop, stream));
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  g_prof.events.emplace_back(start, stop);
+  g_prof.bytes += bytes;
+  g_prof.flops += flops;
+  return 0;
+}
+
+static void profile_clear() {
+  for (auto& e : g_prof.events) {
+    (void)hipEventDestroy(e.first);
+    (void)hipEventDestroy(e.second);
+  }
+  g_prof.events.clear();
+  g_prof.bytes = g_prof.flops = 0;
+}
 const char* last_error() { return g_last_error.c_str(); }
 
 void* workspace(int slot, size_t bytes) {
@@ -57,6 +95,31 @@ void* workspace(int slot, size_t bytes) {
 extern "C" {
 
 const char* dgami_last_error(void) { return dgami::last_error(); }
+
+int dgami_profile_enable(int on) {
+  std::lock_guard<std::mutex> lock(dgami::g_prof_mutex);
+  dgami::profile_clear();
+  dgami::g_prof.on = on != 0;
+  return 0;
+}
+
+int dgami_profile_read(double* ms, double* bytes, double* flops, int64_t* launches) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(ms && bytes && flops && launches, "profile_read: null pointer");
+  std::lock_guard<std::mutex> lock(g_prof_mutex);
+  double total = 0;
+  for (auto& e : g_prof.events) {
+    DGAMI_HIP_CHECK(hipEventSynchronize(e.second));
+    float t = 0.f;
+    DGAMI_HIP_CHECK(hipEventElapsedTime(&t, e.first, e.second));
+    total += t;
+  }
+  *ms = total;
+  *bytes = g_prof.bytes;
+  *flops = g_prof.flops;
+  *launches = static_cast<int64_t>(g_prof.events.size());
+  return 0;
+}
 
 int dgami_version(void) { return 1; }
 

@@ -164,6 +164,12 @@ const char* last_error();
     }                                                                                      \
   } while (0)
 
+// Kernel timing (dgami_profile_enable): profile_begin records a start event on the stream,
+// profile_end the stop event together with the launch's algorithmic bytes and flops.
+bool profile_on();
+int profile_begin(hipStream_t stream, hipEvent_t* start);
+int profile_end(hipStream_t stream, hipEvent_t start, double bytes, double flops);
+
 // Grow-only device scratch owned by the library, one slot per purpose id.
 // Returns nullptr (and sets the error) when the allocation fails.
 void* workspace(int slot, size_t bytes);

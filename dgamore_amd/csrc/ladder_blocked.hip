@@ -521,35 +521,47 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
       accr[i][j] = V4{0.f, 0.f, 0.f, 0.f};
       acci[i][j] = V4{0.f, 0.f, 0.f, 0.f};
     }
-  for (int kc = 0; kc < g.kw; kc += KC) {
+  // operand chunks of KC columns of k: raw values are loaded to registers (the next chunk's
+  // loads are issued before the current chunk's products), then split and packed into LDS
+  constexpr int NP = 64 * (KC / 2) / 256;  // k pairs per thread and operand
+  float2 ra[NP][2], rb[NP][2];
+  auto load_chunk = [&](int kc) {
     const int kn = min(KC, g.kw - kc);
-    if (kc > 0) __syncthreads();
-    // each thread splits a pair of consecutive k of one row (A) or one column (B) and writes
-    // 4 packed bf16 per array
-    for (int e = tid; e < 64 * (KC / 2); e += 256) {
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int e = tid + 256 * p;
       const int r = e / (KC / 2), s2 = 2 * (e - (e / (KC / 2)) * (KC / 2));
       const bool in = rt + r < g.r1;
       const float2* src = W + static_cast<size_t>(rt + r) * ld + g.k0 + kc + s2;
-      const float2 v0 = (in && s2 < kn) ? src[0] : make_float2(0.f, 0.f);
-      const float2 v1 = (in && s2 + 1 < kn) ? src[1] : make_float2(0.f, 0.f);
-      const Bf16Split r0 = bf16_split(v0.x), i0 = bf16_split(v0.y), r1 = bf16_split(v1.x), i1 = bf16_split(v1.y);
-      typedef __bf16 B4 __attribute__((ext_vector_type(4)));
+      ra[p][0] = (in && s2 < kn) ? src[0] : make_float2(0.f, 0.f);
+      ra[p][1] = (in && s2 + 1 < kn) ? src[1] : make_float2(0.f, 0.f);
+      const int s3 = 2 * (e / 64), c = e - (e / 64) * 64;
+      const bool inc = ct + c < g.c1;
+      rb[p][0] = (inc && s3 < kn) ? W[static_cast<size_t>(g.k0 + kc + s3) * ld + ct + c] : make_float2(0.f, 0.f);
+      rb[p][1] = (inc && s3 + 1 < kn) ? W[static_cast<size_t>(g.k0 + kc + s3 + 1) * ld + ct + c] : make_float2(0.f, 0.f);
+    }
+  };
+  load_chunk(0);
+  for (int kc = 0; kc < g.kw; kc += KC) {
+    if (kc > 0) __syncthreads();
+    typedef __bf16 B4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+    for (int p = 0; p < NP; ++p) {
+      const int e = tid + 256 * p;
+      const int r = e / (KC / 2), s2 = 2 * (e - (e / (KC / 2)) * (KC / 2));
+      const Bf16Split r0 = bf16_split(ra[p][0].x), i0 = bf16_split(ra[p][0].y), r1 = bf16_split(ra[p][1].x),
+                      i1 = bf16_split(ra[p][1].y);
       *reinterpret_cast<B4*>(&s_a[0][r][2 * s2]) = B4{-r0.hi, i0.hi, -r1.hi, i1.hi};
       *reinterpret_cast<B4*>(&s_a[1][r][2 * s2]) = B4{-r0.lo, i0.lo, -r1.lo, i1.lo};
       *reinterpret_cast<B4*>(&s_a[2][r][2 * s2]) = B4{-i0.hi, -r0.hi, -i1.hi, -r1.hi};
       *reinterpret_cast<B4*>(&s_a[3][r][2 * s2]) = B4{-i0.lo, -r0.lo, -i1.lo, -r1.lo};
+      const int s3 = 2 * (e / 64), c = e - (e / 64) * 64;
+      const Bf16Split b0 = bf16_split(rb[p][0].x), j0 = bf16_split(rb[p][0].y), b1 = bf16_split(rb[p][1].x),
+                      j1 = bf16_split(rb[p][1].y);
+      *reinterpret_cast<B4*>(&s_b[0][c][2 * s3]) = B4{b0.hi, j0.hi, b1.hi, j1.hi};
+      *reinterpret_cast<B4*>(&s_b[1][c][2 * s3]) = B4{b0.lo, j0.lo, b1.lo, j1.lo};
     }
-    for (int e = tid; e < 64 * (KC / 2); e += 256) {
-      const int s2 = 2 * (e / 64), c = e - (e / 64) * 64;
-      const bool in = ct + c < g.c1;
-      const float2 v0 = (in && s2 < kn) ? W[static_cast<size_t>(g.k0 + kc + s2) * ld + ct + c] : make_float2(0.f, 0.f);
-      const float2 v1 =
-          (in && s2 + 1 < kn) ? W[static_cast<size_t>(g.k0 + kc + s2 + 1) * ld + ct + c] : make_float2(0.f, 0.f);
-      const Bf16Split r0 = bf16_split(v0.x), i0 = bf16_split(v0.y), r1 = bf16_split(v1.x), i1 = bf16_split(v1.y);
-      typedef __bf16 B4 __attribute__((ext_vector_type(4)));
-      *reinterpret_cast<B4*>(&s_b[0][c][2 * s2]) = B4{r0.hi, i0.hi, r1.hi, i1.hi};
-      *reinterpret_cast<B4*>(&s_b[1][c][2 * s2]) = B4{r0.lo, i0.lo, r1.lo, i1.lo};
-    }
+    if (kc + KC < g.kw) load_chunk(kc + KC);
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < 2 * KC; ks += 32) {
@@ -654,6 +666,9 @@ struct BlockedSteps {
     const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw, tall && mode == BLK_GEMM_VECTOR ? 128 : 64);
     const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
     DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
+    hipEvent_t start = nullptr;
+    const bool prof = profile_on();
+    if (prof && profile_begin(stream, &start)) return -2;
     if (mode == BLK_GEMM_VECTOR) {
       if constexpr (tall)
         hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
@@ -670,6 +685,11 @@ struct BlockedSteps {
                            nb);
     }
     DGAMI_HIP_CHECK(hipGetLastError());
+    if (prof) {
+      const double m = r1 - r0, c = c1 - c0, es = sizeof(S);
+      const double bytes = static_cast<double>(nb) * es * (2.0 * m * c + m * kw + static_cast<double>(kw) * c);
+      if (profile_end(stream, start, bytes, static_cast<double>(nb) * 8.0 * m * c * kw)) return -2;
+    }
     return 0;
   }
 };

@@ -59,6 +59,16 @@ const char* dgami_last_error(void);
 /* ABI version of the library (incremented on incompatible changes). */
 int dgami_version(void);
 
+/* Kernel timing of the trailing-matrix updates of the blocked ladder LU (the dominant
+ * kernel for compound sizes above 128). No counterpart in the reference: it is the
+ * measurement facility behind the roofline figures. While enabled, every trailing-update
+ * launch is bracketed by HIP events on its launch stream and its algorithmic work is
+ * accumulated: bytes = C tile read + written + the A and B panels read once, flops =
+ * 8 m n k. dgami_profile_enable(1) resets the counters; dgami_profile_read waits for the
+ * recorded events and returns the totals (time in ms). */
+int dgami_profile_enable(int on);
+int dgami_profile_read(double* ms, double* bytes, double* flops, int64_t* launches);
+
 /* Batched FFT over momentum grids. in/out: [batch][nx*ny*nz] (may alias).
  * sign -1: forward, unnormalised, exp(-i k.R); sign +1: backward exp(+i k.R);
  * the result is multiplied by `scale`.
