@@ -1,6 +1,6 @@
 """Benchmark of the ladder-DGA hot path on one MI355X.
 
-    python bench.py [--steps K] [--warmup W] [--config 2] [--precision mixed|plain] [--no-compare]
+    python bench.py [--steps K] [--warmup W] [--config 2] [--precision bf16x3|mixed|plain] [--no-compare]
 
 A step is one full non-local SDE iteration (G(k) -> bubble -> double-counting
 kernel -> ladder solves of both channels -> three-leg vertices and
@@ -8,12 +8,14 @@ susceptibilities -> kernel -> Sigma(k)) on synthetic inputs already resident
 in HBM. The default workload is the 3-orbital t2g model on a 24x24 k-grid
 (91 irreducible q, niw=30, niv=40: 5642 ladder solves of n=720 per step).
 The JSON line reports (q, w) ladder solves per second of the whole step in
-the headline form (mixed: complex64 LU with FP64 iterative refinement), the
-step wall time, the same for the plain complex128 form with the relative
-errors of chi*, chi and Sigma between the two, the roofline of the dominant
-work (the batched ladder solve) measured with HIP events on the launch
-stream, and a CPU baseline: the NumPy restatement of the reference's per-q
-solve path timed on the host on a bounded sample of the same workload.
+the headline form (bf16x3: complex64 LU whose trailing updates run as three
+split-BF16 matrix-core products, refined in FP64; mixed: the same with FP32
+matrix-core updates; plain: complex128), the step wall time, the same for the
+plain complex128 form with the relative errors of chi*, chi and Sigma between
+the two, the roofline of the dominant kernel (the trailing-matrix update of
+the blocked LU) timed with HIP events on its launch stream, and a CPU
+baseline: the NumPy restatement of the reference's per-q solve path timed on
+the host on a bounded sample of the same workload.
 """
 
 import argparse
@@ -41,7 +43,7 @@ def parse():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--config", type=int, default=2)
-    ap.add_argument("--precision", default="mixed", choices=["plain", "mixed", "bf16x3"],
+    ap.add_argument("--precision", default="bf16x3", choices=["plain", "mixed", "bf16x3"],
                     help="form whose throughput is the headline value (mixed: complex64 LU on the FP32 matrix "
                          "cores + FP64 refinement; bf16x3: the same with split-BF16 MFMA trailing updates)")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison run")
@@ -200,7 +202,7 @@ def dominant_kernel(pipe, stream, ms_ladder, precision_name):
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         for line in open(TRAFFIC_FILE):
-            f = line.split()
+            f = line.split("#", 1)[0].split()
             if len(f) == 3 and f[0] == precision_name and f[1] == m.name:
                 traffic = float(f[2]) * 1e9
     out = {"kernel": "trailing-matrix update of the blocked LU (blk_gemm_*)", "launches": prof["launches"],
