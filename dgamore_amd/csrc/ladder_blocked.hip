@@ -173,11 +173,8 @@ __device__ __forceinline__ void panel_columns(S (&P)[RPT][NB], PanelShared<S, NB
   (panel_column<S, NB, RPT, Cs>(P, sh, st, b, p0, nbw), ...);
 }
 
-// WPE: minimum waves per SIMD the register budget must allow (6 lets two 12-wavefront
-// complex64 panels of n <= 768 share a CU).
-template <typename S, int NB, int RPT, int MAXTH, int WPE>
-__global__ void __launch_bounds__(MAXTH) __attribute__((amdgpu_waves_per_eu(WPE)))
-blk_panel_kernel(BlockedState<S> st, int p0, int slot) {
+template <typename S, int NB, int RPT, int MAXTH>
+__global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, int p0, int slot) {
   const int b = blockIdx.x;
   const int n = st.n, ld = st.ld;
   const int TH = blockDim.x;
@@ -645,8 +642,7 @@ struct BlockedSteps {
     const int rows = st.n - p0;
     const int th = ((rows + RPT - 1) / RPT + 63) / 64 * 64;
     DGAMI_ARG_CHECK(th <= MAXTH, "ladder: compound size too large for the panel kernel");
-    constexpr int WPE = (std::is_same<S, float2>::value && MAXTH == 768) ? 6 : 1;
-    hipLaunchKernelGGL((blk_panel_kernel<S, NB, RPT, MAXTH, WPE>), dim3(nb), dim3(th), 0, stream, st, p0, slot);
+    hipLaunchKernelGGL((blk_panel_kernel<S, NB, RPT, MAXTH>), dim3(nb), dim3(th), 0, stream, st, p0, slot);
     DGAMI_HIP_CHECK(hipGetLastError());
     return 0;
   }
@@ -748,7 +744,6 @@ int blocked_factor<double2>(const LadderArgs& a, BlockedState<double2>& st, int 
 
 template <>
 int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int item0, int nb, hipStream_t stream) {
-  if (st.n <= 768) return blocked_factor_nb<float2, 32, 1, 768>(a, st, item0, nb, stream);
   if (st.n <= 1024) return blocked_factor_nb<float2, 32, 1, 1024>(a, st, item0, nb, stream);
   return blocked_factor_nb<float2, 16, 3, 512>(a, st, item0, nb, stream);
 }
