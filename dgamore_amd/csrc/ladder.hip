@@ -16,17 +16,22 @@ int ladder_col_variant(int n, int nrhs);
 int ladder_col_launch(int variant, const LadderArgs& a, hipStream_t stream);
 int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream);
 
-// Kernel family of the plain FP64 solve: "col" (column-cyclic registers, n <= 128),
-// "reg" (2D register tiles, n <= 128) or "blocked" (workspace LU, any n). The
-// environment variable DGAMI_LADDER_KERNEL forces one family for comparisons.
+// Kernel family of the plain FP64 solve: "reg" (2D register tiles) up to compound size
+// 64, the blocked workspace LU above it. Measured on 8192 matrices per call, the register
+// kernel leads at n = 40 (0.96 vs 1.85 ms) and ties at n = 64, while the blocked LU is
+// 1.7-2.7x faster from n = 72 on. The column-cyclic kernel ("col", n <= 128) is kept as an
+// alternative. The environment variable DGAMI_LADDER_KERNEL (col, reg or blocked) forces
+// one family for comparisons.
+constexpr int LADDER_REG_MAX_N = 64;
+
 int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
   const char* force = getenv("DGAMI_LADDER_KERNEL");
   const std::string f = force ? force : "";
-  if (f != "reg" && f != "blocked") {
+  if (f == "col") {
     const int v = ladder_col_variant(a.n, a.T);
     if (v >= 0) return ladder_col_launch(v, a, stream);
   }
-  if (f != "blocked") {
+  if (f == "reg" || (f.empty() && a.n <= LADDER_REG_MAX_N)) {
     const int v = ladder_reg_variant(a.n, a.T);
     if (v >= 0) return ladder_reg_launch(v, a, stream);
   }

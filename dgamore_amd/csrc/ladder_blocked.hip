@@ -52,7 +52,7 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
       if (j < st.n)
         val = ladder_elem(a, G, U, V, w, q, i, j);
       else if (j < st.ncol)
-        val = ladder_rhs(a, i, j - st.n);
+        val = ladder_rhs_at(a, w, q, i, j - st.n);
       Wb[static_cast<size_t>(i) * st.ld + j] = from_c128<S>(val);
     }
   }
@@ -746,6 +746,13 @@ template <>
 int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int item0, int nb, hipStream_t stream) {
   if (st.n <= 1024) return blocked_factor_nb<float2, 32, 1, 1024>(a, st, item0, nb, stream);
   return blocked_factor_nb<float2, 16, 3, 512>(a, st, item0, nb, stream);
+}
+
+int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,
+                      hipStream_t stream) {
+  const LadderArgs unused{};
+  BlockedSteps<double2, 32, 1, 768> k{unused, st, nb, stream};
+  return k.gemm(r0, r1, c0, c1, k0, kw);
 }
 
 int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream) {

@@ -47,6 +47,11 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
 template <typename S>
 int blocked_factor(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream);
 
+// W[r0:r1, c0:c1] -= W[r0:r1, k0:k0+kw] W[k0:k0+kw, c0:c1] on every complex128 problem of the
+// batch (the trailing-update kernel of the factorisation, reused by the block back-substitution).
+int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,
+                      hipStream_t stream);
+
 // info[item0 + b] = first zero pivot of problem b of the batch (no-op for info == nullptr).
 int blocked_copy_info(const int* st_info, int32_t* info, int item0, int nitems, hipStream_t stream);
 

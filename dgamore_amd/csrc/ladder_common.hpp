@@ -19,7 +19,12 @@ struct LadderArgs {
   int nv;    // 2 * niv
   int n;     // T * nv
   double inv_beta2, inv_beta;
+  int rhs_mode = 0;  // LadderRhs: right-hand sides carried by the blocked LU
 };
+
+// Right-hand sides of the blocked LU: the nb^2 indicator columns of the summed ladder, the
+// identity (full inverse) or the block-diagonal chi0^-1 (X chi0^-1 for the full vertex).
+enum LadderRhs { LADDER_RHS_INDICATOR = 0, LADDER_RHS_IDENTITY = 1, LADDER_RHS_CHI0INV = 2 };
 
 // Element (i, j) of the ladder matrix A(chan, w, q) in compound indices
 // (nonlocal_sde.py:93-95): chi0_inv delta_vv' + (gamma - u_r - v_r)/beta^2.
@@ -40,6 +45,19 @@ __device__ __forceinline__ double2 ladder_elem(const LadderArgs& a, const double
 // Indicator right-hand side B[i][r] = 1 iff the orbital pair of row i is r (four_point.py:661-668).
 __device__ __forceinline__ double2 ladder_rhs(const LadderArgs& a, int i, int r) {
   return make_double2((i / a.nv) == r ? 1.0 : 0.0, 0.0);
+}
+
+// Right-hand side B[i][r] of item (w, q) for every LadderRhs mode.
+__device__ __forceinline__ double2 ladder_rhs_at(const LadderArgs& a, int w, int q, int i, int r) {
+  if (a.rhs_mode == LADDER_RHS_IDENTITY) return make_double2(i == r ? 1.0 : 0.0, 0.0);
+  if (a.rhs_mode == LADDER_RHS_CHI0INV) {
+    const int nv = a.nv, T = a.T;
+    const int ri = i / nv, vi = i - ri * nv;
+    const int cj = r / nv, vj = r - cj * nv;
+    if (vi != vj) return make_double2(0.0, 0.0);
+    return a.chi0_inv[((static_cast<size_t>(w) * T * T + ri * T + cj) * nv + vi) * a.nq + q];
+  }
+  return ladder_rhs(a, i, r);
 }
 
 // Output element chi_aux_sum[chan][w][(ri, r)][vi][q] = x[i][r] / beta.

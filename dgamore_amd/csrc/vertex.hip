@@ -1,0 +1,352 @@
+// Full inverse of the ladder matrices, the full vertex built from it, and the
+// particle-particle pairing vertices of the Eliashberg equation (complex128).
+//
+//   X(q,w)   = A^-1,  A = chi0^-1 delta_vv' + (Gamma_r - U_r - V_r^q) / beta^2
+//   F_r(q,w) = beta^2 (chi0^-1 - chi0^-1 X chi0^-1) + (u - u chi_r u) vrg(v) vrg(v'),  u = U_r + V_r^q
+//
+// Every (q, w) matrix is factored by the batched blocked LU of the ladder path, which carries
+// n right-hand-side columns: the identity for X, the block-diagonal chi0^-1 for F (so that
+// the columns end as X chi0^-1 and the two GEMMs of chi0^-1 X chi0^-1 reduce to one
+// nb^2-wide product per row). The back substitution runs in steps of 64 rows from the bottom:
+// the two inverse 32x32 diagonal blocks of U are applied in LDS, then the rows above are
+// updated by the trailing-update GEMM of the factorisation.
+#include "dgami.h"
+#include "dgami_internal.hpp"
+#include "dgami_launch.hpp"
+#include "ladder_blocked.hpp"
+#include "tri_solve.hpp"
+
+namespace dgami {
+
+constexpr int VTX_STEP = 2 * TRI_B;  // rows per back-substitution step
+
+// s_y[h*TRI_B + r][lane] = (D_h s_y_h)[r][lane] for the 32 rows of half h; D_h = inverse of the
+// diagonal block of U (row-major TRI_B x TRI_B). Thread (grp, lane) forms rows grp*8 .. grp*8+7.
+__device__ __forceinline__ void vtx_apply_inverse(double2 (&s_y)[VTX_STEP][65], double2 (&s_m)[TRI_B][TRI_B + 1],
+                                                  const double2* __restrict__ D, int h, int grp, int lane) {
+  __syncthreads();
+  for (int e = threadIdx.x; e < TRI_B * TRI_B; e += 256) s_m[e / TRI_B][e % TRI_B] = D[e];
+  __syncthreads();
+  double2 acc[8];
+#pragma unroll
+  for (int i = 0; i < 8; ++i) acc[i] = make_double2(0.0, 0.0);
+#pragma unroll 4
+  for (int k = 0; k < TRI_B; ++k) {
+    const double2 y = s_y[h * TRI_B + k][lane];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = cfma(acc[i], s_m[grp * 8 + i][k], y);
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < 8; ++i) s_y[h * TRI_B + grp * 8 + i][lane] = acc[i];
+}
+
+// One back-substitution step on rows [c0, c0 + cb) (c0 a multiple of 64, cb <= 64) of the
+// right-hand-side columns [n, n + nr): rows below c0 + cb are final and already eliminated
+// from these rows, so with the halves 0 = [c0, c0+32) and 1 = [c0+32, c0+cb)
+//   X_1 = U_11^-1 Y_1,  Y_0 -= U_01 X_1,  X_0 = U_00^-1 Y_0.
+// grid: xcd_grid(nprob, ntile) workgroups of 256 threads, one 64-column tile each.
+__global__ void __launch_bounds__(256) vtx_backsolve_kernel(BlockedState<double2> st, const double2* __restrict__ dinvblk,
+                                                            int nblk, int c0, int cb, int nr, int ntile, int nprob) {
+  int b, tile;
+  if (!xcd_tile(blockIdx.x, nprob, ntile, b, tile)) return;
+  const int n = st.n, ld = st.ld;
+  double2* W = st.W + static_cast<size_t>(b) * n * ld;
+  const int cc = n + tile * 64;
+  __shared__ double2 s_y[VTX_STEP][65];
+  __shared__ double2 s_m[TRI_B][TRI_B + 1];
+  const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
+  const bool colok = tile * 64 + lane < nr;
+  for (int r = grp; r < VTX_STEP; r += 4)
+    s_y[r][lane] = (r < cb && colok) ? W[static_cast<size_t>(c0 + r) * ld + cc + lane] : make_double2(0.0, 0.0);
+  // inverse diagonal blocks of U: [nblk][2][TRI_B][TRI_B], U at offset TRI_B*TRI_B
+  const double2* Dinv = dinvblk + static_cast<size_t>(b) * nblk * 2 * TRI_B * TRI_B + TRI_B * TRI_B;
+  const int kb0 = c0 / TRI_B;
+  if (cb > TRI_B) {
+    vtx_apply_inverse(s_y, s_m, Dinv + static_cast<size_t>(kb0 + 1) * 2 * TRI_B * TRI_B, 1, grp, lane);
+    __syncthreads();
+    const int kw = cb - TRI_B;
+    for (int e = threadIdx.x; e < TRI_B * TRI_B; e += 256) {
+      const int r = e / TRI_B, k = e % TRI_B;
+      s_m[r][k] = k < kw ? W[static_cast<size_t>(c0 + r) * ld + c0 + TRI_B + k] : make_double2(0.0, 0.0);
+    }
+    __syncthreads();
+    double2 acc[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) acc[i] = s_y[grp * 8 + i][lane];
+#pragma unroll 4
+    for (int k = 0; k < TRI_B; ++k) {
+      const double2 x = s_y[TRI_B + k][lane];
+#pragma unroll
+      for (int i = 0; i < 8; ++i) acc[i] = cfms(acc[i], s_m[grp * 8 + i][k], x);
+    }
+    // each thread rewrites only the elements it read
+#pragma unroll
+    for (int i = 0; i < 8; ++i) s_y[grp * 8 + i][lane] = acc[i];
+  }
+  vtx_apply_inverse(s_y, s_m, Dinv + static_cast<size_t>(kb0) * 2 * TRI_B * TRI_B, 0, grp, lane);
+  __syncthreads();
+  if (colok)
+    for (int r = grp; r < cb; r += 4) W[static_cast<size_t>(c0 + r) * ld + cc + lane] = s_y[r][lane];
+}
+
+// out[item][i][j] = X[i][j] (the right-hand-side columns after the back substitution).
+// grid (ceil(n/4), nprob), 256 threads.
+__global__ void __launch_bounds__(256) vtx_copy_kernel(BlockedState<double2> st, double2* __restrict__ out, int item0) {
+  const int b = blockIdx.y, n = st.n;
+  const double2* W = st.W + static_cast<size_t>(b) * n * st.ld + n;
+  double2* O = out + static_cast<size_t>(item0 + b) * n * n;
+  const int r1 = min(n, (blockIdx.x + 1) * 4);
+  for (int i = blockIdx.x * 4; i < r1; ++i)
+    for (int j = threadIdx.x; j < n; j += 256) O[static_cast<size_t>(i) * n + j] = W[static_cast<size_t>(i) * st.ld + j];
+}
+
+struct VertexArgs {
+  const double2* vrg;       // QF [nw][T*T][nv][nq] or nullptr (first term only)
+  const double2* chi_phys;  // QW [nw][T*T][nq]
+  double2* out;             // [nw][nq][n][n]
+  double beta2;
+};
+
+// Full vertex of one problem, rows (ri, v) for every ri < T at one v, every column j = (cj, v'):
+//   F = beta^2 (D(v) delta_vv' - sum_s D(v)[ri][s] Y[(s, v)][j]) + sum_s P(v)[ri][s] V(v')[s][cj]
+// with Y = X chi0^-1 (right-hand-side columns), D(v) = chi0^-1(q,w,v), V(v) = vrg(q,w,v) and
+// P(v) = (u - u chi_r u) V(v), all nb^2 x nb^2 in M order. grid (nv, nprob), 256 threads.
+template <int T>
+__global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedState<double2> st, VertexArgs o,
+                                                       int item0) {
+  constexpr int TT = T * T;
+  const int v = blockIdx.x, b = blockIdx.y;
+  const int item = item0 + b;
+  const int nq = a.nq, nv = a.nv, n = a.n;
+  const int q = item % nq, w = item / nq;
+  __shared__ double2 s_d[T][T + 1], s_u[T][T + 1], s_t[T][T + 1], s_p[T][T + 1];
+  const int tid = threadIdx.x;
+  const bool f2 = o.vrg != nullptr;
+  for (int e = tid; e < TT; e += 256) {
+    const int r = e / T, c = e % T;
+    s_d[r][c] = a.chi0_inv[((static_cast<size_t>(w) * TT + e) * nv + v) * nq + q];
+    if (f2) {
+      double2 u = a.u_r[e];
+      if (a.v_r) u = cadd(u, a.v_r[static_cast<size_t>(q) * TT + e]);
+      s_u[r][c] = u;
+    }
+  }
+  __syncthreads();
+  if (f2) {
+    for (int e = tid; e < TT; e += 256) {  // s_t = chi_r u
+      const int r = e / T, c = e % T;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int k = 0; k < T; ++k)
+        acc = cfma(acc, o.chi_phys[(static_cast<size_t>(w) * TT + r * T + k) * nq + q], s_u[k][c]);
+      s_t[r][c] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < TT; e += 256) {  // s_p = u - u chi_r u
+      const int r = e / T, c = e % T;
+      double2 acc = s_u[r][c];
+      for (int k = 0; k < T; ++k) acc = cfms(acc, s_u[r][k], s_t[k][c]);
+      s_p[r][c] = acc;
+    }
+    __syncthreads();
+    for (int e = tid; e < TT; e += 256)  // s_t = V(v)
+      s_t[e / T][e % T] = o.vrg[((static_cast<size_t>(w) * TT + e) * nv + v) * nq + q];
+    __syncthreads();
+    for (int e = tid; e < TT; e += 256) {  // s_u = P(v) = (u - u chi_r u) V(v)
+      const int r = e / T, c = e % T;
+      double2 acc = make_double2(0.0, 0.0);
+      for (int k = 0; k < T; ++k) acc = cfma(acc, s_p[r][k], s_t[k][c]);
+      s_u[r][c] = acc;
+    }
+    __syncthreads();
+  }
+  const double2* Y = st.W + static_cast<size_t>(b) * n * st.ld + n;
+  double2* O = o.out + static_cast<size_t>(item) * n * n;
+  for (int j = tid; j < n; j += 256) {
+    const int cj = j / nv, vj = j - cj * nv;
+    double2 y[T], vc[T];
+#pragma unroll
+    for (int s = 0; s < T; ++s) y[s] = Y[static_cast<size_t>(s * nv + v) * st.ld + j];
+    if (f2) {
+#pragma unroll
+      for (int s = 0; s < T; ++s) vc[s] = o.vrg[((static_cast<size_t>(w) * TT + s * T + cj) * nv + vj) * nq + q];
+    }
+#pragma unroll 1
+    for (int ri = 0; ri < T; ++ri) {
+      double2 dy = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int s = 0; s < T; ++s) dy = cfma(dy, s_d[ri][s], y[s]);
+      double2 f = csub(vj == v ? s_d[ri][cj] : make_double2(0.0, 0.0), dy);
+      f = cscale(f, o.beta2);
+      if (f2) {
+#pragma unroll
+        for (int s = 0; s < T; ++s) f = cfma(f, s_u[ri][s], vc[s]);
+      }
+      O[static_cast<size_t>(ri * nv + v) * n + j] = f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Pairing vertices at w_pp = 0 from F_dens and F_magn (half bosonic range [nw][nq][n][n]):
+//   F_pp[q][o1][o2][o3][o4][i][j] = -F(w = i - j)[(o1,o2, i + off)][(o4,o3, 2np-1-j + off)]   (w >= 0)
+//                                 = -conj F(-w)[(o1,o2, 2np-1-i + off)][(o4,o3, j + off)]      (w < 0)
+// with off = niv - np, zero where |w| >= nw; gamma_sing = 0.5 F_pp,d - 1.5 F_pp,m and
+// gamma_trip = 0.5 F_pp,d + 0.5 F_pp,m; without fm, gs = F_pp of fd alone. One thread per
+// output element, j fastest.
+__global__ void __launch_bounds__(256) vtx_pp_kernel(const double2* __restrict__ fd, const double2* __restrict__ fm,
+                                                     double2* __restrict__ gs, double2* __restrict__ gt, int nq, int nb,
+                                                     int nw, int niv, int np) {
+  const int T = nb * nb, TT = T * T, P2 = 2 * np, nv = 2 * niv, n = T * nv, off = niv - np;
+  const size_t total = static_cast<size_t>(nq) * TT * P2 * P2;
+  for (size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int j = static_cast<int>(idx % P2);
+    const int i = static_cast<int>((idx / P2) % P2);
+    const size_t rest = idx / (static_cast<size_t>(P2) * P2);
+    const int o = static_cast<int>(rest % TT), q = static_cast<int>(rest / TT);
+    const int o4 = o % nb, o3 = (o / nb) % nb, o2 = (o / T) % nb, o1 = o / (T * nb);
+    const int ri = o1 * nb + o2, cj = o4 * nb + o3;
+    const int w = i - j;
+    double2 vd = make_double2(0.0, 0.0), vm = vd;
+    if (w >= 0 && w < nw) {
+      const size_t e = ((static_cast<size_t>(w) * nq + q) * n + ri * nv + i + off) * n + cj * nv + (P2 - 1 - j) + off;
+      vd = fd[e];
+      if (fm) vm = fm[e];
+    } else if (w < 0 && -w < nw) {
+      const size_t e = ((static_cast<size_t>(-w) * nq + q) * n + ri * nv + (P2 - 1 - i) + off) * n + cj * nv + j + off;
+      vd = cconj(fd[e]);
+      if (fm) vm = cconj(fm[e]);
+    }
+    if (!fm) {  // one channel: the transformed vertex itself
+      gs[idx] = make_double2(-vd.x, -vd.y);
+      continue;
+    }
+    gs[idx] = make_double2(-(0.5 * vd.x - 1.5 * vm.x), -(0.5 * vd.y - 1.5 * vm.y));
+    if (gt) gt[idx] = make_double2(-(0.5 * vd.x + 0.5 * vm.x), -(0.5 * vd.y + 0.5 * vm.y));
+  }
+}
+
+// ---------------------------------------------------------------------------
+enum VertexOut { VTX_OUT_INVERSE = 0, VTX_OUT_FULL = 1 };
+
+static int vertex_solve(LadderArgs a, int what, const VertexArgs& o, hipStream_t stream) {
+  const int n = a.n, nr = n, ncol = 2 * n;
+  const int ld = (ncol + 3) & ~3;
+  DGAMI_ARG_CHECK(n <= 1536, "vertex: compound size above 1536 is not supported");
+  a.rhs_mode = what == VTX_OUT_FULL ? LADDER_RHS_CHI0INV : LADDER_RHS_IDENTITY;
+  const size_t items = static_cast<size_t>(a.nw) * a.nq;
+  const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);
+  // up to 32 GiB of problems (factors and n right-hand sides) per pass
+  size_t batch = (size_t(32) << 30) / per;
+  if (batch < 1) batch = 1;
+  if (batch > items) batch = items;
+  if (batch > 65535) batch = 65535;
+  BlockedState<double2> st;
+  if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
+  const int nblk = static_cast<int>(ceil_div(n, TRI_B));
+  double2* tri = static_cast<double2*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(double2)));
+  if (!tri) return -2;
+  const int ntile = static_cast<int>(ceil_div(nr, 64));
+  for (size_t item0 = 0; item0 < items; item0 += batch) {
+    const int nb = static_cast<int>(items - item0 < batch ? items - item0 : batch);
+    int rc = blocked_factor<double2>(a, st, static_cast<int>(item0), nb, stream);
+    if (rc) return rc;
+    TriFactors<double2> f{st.W, tri, st.dinv, n, ld, nblk};
+    rc = tri_diag_inverse(f, nb, false, stream);
+    if (rc) return rc;
+    for (int c0 = (n - 1) / VTX_STEP * VTX_STEP; c0 >= 0; c0 -= VTX_STEP) {
+      const int cb = min(VTX_STEP, n - c0);
+      hipLaunchKernelGGL(vtx_backsolve_kernel, dim3(static_cast<unsigned>(xcd_grid(nb, ntile))), dim3(256), 0, stream,
+                         st, tri, nblk, c0, cb, nr, ntile, nb);
+      DGAMI_HIP_CHECK(hipGetLastError());
+      if (c0 > 0 && (rc = blocked_gemm_fp64(st, nb, 0, c0, n, n + nr, c0, cb, stream)) != 0) return rc;
+    }
+    if (what == VTX_OUT_INVERSE) {
+      hipLaunchKernelGGL(vtx_copy_kernel, dim3(static_cast<unsigned>(ceil_div(n, 4)), nb), dim3(256), 0, stream, st,
+                         o.out, static_cast<int>(item0));
+    } else {
+#define DGAMI_VTX_FULL(TV)                                                                                       \
+  hipLaunchKernelGGL(vtx_full_kernel<TV>, dim3(a.nv, nb), dim3(256), 0, stream, a, st, o, static_cast<int>(item0))
+      switch (a.T) {
+        case 1: DGAMI_VTX_FULL(1); break;
+        case 4: DGAMI_VTX_FULL(4); break;
+        case 9: DGAMI_VTX_FULL(9); break;
+        case 16: DGAMI_VTX_FULL(16); break;
+        default: DGAMI_ARG_CHECK(false, "vertex: nb must be 1..4");
+      }
+#undef DGAMI_VTX_FULL
+    }
+    DGAMI_HIP_CHECK(hipGetLastError());
+    rc = blocked_copy_info(st.info, a.info, static_cast<int>(item0), nb, stream);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+static LadderArgs vertex_ladder_args(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                                     const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r,
+                                     int32_t* info) {
+  LadderArgs a{};
+  a.chi0_inv = reinterpret_cast<const double2*>(chi0_inv);
+  a.gamma = reinterpret_cast<const double2*>(gamma);
+  a.u_r = reinterpret_cast<const double2*>(u_r);
+  a.v_r = reinterpret_cast<const double2*>(v_r);
+  a.out = nullptr;
+  a.info = info;
+  a.nchan = 1;
+  a.nq = nq;
+  a.nw = nw;
+  a.T = nb * nb;
+  a.nv = 2 * niv;
+  a.n = a.T * a.nv;
+  a.inv_beta2 = 1.0 / (beta * beta);
+  a.inv_beta = 1.0 / beta;
+  a.dense = 0;
+  return a;
+}
+
+}  // namespace dgami
+
+extern "C" int dgami_ladder_invert(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                                   const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r,
+                                   dgami_c128* out, int32_t* info, void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && out, "ladder_invert: null pointer");
+  DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "ladder_invert: invalid sizes");
+  const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
+  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(out), 0.0};
+  return vertex_solve(a, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                                 const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r,
+                                 const dgami_c128* vrg, const dgami_c128* chi_phys, dgami_c128* out, int32_t* info,
+                                 void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && out, "full_vertex: null pointer");
+  DGAMI_ARG_CHECK((vrg == nullptr) == (chi_phys == nullptr), "full_vertex: vrg and chi_phys go together");
+  DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "full_vertex: invalid sizes");
+  const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
+  VertexArgs o{reinterpret_cast<const double2*>(vrg), reinterpret_cast<const double2*>(chi_phys),
+               reinterpret_cast<double2*>(out), beta * beta};
+  return vertex_solve(a, VTX_OUT_FULL, o, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_pairing_vertex_pp(int nq, int nb, int nw, int niv, int np, const dgami_c128* f_dens,
+                                       const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip,
+                                       void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(f_dens && gamma_sing, "pairing_vertex_pp: null pointer");
+  DGAMI_ARG_CHECK(f_magn || !gamma_trip, "pairing_vertex_pp: the triplet vertex needs f_magn");
+  DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && np >= 1 && np <= niv,
+                  "pairing_vertex_pp: invalid sizes (need 1 <= np <= niv)");
+  const size_t total = static_cast<size_t>(nq) * nb * nb * nb * nb * 4 * np * np;
+  const size_t blocks = ceil_div(total, 256);
+  hipLaunchKernelGGL(vtx_pp_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(f_dens),
+                     reinterpret_cast<const double2*>(f_magn), reinterpret_cast<double2*>(gamma_sing),
+                     reinterpret_cast<double2*>(gamma_trip), nq, nb, nw, niv, np);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}

@@ -1,4 +1,5 @@
-"""Linearised Eliashberg equation with the pairing matvec on the GPU (eliashberg_solver.py:376-471).
+"""Linearised Eliashberg equation with the pairing matvec on the GPU (eliashberg_solver.py:376-471),
+and the full ladder vertex and pairing vertices it starts from (eliashberg_solver.py:44-147, 620-629).
 
 solve_eliashberg_lanczos keeps the reference's driver -- scipy's ARPACK eigsh
 on a LinearOperator, largest algebraic eigenvalues -- and replaces its matvec
@@ -13,7 +14,68 @@ import scipy.sparse.linalg as spla
 
 from dgamore_amd import config
 from dgamore_amd import kernels as K
-from dgamore_amd.four_point import SpinChannel
+from dgamore_amd.four_point import FourPoint, FrequencyNotation, SpinChannel
+from dgamore_amd.nonlocal_sde import _ladder_operands
+
+
+def _ph_compound(f_q_r):
+    """FourPoint [q, o1..o4, w(>=0), v, v'] -> device [nw][nq][n][n] compound (four_point.py:184-221)."""
+    mat = f_q_r.to_half_niw_range().compress_q_dimension().mat
+    nq, nb, nw, nv = mat.shape[0], mat.shape[1], mat.shape[5], mat.shape[6]
+    n = nb * nb * nv
+    d = K.to_dev(mat)
+    return d.permute(5, 0, 1, 2, 6, 4, 3, 7).reshape(nw, nq, n, n).contiguous(), nb
+
+
+def create_full_vertex_q_r(u_loc, v_nonloc, gamma_r, niv_pp, gchi0_q_inv, vrg_q_r, gchi_aux_q_r_sum,
+                           save_fq=None, construct_fq_cheap=None):
+    """Full ladder vertex of one channel (eliashberg_solver.py:78-147, and the per-q _v2 form :175-209):
+
+        F_r = beta^2 (chi0^-1 - chi0^-1 chi*_r chi0^-1) + u (vrg * vrg) - u chi_r u (vrg * vrg),  u = U_r + V_r^q
+
+    The reference reads chi0^-1, vrg and chi_r (its gchi_aux_q_r_sum file, the shell-corrected
+    physical susceptibility) from per-rank files; here they are passed in. With save_fq the
+    particle-hole vertex [q, o1..o4, w, v, v'] is returned, otherwise its particle-particle
+    transform at w = 0 [q, o1..o4, 2 niv_pp, 2 niv_pp] (transform_vertex_q_frequencies_w0).
+    construct_fq_cheap cuts every operand to niv_pp fermionic frequencies first.
+    """
+    save_fq = config.eliashberg.save_fq if save_fq is None else save_fq
+    cheap = config.eliashberg.construct_fq_cheap if construct_fq_cheap is None else construct_fq_cheap
+    if cheap:
+        gamma_r = gamma_r.cut_niv(niv_pp)
+        gchi0_q_inv = gchi0_q_inv.cut_niv(niv_pp)
+        vrg_q_r = vrg_q_r.cut_niv(niv_pp)
+    chi0_inv, gam, ur, vr, inv = _ladder_operands(gamma_r, gchi0_q_inv, u_loc, v_nonloc)
+    nb = inv.n_bands
+    vrg = K.ref4_to_qf(K.to_dev(vrg_q_r.to_half_niw_range().compress_q_dimension().mat))
+    chi = K.ref4w_to_qw(K.to_dev(gchi_aux_q_r_sum.to_half_niw_range().compress_q_dimension().mat))
+    f, _ = K.full_vertex(chi0_inv, gam, ur, vr, vrg, chi, config.sys.beta)
+    if save_fq:
+        return FourPoint(K.qcmp_to_ref8(f, nb).cpu().numpy(), gamma_r.channel, inv.nq, 1, 2, False,
+                         has_compressed_q_dimension=True)
+    pp, _ = K.pairing_vertex_pp(f, None, nb, niv_pp)
+    return FourPoint(pp.cpu().numpy(), gamma_r.channel, inv.nq, 0, 2, False, True, True, FrequencyNotation.PP)
+
+
+def transform_vertex_q_frequencies_w0(f_q_r, niv_pp):
+    """Particle-hole -> particle-particle notation at w = 0 (eliashberg_solver.py:44-58, 69-75):
+    F_pp[..., v, v'] = -F[..., w = v - v', v, -v' (flipped index)], negative w through F(-w) = conj F(w)
+    with both fermionic axes flipped."""
+    f, nb = _ph_compound(f_q_r)
+    pp, _ = K.pairing_vertex_pp(f, None, nb, niv_pp)
+    return FourPoint(pp.cpu().numpy(), f_q_r.channel, f_q_r.nq, 0, 2, False, True, True, FrequencyNotation.PP)
+
+
+def create_pairing_vertices_pp(f_dens, f_magn, niv_pp):
+    """Singlet and triplet pairing vertices from the particle-hole full vertices of the density and
+    magnetic channels: 0.5 F_dens,pp - 1.5 F_magn,pp and 0.5 F_dens,pp + 0.5 F_magn,pp
+    (the combination in solve, eliashberg_solver.py:620-629, fused with the w = 0 transform)."""
+    fd, nb = _ph_compound(f_dens)
+    fm, _ = _ph_compound(f_magn)
+    gs, gt = K.pairing_vertex_pp(fd, fm, nb, niv_pp)
+    nq = f_dens.nq
+    return (FourPoint(gs.cpu().numpy(), SpinChannel.SING, nq, 0, 2, False, True, True, FrequencyNotation.PP),
+            FourPoint(gt.cpu().numpy(), SpinChannel.TRIP, nq, 0, 2, False, True, True, FrequencyNotation.PP))
 
 
 class PairingOperator:

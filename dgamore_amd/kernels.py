@@ -83,6 +83,13 @@ def vertex_to_cmp(g):
     return g.permute(4, 0, 1, 5, 3, 2, 6).reshape(nw, n, n).contiguous()
 
 
+def qcmp_to_ref8(x, nb):
+    """[nw][nq][n][n] per-(q, w) compound matrices -> [q,o1,o2,o3,o4,w,v,v'] (four_point.py:251-293)."""
+    nw, nq, n, _ = x.shape
+    nv = n // (nb * nb)
+    return x.reshape(nw, nq, nb, nb, nv, nb, nb, nv).permute(1, 2, 3, 6, 5, 0, 4, 7).contiguous()
+
+
 # ---------------------------------------------------------------------------
 # kernels
 # ---------------------------------------------------------------------------
@@ -139,6 +146,71 @@ def ladder_solve_sum(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, precision=_lib.PLA
         if len(bad):
             raise np.linalg.LinAlgError(f"singular ladder matrix at (chan, w, q) = {bad[:8].tolist()}")
     return out, info
+
+
+def _raise_singular(info, what):
+    bad = torch.nonzero(info).cpu().numpy()
+    if len(bad):
+        raise np.linalg.LinAlgError(f"singular {what} at (w, q) = {bad[:8].tolist()}")
+
+
+def ladder_invert(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, check_info=True):
+    """Full inverse X = A^-1 of one channel's ladder matrices.
+
+    chi0_inv: QF [nw][nb^4][2niv][nq]; gamma_cmp: CMP [nw][n][n]; u_r_m: [nb^4]; v_r_m: [nq][nb^4] or None.
+    Returns (X [nw][nq][n][n], info int32 [nw][nq]).
+    """
+    nw, nb4, nv, nq = chi0_inv.shape
+    nb = int(round(nb4 ** 0.25))
+    n = nb * nb * nv
+    if tuple(gamma_cmp.shape) != (nw, n, n):
+        raise ValueError(f"gamma must be [{nw},{n},{n}], got {tuple(gamma_cmp.shape)}")
+    out = torch.empty((nw, nq, n, n), dtype=C128, device=chi0_inv.device)
+    info = torch.zeros((nw, nq), dtype=torch.int32, device=chi0_inv.device)
+    _lib.call("dgami_ladder_invert", nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp), _p(u_r_m),
+              _p(v_r_m), _p(out), _p(info, torch.int32), _stream())
+    if check_info:
+        _raise_singular(info, "ladder matrix")
+    return out, info
+
+
+def full_vertex(chi0_inv, gamma_cmp, u_r_m, v_r_m, vrg, chi_phys, beta, check_info=True):
+    """Full vertex F_r [nw][nq][n][n] of one channel (vrg QF, chi_phys QW; both None: first term only)."""
+    nw, nb4, nv, nq = chi0_inv.shape
+    nb = int(round(nb4 ** 0.25))
+    n = nb * nb * nv
+    if tuple(gamma_cmp.shape) != (nw, n, n):
+        raise ValueError(f"gamma must be [{nw},{n},{n}], got {tuple(gamma_cmp.shape)}")
+    if vrg is not None and tuple(vrg.shape) != tuple(chi0_inv.shape):
+        raise ValueError("vrg must have the shape of chi0_inv")
+    if chi_phys is not None and tuple(chi_phys.shape) != (nw, nb4, nq):
+        raise ValueError(f"chi_phys must be [{nw},{nb4},{nq}]")
+    out = torch.empty((nw, nq, n, n), dtype=C128, device=chi0_inv.device)
+    info = torch.zeros((nw, nq), dtype=torch.int32, device=chi0_inv.device)
+    _lib.call("dgami_full_vertex", nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp), _p(u_r_m),
+              _p(v_r_m), _p(vrg), _p(chi_phys), _p(out), _p(info, torch.int32), _stream())
+    if check_info:
+        _raise_singular(info, "ladder matrix")
+    return out, info
+
+
+def pairing_vertex_pp(f_dens, f_magn, nb, niv_pp, want_trip=True):
+    """Singlet (and triplet) pairing vertices [nq,nb,nb,nb,nb,2np,2np] from F_dens, F_magn [nw][nq][n][n].
+
+    f_magn None: (F_pp of f_dens, None), the transform of one channel.
+    """
+    nw, nq, n, _ = f_dens.shape
+    if f_magn is None:
+        want_trip = False
+    elif tuple(f_magn.shape) != tuple(f_dens.shape):
+        raise ValueError("f_dens and f_magn must have the same shape")
+    niv = n // (2 * nb * nb)
+    shape = (nq, nb, nb, nb, nb, 2 * niv_pp, 2 * niv_pp)
+    gs = torch.empty(shape, dtype=C128, device=f_dens.device)
+    gt = torch.empty(shape, dtype=C128, device=f_dens.device) if want_trip else None
+    _lib.call("dgami_pairing_vertex_pp", nq, nb, nw, niv, int(niv_pp), _p(f_dens), _p(f_magn), _p(gs), _p(gt),
+              _stream())
+    return gs, gt
 
 
 def vrg_chi_phys(chi0_inv, aux, full, core, u_r_m, v_r_m, nb, beta, want_vrg=True):

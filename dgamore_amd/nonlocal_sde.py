@@ -39,6 +39,33 @@ def _dev_m(x):
     return None if x is None else K.orb_to_m(K.to_dev(x))
 
 
+def _ladder_operands(gamma_r, gchi0_q_inv, u_loc, v_nonloc):
+    """Device operands of one channel's ladder: (chi0_inv QF, Gamma CMP, U_r M order, V_r QO or None, inv)."""
+    ch = _ch(gamma_r.channel)
+    inv = gchi0_q_inv.to_half_niw_range().compress_q_dimension()
+    gam = gamma_r.mat if gamma_r.mat.ndim == 7 else None
+    if gam is None:
+        raise ValueError("gamma_r must be a local vertex [o1,o2,o3,o4,w,v,v']")
+    if gam.shape[4] != inv.mat.shape[5] or gam.shape[5] != inv.mat.shape[6]:
+        raise ValueError(f"gamma_r frequency box {gam.shape[4:]} does not match chi0^-1 {inv.mat.shape[5:]}")
+    ur = u_as_channel(_mat(u_loc), ch)
+    vr = _v_as_channel(_mat(v_nonloc), ch)
+    return (K.ref4_to_qf(K.to_dev(inv.mat)), K.vertex_to_cmp(K.to_dev(gam)), _dev_m(ur),
+            None if vr is None else _dev_m(vr), inv)
+
+
+def create_auxiliary_chi_r_q(gamma_r, gchi0_q_inv, u_loc, v_nonloc):
+    """chi*_r(q) = ((chi0^-1 + Gamma_r/beta^2) - (V_r + U_r)/beta^2)^-1, the full inverse of every
+    (q, w) ladder matrix (nonlocal_sde.py:72-82 with FourPoint.invert, four_point.py:590-600).
+
+    Returns a FourPoint [q, o1, o2, o3, o4, w(>=0), v, v'] (two fermionic axes, half bosonic range).
+    """
+    chi0_inv, gam, ur, vr, inv = _ladder_operands(gamma_r, gchi0_q_inv, u_loc, v_nonloc)
+    x, _ = K.ladder_invert(chi0_inv, gam, ur, vr, config.sys.beta)
+    return FourPoint(K.qcmp_to_ref8(x, inv.n_bands).cpu().numpy(), gamma_r.channel, inv.nq, 1, 2, False,
+                     has_compressed_q_dimension=True)
+
+
 def create_auxiliary_chi_r_q_sum_v1(gamma_r, gchi0_q_inv, u_loc, v_nonloc):
     """sum_v' chi*_r = sum_v' ((chi0^-1 + Gamma_r/beta^2) - (V_r + U_r)/beta^2)^-1 (nonlocal_sde.py:85-96)."""
     import torch

@@ -118,6 +118,39 @@ int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double be
 int dgami_solve_sum_dense(int nmat, int nb, int niv, double beta, const dgami_c128* a_cmp, int precision,
                           dgami_c128* out, int32_t* info, void* stream);
 
+/* Full inverse of the ladder matrices of one channel (complex128 blocked LU carrying the
+ * identity as n right-hand sides, then block back substitution):
+ *   X(q,w) = A^-1,  A = chi0^-1 delta_vv' + (Gamma_r - U_r - V_r^q)/beta^2   (as in dgami_ladder_solve_sum)
+ * chi0_inv: QF; gamma: CMP [nw][n][n]; u_r: [nb^4] M order; v_r: QO or NULL;
+ * out: [nw][nq][n][n] compound (rows (o1,o2,v), cols (o4,o3,v')); info: [nw][nq] or NULL.
+ * Replaces create_auxiliary_chi_r_q with FourPoint.invert (nonlocal_sde.py:72-82, four_point.py:590-600). */
+int dgami_ladder_invert(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                        const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r, dgami_c128* out,
+                        int32_t* info, void* stream);
+
+/* Full vertex of one channel in particle-hole notation, products in compound indices:
+ *   F_r(q,w) = beta^2 (chi0^-1 - chi0^-1 X chi0^-1) + (u - u chi_r u) . (vrg * vrg),  u = U_r + V_r^q
+ * with X as in dgami_ladder_invert, (vrg * vrg)[(a,b,v),(f,e,v')] = sum_dc vrg_abcd(v) vrg_dcef(v'),
+ * vrg: QF and chi_r: QW (the outputs of dgami_vrg_chi_phys). vrg = chi_phys = NULL: first term only.
+ * out: [nw][nq][n][n] compound; info: [nw][nq] or NULL.
+ * Replaces create_full_vertex_q_r / create_full_vertex_q_r_v2 (eliashberg_solver.py:78-147, 175-209). */
+int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
+                      const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r, const dgami_c128* vrg,
+                      const dgami_c128* chi_phys, dgami_c128* out, int32_t* info, void* stream);
+
+/* Singlet and triplet pairing vertices at w_pp = 0 from the particle-hole full vertices of the
+ * density and magnetic channels (f_dens, f_magn: [nw][nq][n][n], half bosonic range, n = nb^2*2niv):
+ *   F_pp[q][o1][o2][o3][o4][i][j] = -F(w=i-j)[(o1,o2,i+off)][(o4,o3,2np-1-j+off)]        (w >= 0)
+ *                                 = -conj F(j-i)[(o1,o2,2np-1-i+off)][(o4,o3,j+off)]     (w < 0)
+ *   off = niv - np, zero where |w| >= nw;
+ *   gamma_sing = 0.5 F_pp,dens - 1.5 F_pp,magn,  gamma_trip = 0.5 F_pp,dens + 0.5 F_pp,magn (or NULL).
+ * f_magn = gamma_trip = NULL: gamma_sing receives F_pp of f_dens alone.
+ * Output layout [nq][nb][nb][nb][nb][2np][2np], the input of dgami_pairing_setup. Needs np <= niv.
+ * Replaces transform_vertex_q_frequencies_w0 (eliashberg_solver.py:44-58, 69-75) and the channel
+ * combination of solve (eliashberg_solver.py:620-629). */
+int dgami_pairing_vertex_pp(int nq, int nb, int nw, int niv, int np, const dgami_c128* f_dens,
+                            const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip, void* stream);
+
 /* Batched inverse of nmat nb^2 x nb^2 blocks stored element-major: in/out [nb^4 (M order)][nmat].
  * info: [nmat] (1 where a pivot was exactly zero) or NULL.
  * Replaces FourPoint.invert for objects without fermionic axes (four_point.py:207-209, 615-618)

@@ -41,6 +41,13 @@ PARITY = {
     "sigma_direct": "brute-force loop over (k,q,w,a,d) on a tiny grid (test_oracle_sigma.py)",
     "find_lambda": "reference known answers lambda=-37.450340, 4.328781, 4.281153 + fixtures (test_oracle_golden.py)",
     "pairing_matvec": "equals the explicit dense pairing operator built element by element (test_oracle_eliashberg.py)",
+    "auxiliary_chi_q": "A X == 1, row sums equal ladder_sum_solve (test_oracle_vertex.py)",
+    "full_vertex_q": "BSE form beta^2 (K - K X K), Woodbury form beta^2 D X K, K = 0 gives 0, second term against "
+                     "the reference's einsum contractions (test_oracle_vertex.py)",
+    "vertex_pp_w0": "Matsubara-frequency form F_pp(v,v') = -F(w=v-v'; v, -v'), both bosonic signs, on an "
+                    "analytic vertex (test_oracle_vertex.py)",
+    "pairing_vertices": "reference-held gamma_irrq_{sing,trip}_pp.npy = combination of f_irrq_{dens,magn}_pp.npy "
+                        "(test_oracle_vertex.py)",
 }
 
 
@@ -375,6 +382,86 @@ def dc_kernel(f_magn_loc, chi0, u, beta, niv_core):
     out = np.einsum("qabcdwv,dcefwvp->qabefwp", k0, f_dc)
     niv = out.shape[-1] // 2
     return out[..., niv - niv_core: niv + niv_core]
+
+
+# -----------------------------------------------------------------------------
+# Full vertex and pairing vertices  (dgamore/nonlocal_sde.py:72-82,
+# dgamore/eliashberg_solver.py:44-147, 175-209, 620-629)
+# -----------------------------------------------------------------------------
+def orb_matrix(x):
+    """[..., o1, o2, o3, o4] -> [..., nb^2, nb^2] with rows (o1,o2), cols (o4,o3) (four_point.py:184-221)."""
+    nb = x.shape[-1]
+    return x.swapaxes(-1, -2).reshape(*x.shape[:-4], nb * nb, nb * nb)
+
+
+def chi0_inv_compound(chi0_inv_q):
+    """chi0^-1 of one q ([o1..o4, w, v]) as the block-diagonal compound matrix [w, n, n]
+    (local_n_point.py:163-175 extend_vn_to_diagonal)."""
+    nv = chi0_inv_q.shape[-1]
+    return compound_local_vertex(chi0_inv_q[..., None] * np.eye(nv))
+
+
+def auxiliary_chi_q(chi0_inv, gamma, u_r, v_r, beta):
+    """chi*_r(q,w) = ((chi0^-1 + Gamma_r/beta^2) - (U_r + V_r^q)/beta^2)^-1 as full inverses
+    (nonlocal_sde.py:72-82, four_point.py:590-600). Returns compound [q, w, n, n]."""
+    nq = chi0_inv.shape[0]
+    return np.stack([np.linalg.inv(ladder_matrix(chi0_inv[iq], gamma, u_r, v_r[iq], beta)) for iq in range(nq)])
+
+
+def full_vertex_q(chi0_inv, gamma, u_r, v_r, vrg_r, chi_phys, beta):
+    """Full ladder vertex F_r(q) (eliashberg_solver.py:78-147 / 175-209), compound [q, w, n, n]:
+
+        beta^2 (chi0^-1 - chi0^-1 chi* chi0^-1) + u @ (vrg * vrg) - u @ chi_r @ u @ (vrg * vrg),
+
+    u = U_r + V_r^q, (vrg * vrg)[(a,b,v),(f,e,v')] = sum_dc vrg_abcd(v) vrg_dcef(v') (four_point.py:442-473).
+    vrg_r: [q,o1..o4,w,v] or None (then only the first term); chi_phys: [q,o1..o4,w].
+    """
+    x = auxiliary_chi_q(chi0_inv, gamma, u_r, v_r, beta)
+    nq, nw, n = x.shape[0], x.shape[1], x.shape[2]
+    out = np.empty_like(x)
+    for iq in range(nq):
+        d = chi0_inv_compound(chi0_inv[iq])
+        f = beta**2 * (d - d @ x[iq] @ d)
+        if vrg_r is not None:
+            u = orb_matrix(u_r + v_r[iq])
+            s = orb_matrix(np.moveaxis(chi_phys[iq], -1, 0))           # [w, T, T]
+            y = u - u @ s @ u
+            vm = orb_matrix(np.moveaxis(vrg_r[iq], (-2, -1), (0, 1)))   # [w, v, T, T]
+            p = y[:, None] @ vm
+            f = f + np.einsum("wvab,wpbc->wavcp", p, vm).reshape(nw, n, n)
+        out[iq] = f
+    return out
+
+
+def compound_to_ref8(x, nb):
+    """[q, w, n, n] compound -> [q, o1, o2, o3, o4, w, v, v'] (four_point.py:251-293)."""
+    nq, nw, n = x.shape[0], x.shape[1], x.shape[2]
+    nv = n // (nb * nb)
+    return x.reshape(nq, nw, nb, nb, nv, nb, nb, nv).transpose(0, 2, 3, 6, 5, 1, 4, 7)
+
+
+def vertex_pp_w0(f_ref, niv_pp):
+    """ph -> pp notation at w_pp = 0 (eliashberg_solver.py:44-58): cut to niv_pp, full bosonic
+    range, flip v', then F_pp[..., v, v'] = -F[..., w = v - v', v, v'] on the integer grids.
+
+    f_ref: [q, o1..o4, w(>=0), v, v'] (niv >= niv_pp). Returns [q, o1..o4, 2niv_pp, 2niv_pp].
+    """
+    niv = f_ref.shape[-1] // 2
+    niw = f_ref.shape[-3] - 1
+    cut = f_ref[..., niv - niv_pp: niv + niv_pp, niv - niv_pp: niv + niv_pp]
+    full = np.flip(to_full_niw(cut, 2), axis=-1)
+    v = vn_int(niv_pp)
+    omega = v[:, None] - v[None, :]
+    out = np.zeros(f_ref.shape[:5] + (2 * niv_pp, 2 * niv_pp), dtype=f_ref.dtype)
+    for idx, w in enumerate(wn_int(niw)):
+        mask = omega == w
+        out[..., mask] = -full[..., idx, :, :][..., mask]
+    return out
+
+
+def pairing_vertices(f_dens_pp, f_magn_pp):
+    """Singlet and triplet pairing vertices (eliashberg_solver.py:620-629)."""
+    return 0.5 * f_dens_pp - 1.5 * f_magn_pp, 0.5 * f_dens_pp + 0.5 * f_magn_pp
 
 
 # -----------------------------------------------------------------------------

@@ -36,3 +36,29 @@ def e2e_inputs():
     u[0, 0, 0, 0] = u[1, 1, 1, 1] = 8.0  # test_local_sde_end_to_end.py:106-113
     return {"ek": ek, "sigma": sigma, "mu": mu, "beta": beta, "giwk": giwk, "u": u,
             "nk": (4, 4, 1), "niw": 20, "niv_core": 20, "niv_shell": 10}
+
+
+def ladder_case(nb, niv, niw, nq, seed, gamma=None, beta=10.0, with_v=True):
+    """Random but well-conditioned ladder inputs in the reference layouts:
+    (chi0_inv [q,o1..o4,w,v], gamma [o1..o4,w,v,v'], u [nb]*4, v [q,nb,nb,nb,nb], beta)."""
+    from oracle import dga_oracle as O
+
+    rng = np.random.default_rng(seed)
+    nw = niw + 1
+    nv = 2 * niv
+    vv = O.vn(niv, beta)
+    # chi0 blocks ~ diagonal-dominant with frequency decay
+    base = np.einsum("ad,bc->abcd", np.eye(nb), np.eye(nb))
+    chi0 = np.empty((nq, nb, nb, nb, nb, nw, nv), dtype=complex)
+    for iq in range(nq):
+        for w in range(nw):
+            pert = 0.2 * (rng.standard_normal((nb,) * 4 + (nv,)) + 1j * rng.standard_normal((nb,) * 4 + (nv,)))
+            chi0[iq, ..., w, :] = (base[..., None] + pert) * (beta / (1.0 + vv**2))
+    chi0_inv = O.block_invert_vn1(chi0)
+    if gamma is None:
+        gamma = rng.standard_normal((nb,) * 4 + (nw, nv, nv)) + 1j * rng.standard_normal((nb,) * 4 + (nw, nv, nv))
+        gamma *= 2.0
+    u = rng.standard_normal((nb,) * 4)
+    v = 0.3 * (rng.standard_normal((nq,) + (nb,) * 4) + 1j * rng.standard_normal((nq,) + (nb,) * 4)) if with_v \
+        else np.zeros((nq,) + (nb,) * 4)
+    return chi0_inv, gamma, u, v, beta

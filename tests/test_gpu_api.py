@@ -131,3 +131,41 @@ def test_eliashberg_lanczos_leading_eigenvalues(api):
     lambdas, gaps = es.solve_eliashberg_lanczos(gam, chi0pp, v0=v0)
     assert np.allclose(lambdas, [3.9108, 3.7144, 3.5533, 3.4945], atol=2e-4)
     assert gaps[0].shape == (4, 4, 1, 2, 2, 20)
+
+
+def test_auxiliary_chi_and_full_vertex_mirror(api):
+    """create_auxiliary_chi_r_q, create_full_vertex_q_r (ph and pp) and the pairing vertices on the
+    SrVO3 local vertex, against the oracle."""
+    print("[gpu-case] create_auxiliary_chi_r_q / create_full_vertex_q_r srvo3 Gamma", flush=True)
+    from dgamore_amd import eliashberg_solver as es
+
+    fp, sde, cfg = api["fp"], api["sde"], api["config"]
+    beta, nq, niv_pp = 12.5, 3, 2
+    cfg.sys.beta = beta
+    rng = np.random.default_rng(12)
+    u = np.einsum("ad,bc->abcd", np.eye(3), np.eye(3)) * 4.0
+    v = 0.2 * (rng.standard_normal((nq, 3, 3, 3, 3)) + 1j * rng.standard_normal((nq, 3, 3, 3, 3)))
+    got, ref = {}, {}
+    for ch, name in ((fp.SpinChannel.DENS, "dens"), (fp.SpinChannel.MAGN, "magn")):
+        gd = golden(f"srvo3_gamma_{name}_loc.npy")[:, :, :, :, :4, 6:14, 6:14]   # w = 0..3, niv = 4
+        chi0_inv = rng.standard_normal((nq, 3, 3, 3, 3, 4, 8)) + 1j * rng.standard_normal((nq, 3, 3, 3, 3, 4, 8))
+        chi0_inv += 20 * np.einsum("ad,bc->abcd", np.eye(3), np.eye(3))[None, ..., None, None]
+        vrg = rng.standard_normal(chi0_inv.shape) + 1j * rng.standard_normal(chi0_inv.shape)
+        chi = 0.1 * (rng.standard_normal(chi0_inv.shape[:-1]) + 1j * rng.standard_normal(chi0_inv.shape[:-1]))
+        gam = fp.FourPoint(gd, ch, (1, 1, 1), 1, 2, False)
+        inv = fp.FourPoint(chi0_inv, fp.SpinChannel.NONE, (3, 1, 1), 1, 1, False, has_compressed_q_dimension=True)
+        vrg_fp = fp.FourPoint(vrg, ch, (3, 1, 1), 1, 1, False, has_compressed_q_dimension=True)
+        chi_fp = fp.FourPoint(chi, ch, (3, 1, 1), 1, 0, False, has_compressed_q_dimension=True)
+        ur, vr = O.u_as_channel(u, name), O.v_as_channel(v, name)
+        x = sde.create_auxiliary_chi_r_q(gam, inv, u, v)
+        assert x.mat.shape == (nq, 3, 3, 3, 3, 4, 8, 8)
+        assert rel(x.mat, O.compound_to_ref8(O.auxiliary_chi_q(chi0_inv, gd, ur, vr, beta), 3)) < 1e-10
+        f_ph = es.create_full_vertex_q_r(u, v, gam, niv_pp, inv, vrg_fp, chi_fp, save_fq=True)
+        ref[name] = O.compound_to_ref8(O.full_vertex_q(chi0_inv, gd, ur, vr, vrg, chi, beta), 3)
+        assert rel(f_ph.mat, ref[name]) < 1e-10
+        f_pp = es.create_full_vertex_q_r(u, v, gam, niv_pp, inv, vrg_fp, chi_fp, save_fq=False)
+        assert rel(f_pp.mat, O.vertex_pp_w0(ref[name], niv_pp)) < 1e-10
+        got[name] = f_ph
+    sing, trip = es.create_pairing_vertices_pp(got["dens"], got["magn"], niv_pp)
+    ref_s, ref_t = O.pairing_vertices(O.vertex_pp_w0(ref["dens"], niv_pp), O.vertex_pp_w0(ref["magn"], niv_pp))
+    assert rel(sing.mat, ref_s) < 1e-10 and rel(trip.mat, ref_t) < 1e-10

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from oracle import dga_oracle as O
-from tests.conftest import golden
+from tests.conftest import golden, ladder_case as _ladder_case
 
 pytestmark = pytest.mark.gpu
 
@@ -92,29 +92,6 @@ def test_chi0_prepare(K):
 
 
 # ----------------------------------------------------------------------------- ladder
-def _ladder_case(nb, niv, niw, nq, seed, gamma=None, beta=10.0, with_v=True):
-    """Random but well-conditioned ladder inputs in the reference layouts."""
-    rng = np.random.default_rng(seed)
-    nw = niw + 1
-    nv = 2 * niv
-    vv = O.vn(niv, beta)
-    # chi0 blocks ~ diagonal-dominant with frequency decay
-    base = np.einsum("ad,bc->abcd", np.eye(nb), np.eye(nb))
-    chi0 = np.empty((nq, nb, nb, nb, nb, nw, nv), dtype=complex)
-    for iq in range(nq):
-        for w in range(nw):
-            pert = 0.2 * (rng.standard_normal((nb,) * 4 + (nv,)) + 1j * rng.standard_normal((nb,) * 4 + (nv,)))
-            chi0[iq, ..., w, :] = (base[..., None] + pert) * (beta / (1.0 + vv**2))
-    chi0_inv = O.block_invert_vn1(chi0)
-    if gamma is None:
-        gamma = rng.standard_normal((nb,) * 4 + (nw, nv, nv)) + 1j * rng.standard_normal((nb,) * 4 + (nw, nv, nv))
-        gamma *= 2.0
-    u = rng.standard_normal((nb,) * 4)
-    v = 0.3 * (rng.standard_normal((nq,) + (nb,) * 4) + 1j * rng.standard_normal((nq,) + (nb,) * 4)) if with_v \
-        else np.zeros((nq,) + (nb,) * 4)
-    return chi0_inv, gamma, u, v, beta
-
-
 def _run_ladder(K, chi0_inv, gammas, us, vs, beta, precision=0):
     nb = chi0_inv.shape[1]
     g = torch.stack([K.vertex_to_cmp(K.to_dev(x)) for x in gammas])
@@ -125,15 +102,15 @@ def _run_ladder(K, chi0_inv, gammas, us, vs, beta, precision=0):
 
 
 @pytest.mark.parametrize("nb,niv,niw,nq", [
-    (1, 20, 4, 7),    # n = 40 (register kernel)
-    (1, 60, 2, 3),    # n = 120 (register kernel, 1-band GPU-scale box)
+    (1, 20, 4, 7),    # n = 40 (register kernel by default)
+    (1, 60, 2, 3),    # n = 120 (blocked LU by default; 1-band GPU-scale box)
     (2, 8, 3, 5),     # n = 32, 4 right-hand sides
     (2, 15, 2, 4),    # n = 60
     (3, 4, 2, 3),     # n = 72, 9 right-hand sides
-    (2, 20, 2, 3),    # n = 160 (blocked LU path)
-    (3, 12, 1, 2),    # n = 216 (blocked)
+    (2, 20, 2, 3),    # n = 160 (beyond the register kernels: blocked LU for every family)
+    (3, 12, 1, 2),    # n = 216
 ])
-@pytest.mark.parametrize("family", ["auto", "reg", "blocked", "blocked_vector"])
+@pytest.mark.parametrize("family", ["auto", "col", "reg", "blocked", "blocked_vector"])
 def test_ladder_plain_fp64(K, nb, niv, niw, nq, family, monkeypatch):
     say(f"ladder fp64 nb={nb} niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv} kernel={family}")
     if family == "blocked_vector":
