@@ -285,7 +285,8 @@ def main():
             "rel_err_vs_fp64": {
                 "chi_aux_sum": rel_err(a["aux"], b["aux"]),
                 "chi_phys": max(rel_err(x, y) for x, y in zip(a["chi_phys"], b["chi_phys"])),
-                "sigma": rel_err(a["sigma"], b["sigma"]),
+                "sigma": rel_err(a["sigma_nl"], b["sigma_nl"]),
+                "sigma_with_hf_and_tails": rel_err(a["sigma"], b["sigma"]),
             },
         }
 

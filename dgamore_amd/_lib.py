@@ -46,6 +46,10 @@ SIGNATURES = {
     "dgami_kernel_r": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _i, _c, _c], _i),
     "dgami_dc_kernel": ([_i, _i, _i, _i, _i, _d, _c, _c, _c, _c, _c], _i),
     "dgami_sigma_from_kernel": ([_i, _i, _i, _i, _i, _i, _i, _d, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_hartree_fock": ([_i, _i, _i, _i, _c, _c, _c, _c, _c, _c, _c], _i),
+    "dgami_sigma_assemble": ([_i, _i, _i, _i, _c, _c, _c, _c, _c, _i, _c, _d, _c, _c], _i),
+    "dgami_fill_sums": ([_i, _i, _i, _d, _d, _c, _c, _i, _c, _c, _c, _c, _c], _i),
+    "dgami_kmean": ([_i, _i64, _c, _c, _c], _i),
     "dgami_pairing_setup": ([_i, _i, _i, _i, _i, _i, _c, _c, _i, _c, _c, _c], _i),
     "dgami_pairing_matvec": ([_i, _i, _i, _i, _i, _i, _d, _c, _c, _c, _i, _c, _c, _c], _i),
 }

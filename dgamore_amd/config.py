@@ -11,7 +11,8 @@ from types import SimpleNamespace
 sys = SimpleNamespace(beta=1.0, mu=0.0, n=0.0, n_bands=1, occ=None, occ_k=None)
 box = SimpleNamespace(niw_core=0, niv_core=0, niv_shell=0, niv_full=0, niv_dmft=0)
 lattice = SimpleNamespace(nk=(1, 1, 1), nq=(1, 1, 1), k_grid=None, q_grid=None)
-self_consistency = SimpleNamespace(mixing=1.0, max_iter=1, epsilon=1e-4)
+self_consistency = SimpleNamespace(mixing=1.0, max_iter=1, epsilon=1e-4, mixing_strategy="linear",
+                                   mixing_history_length=2)
 lambda_correction = SimpleNamespace(perform_lambda_correction=False, type="spch")
 eliashberg = SimpleNamespace(n_eig=4, epsilon=1e-12, symmetry="random", save_fq=False, construct_fq_cheap=False)
 memory = SimpleNamespace(save_memory_for_chiq_aux=False, save_memory_for_chi0q=False, save_memory_for_sde=False)

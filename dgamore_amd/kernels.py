@@ -258,6 +258,47 @@ def sigma_from_kernel(kernel_irr, irrk_inv, giwk, nk, niw, beta):
     return sigma
 
 
+def hartree_fock(u_ref, v_full, occ, occ_k, nk):
+    """(hartree [nb,nb], fock [N,nb,nb]); u_ref [nb]*4, v_full [N,nb,nb,nb,nb] or None (reference order),
+    occ [nb,nb], occ_k [N,nb,nb] (complex)."""
+    nx, ny, nz = nk
+    nb = u_ref.shape[-1]
+    N = nx * ny * nz
+    hart = torch.empty((nb, nb), dtype=C128, device=u_ref.device)
+    fock = torch.empty((N, nb, nb), dtype=C128, device=u_ref.device)
+    _lib.call("dgami_hartree_fock", nx, ny, nz, nb, _p(u_ref), _p(v_full), _p(occ), _p(occ_k), _p(hart), _p(fock),
+              _stream())
+    return hart, fock
+
+
+def sigma_assemble(sigma_nl, hartree, fock, delta_sigma, sigma_dmft, niv_out, sigma_old=None, alpha=1.0):
+    """New self-energy [N,nb,nb,2 niv_out]: core window sigma_nl + H + F + delta_sigma, DMFT outside, then
+    alpha new + (1 - alpha) old. sigma_nl [N,nb,nb,2 niv_core]; sigma_dmft [nb,nb,2 niv_dmft]."""
+    N, nb, _, nvc = sigma_nl.shape
+    out = torch.empty((N, nb, nb, 2 * niv_out), dtype=C128, device=sigma_nl.device)
+    _lib.call("dgami_sigma_assemble", N, nb, nvc // 2, int(niv_out), _p(sigma_nl), _p(hartree), _p(fock),
+              _p(delta_sigma), _p(sigma_dmft), sigma_dmft.shape[-1] // 2, _p(sigma_old), float(alpha), _p(out),
+              _stream())
+    return out
+
+
+def fill_sums(ek, sigma, mu, beta, smom0, hloc=None, want_mean=True):
+    """(occ_k [N,nb,nb] float64, occ_mean [nb,nb] or None): (1/beta) sum_v Re[G - G_model]."""
+    N, nb = ek.shape[0], ek.shape[-1]
+    occ_k = torch.empty((N, nb, nb), dtype=torch.float64, device=ek.device)
+    occ_mean = torch.empty((nb, nb), dtype=torch.float64, device=ek.device) if want_mean else None
+    _lib.call("dgami_fill_sums", N, nb, sigma.shape[-1] // 2, float(beta), float(mu), _p(ek), _p(sigma),
+              sigma.shape[0], _p(smom0), _p(hloc), _p(occ_k, torch.float64), _p(occ_mean, torch.float64), _stream())
+    return occ_k, occ_mean
+
+
+def kmean(x):
+    """Mean over the leading (momentum) axis of a complex tensor."""
+    out = torch.empty(x.shape[1:], dtype=C128, device=x.device)
+    _lib.call("dgami_kmean", x.shape[0], int(np.prod(x.shape[1:])), _p(x), _p(out), _stream())
+    return out
+
+
 def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel):
     """gamma_pp_irr [nq,nb,nb,nb,nb,np2,np2] -> (g_hat, g_hat_flip) [N][n][n]."""
     nx, ny, nz = nk

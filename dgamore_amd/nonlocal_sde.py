@@ -195,3 +195,112 @@ def calculate_sigma_from_kernel_fft_auto(mpi_distributor, kernel, giwk):
 def calculate_sigma_from_kernel_auto(mpi_distributor, kernel, giwk, my_full_q_list=None):
     """Direct q-sum form of the reference (nonlocal_sde.py:404-541); same values by the FFT route."""
     return calculate_sigma_from_kernel_fft_auto(mpi_distributor, kernel, giwk)
+
+
+def get_hartree_fock(u_loc, v_nonloc, q_list=None):
+    """Hartree [1,nb,nb,1] and Fock [nk,nb,nb,1] terms (nonlocal_sde.py:31-69):
+
+        Sigma_H = 2 (U + V^{q=0})_abcd n_dc,   Sigma_F(k) = -(1/N_q) sum_q (U + V^q)_abcd n_dc(k+q),
+
+    with n = config.sys.occ and n(k) = config.sys.occ_k on config.lattice.nk. v_nonloc covers the full
+    momentum grid ([nx,ny,nz,nb,nb,nb,nb] or flattened, q = 0 first) or is None. The sum over every q
+    of the grid is one FFT cross-correlation (q_list is accepted for signature compatibility).
+    """
+    nk = tuple(config.lattice.nk)
+    u = np.asarray(_mat(u_loc), dtype=complex)
+    nb = u.shape[0]
+    v = _mat(v_nonloc)
+    dv = None if v is None else K.to_dev(np.asarray(v, dtype=complex).reshape(-1, nb, nb, nb, nb))
+    occ_k = np.asarray(config.sys.occ_k, dtype=complex).reshape(-1, nb, nb)
+    hart, fock = K.hartree_fock(K.to_dev(u), dv, K.to_dev(np.asarray(config.sys.occ, dtype=complex)),
+                                K.to_dev(occ_k), nk)
+    return hart.cpu().numpy()[None, ..., None], fock.cpu().numpy()[..., None]
+
+
+def assemble_sigma(sigma_nl, hartree, fock, delta_sigma, sigma_dmft, niv_out, sigma_old=None, alpha=1.0):
+    """New self-energy of one iteration, [nk,nb,nb,2 niv_out] (nonlocal_sde.py:884-893):
+    sigma + hartree + fock + delta_sigma on the core window, concatenated with the DMFT
+    self-energy outside it (self_energy.py:153-168), then alpha new + (1 - alpha) old when sigma_old
+    is given (the linear branch of apply_mixing_strategy, nonlocal_sde.py:1145-1151)."""
+    snl = np.asarray(sigma_nl)
+    nb = snl.shape[-2]
+    nk = int(np.prod(snl.shape[:-3]))
+    dev = lambda x, shape: None if x is None else K.to_dev(np.asarray(x, dtype=complex).reshape(shape))  # noqa
+    nvc = snl.shape[-1]
+    out = K.sigma_assemble(K.to_dev(snl.reshape(nk, nb, nb, nvc)), dev(hartree, (nb, nb)), dev(fock, (nk, nb, nb)),
+                           dev(delta_sigma, (nb, nb, nvc)), K.to_dev(np.asarray(sigma_dmft).reshape(nb, nb, -1)),
+                           niv_out, dev(sigma_old, (nk, nb, nb, 2 * niv_out)), alpha)
+    return out.cpu().numpy()
+
+
+def apply_mixing_strategy(sigma_new, sigma_old, sigma_dmft, current_iter, history=None):
+    """Mixing of the self-energy (nonlocal_sde.py:979-1151). Arrays [nk,nb,nb,2niv].
+
+    linear: alpha sigma_new + (1 - alpha) sigma_old on the device. pulay / anderson (after
+    mixing_history_length iterations) follow the reference's truncated-SVD least squares on the
+    core window in FP64 on the host; `history` holds the previous iterations' self-energies
+    (the reference reads them from its output files), oldest first.
+    """
+    n_hist = config.self_consistency.mixing_history_length
+    alpha = config.self_consistency.mixing
+    strategy = config.self_consistency.mixing_strategy.lower()
+    new = np.asarray(sigma_new)
+    old = np.asarray(sigma_old)
+    if strategy in ("pulay", "anderson") and current_iter > n_hist and history is not None:
+        return _history_mixing(strategy, new, old, np.asarray(sigma_dmft), list(history)[-n_hist:], n_hist, alpha)
+    nb, nv = new.shape[-2], new.shape[-1]
+    out = K.sigma_assemble(K.to_dev(new.reshape(-1, nb, nb, nv)), None, None, None,
+                           K.to_dev(new.reshape(-1, nb, nb, nv)[0]), nv // 2,
+                           K.to_dev(old.reshape(-1, nb, nb, nv)), alpha)
+    return out.cpu().numpy().reshape(new.shape)
+
+
+def _history_mixing(strategy, new, old, sigma_dmft, hist, n_hist, alpha):
+    """Pulay / Anderson step on the core window (nonlocal_sde.py:990-1143), FP64 host least squares."""
+    niv = new.shape[-1] // 2
+    nc = config.box.niv_core
+    sl = slice(niv - nc, niv + nc)
+    dmft = np.broadcast_to(sigma_dmft.reshape((1,) + sigma_dmft.shape[-3:]), new.shape)
+    proposals = [np.asarray(dmft)] + hist
+    results = hist + [new]
+    proposals = [p[..., sl].reshape(-1) for p in proposals]
+    results = [r[..., sl].reshape(-1) for r in results]
+    ri = lambda z: np.concatenate([z.real, z.imag])  # noqa: E731
+    f_i = ri(results[-1] - proposals[-1])
+    norm_f = np.linalg.norm(f_i)
+    out = new.copy()
+    if strategy == "pulay":
+        r_m = np.column_stack([ri(proposals[-1 - i] - proposals[-2 - i]) for i in range(n_hist)])
+        f_m = np.column_stack([ri(results[-1 - i] - results[-2 - i]) for i in range(n_hist)]) - r_m
+        u, s, vh = np.linalg.svd(f_m, full_matrices=False)
+        mask = s > 1e-5 * (s[0] if s.size else 1.0)
+        if not np.any(mask):
+            return alpha * new + (1 - alpha) * old
+        coeffs = vh[mask].T @ ((u[:, mask].T @ f_i) / s[mask])
+        update = alpha * f_i - (r_m + alpha * f_m) @ coeffs
+        norm_u = np.linalg.norm(update)
+        if norm_f > 0 and norm_u > 10.0 * norm_f:
+            update *= 10.0 * norm_f / norm_u
+        n = update.size // 2
+        out[..., sl] = (proposals[-1] + update[:n] + 1j * update[n:]).reshape(out[..., sl].shape)
+        return out
+    dx = np.column_stack([ri(proposals[-1 - i] - proposals[-2 - i]) for i in range(n_hist)])
+    df = np.column_stack([ri((results[-1 - i] - proposals[-1 - i]) - (results[-2 - i] - proposals[-2 - i]))
+                          for i in range(n_hist)])
+    u, s, vh = np.linalg.svd(df, full_matrices=False)
+    cutoff = 1e-5 * (s[0] if s.size else 1.0)
+    mask = s > cutoff
+    if not np.any(mask):
+        return alpha * new + (1 - alpha) * old
+    coeffs = vh[mask].T @ ((s[mask] / (s[mask] ** 2 + cutoff**2)) * (u[:, mask].T @ f_i))
+    x_n = proposals[-1]
+    xa = ri(x_n) + f_i - (dx + df) @ coeffs
+    n = xa.size // 2
+    cand = (1 - alpha) * x_n + alpha * (xa[:n] + 1j * xa[n:])
+    upd = cand - x_n
+    norm_u = np.linalg.norm(upd)
+    if norm_f > 0 and norm_u > 3.0 * norm_f:
+        cand = x_n + upd * (3.0 * norm_f / norm_u)
+    out[..., sl] = cand.reshape(out[..., sl].shape)
+    return out
+

@@ -12,6 +12,13 @@ every intermediate kept in HBM in the library's q-fastest layouts:
     vrg, chi_r      dgami_vrg_chi_phys      (nonlocal_sde.py:159-182)
     kernel += K_d + 3 K_m   dgami_kernel_r  (nonlocal_sde.py:201-215, 838-857)
     Sigma(k)        dgami_sigma_from_kernel (nonlocal_sde.py:544-594, 874-879)
+    Hartree, Fock   dgami_hartree_fock      (nonlocal_sde.py:31-69)
+    new Sigma       dgami_sigma_assemble    (+ H + F + delta_sigma, DMFT tails, linear mixing;
+                                             nonlocal_sde.py:884-893, 1145-1151)
+
+iterate() adds the chemical-potential update and the new occupations of the
+self-consistency loop (nonlocal_sde.py:895-914): the frequency sums run in
+dgami_fill_sums, the secant iteration on mu stays on the host.
 
 The MPI distribution of the reference (irreducible q over ranks) becomes the
 (channel, w, q) work list of a single ladder launch.
@@ -36,8 +43,9 @@ class SDEPipeline:
 
     CHANNELS = (("dens", 1.0), ("magn", 3.0))
 
-    def __init__(self, model, precision=_lib.PLAIN_FP64, symmetries="two_dimensional_square"):
+    def __init__(self, model, precision=_lib.PLAIN_FP64, symmetries="two_dimensional_square", mixing=1.0):
         self.m = model
+        self.mixing = mixing
         self.precision = precision
         self.grid = KGrid(model.nk, symmetries)
         self.nq = self.grid.nk_irr
@@ -53,6 +61,8 @@ class SDEPipeline:
         self.f_magn = K.to_dev(model.f_magn)
         self.sigma_dmft = K.to_dev(model.sigma_loc[None])  # local Sigma as [1][nb][nb][2 niv_s]
         self.niv_s = model.sigma_loc.shape[-1] // 2
+        self.delta_sigma = None  # sigma_dmft - sigma_local on the core window (zero for the synthetic models)
+        self.occ = self.occ_k = None
         self.last = {}
 
     # -- pieces -------------------------------------------------------------------------------
@@ -65,12 +75,27 @@ class SDEPipeline:
                   sigma.shape[-1] // 2, K._p(g), K._stream())
         return g
 
-    def step(self, sigma=None, mu=None, keep=False):
-        """One non-local SDE iteration; returns Sigma(k, v) [nx,ny,nz,nb,nb,2 niv_core] on the device."""
+    def update_occupation(self, sigma, mu):
+        """n, occ and occ_k of G(sigma, mu) with the k-dependent model (greens_function.py:212-240)."""
+        from dgamore_amd import greens_function as gf
+        from dgamore_amd.self_energy import fit_smom
+
+        smom0, _ = fit_smom(sigma, self.m.beta)
+        n, occ, occ_k = gf.get_fill_nonlocal(sigma, self.m.ek, mu, self.m.beta, smom0)
+        self.occ = K.to_dev(occ)
+        self.occ_k = K.to_dev(occ_k.reshape(self.N, self.m.nb, self.m.nb))
+        return n
+
+    def step(self, sigma=None, mu=None, keep=False, sigma_old=None):
+        """One non-local SDE iteration from Sigma to the new, mixed Sigma [N,nb,nb,2 niv_s] on the device
+        (the DMFT self-energy outside the core window). sigma, sigma_old: [1 or N][nb][nb][2 niv_s];
+        sigma_old defaults to sigma."""
         m = self.m
         nb, beta = m.nb, m.beta
         sigma = self.sigma_dmft if sigma is None else sigma
         mu = m.mu if mu is None else mu
+        if self.occ is None:
+            self.update_occupation(sigma, mu)
         g = self.g_full(sigma, mu, m.niv_g).reshape(m.nk + (nb, nb, 2 * m.niv_g))
         chi0 = K.bubble_irrq(g, m.nk, m.niw, m.niv_full, beta, self.irrk_ind)
         inv, full, core = K.chi0_prepare(chi0, nb, m.niv, beta)
@@ -83,10 +108,37 @@ class SDEPipeline:
             vrgs.append(vrg)
             chis.append(chi)
         sig = K.sigma_from_kernel(kern, self.irrk_inv, g, m.nk, m.niw, beta)
+        hart, fock = K.hartree_fock(self.u_ref, None, self.occ, self.occ_k, m.nk)
+        old = sigma if sigma_old is None else sigma_old
+        if self.mixing != 1.0 and old.shape[0] != self.N:
+            old = old.expand(self.N, -1, -1, -1).contiguous()
+        new = K.sigma_assemble(sig.reshape(self.N, nb, nb, -1), hart, fock, self.delta_sigma, self.sigma_dmft[0],
+                               self.niv_s, old if self.mixing != 1.0 else None, self.mixing)
         if keep:
             self.last = dict(g=g, chi0=chi0, chi0_inv=inv, full=full, core=core, aux=aux, info=info, vrg=vrgs,
-                             chi_phys=chis, kernel=kern, sigma=sig)
-        return sig
+                             chi_phys=chis, kernel=kern, sigma_nl=sig, hartree=hart, fock=fock, sigma=new)
+        return new
+
+    def iterate(self, n_iter, sigma=None, mu=None, n_target=None):
+        """Self-consistency loop: SDE step, mu update to the target filling, new occupations
+        (nonlocal_sde.py:777-914). Returns (sigma [N,nb,nb,2 niv_s], mu history)."""
+        from dgamore_amd import greens_function as gf
+        from dgamore_amd.self_energy import fit_smom
+
+        sigma = self.sigma_dmft if sigma is None else sigma
+        mu = self.m.mu if mu is None else mu
+        if n_target is None:
+            n_target = self.update_occupation(sigma, mu)
+        history = [mu]
+        for _ in range(n_iter):
+            sigma = self.step(sigma, mu)
+            smom0, _ = fit_smom(sigma, self.m.beta)
+            new_mu = gf.update_mu(mu, n_target, self.m.ek, sigma, self.m.beta, smom0)
+            if not np.isnan(new_mu):
+                mu = self.mixing * new_mu + (1 - self.mixing) * mu
+            history.append(mu)
+            self.update_occupation(sigma, mu)
+        return sigma, history
 
     # -- accounting ---------------------------------------------------------------------------
     def solves_per_step(self):

@@ -206,6 +206,45 @@ int dgami_sigma_from_kernel(int nx, int ny, int nz, int nb, int niw, int niv_cor
                             const dgami_c128* kernel_irr, const int32_t* irrk_inv, int nq_irr, const dgami_c128* g_k,
                             dgami_c128* sigma, void* stream);
 
+/* Hartree and Fock terms of the self-energy (complex128):
+ *   H_ab    = 2 sum_cd (U + V^{q=0})_abcd n_dc
+ *   F_ab(k) = -(1/N) sum_q sum_cd (U + V^q)_abcd n_dc(k + q)   (as an FFT cross-correlation)
+ * u: [nb^4] and v: [nk][nb^4] (full momentum grid, q = 0 first, or NULL) in the reference orbital
+ * order abcd; occ: [nb][nb] k-mean occupation; occ_k: [nk][nb][nb]; hartree: [nb][nb]; fock: [nk][nb][nb].
+ * Replaces get_hartree_fock (nonlocal_sde.py:31-69, 782-783). */
+int dgami_hartree_fock(int nx, int ny, int nz, int nb, const dgami_c128* u, const dgami_c128* v,
+                       const dgami_c128* occ, const dgami_c128* occ_k, dgami_c128* hartree, dgami_c128* fock,
+                       void* stream);
+
+/* New self-energy of one iteration and its linear mixing:
+ *   val[k][a][b][v] = sigma_nl + H_ab + F_ab(k) + delta_sigma_ab(v)   for |v| < niv_core (centred)
+ *                   = sigma_dmft_ab(v)                                otherwise (2*niv_out frequencies)
+ *   out = alpha val + (1 - alpha) sigma_old   (sigma_old NULL: out = val)
+ * sigma_nl: [nk][nb][nb][2*niv_core]; delta_sigma: [nb][nb][2*niv_core] or NULL; hartree/fock or NULL;
+ * sigma_dmft: [nb][nb][2*niv_dmft] (centred cut to niv_out); sigma_old, out: [nk][nb][nb][2*niv_out].
+ * Replaces sigma + hartree + fock + delta_sigma, concatenate_self_energies and linear mixing
+ * (nonlocal_sde.py:884-893, 1145-1151; self_energy.py:153-168). */
+int dgami_sigma_assemble(int nk, int nb, int niv_core, int niv_out, const dgami_c128* sigma_nl,
+                         const dgami_c128* hartree, const dgami_c128* fock, const dgami_c128* delta_sigma,
+                         const dgami_c128* sigma_dmft, int niv_dmft, const dgami_c128* sigma_old, double alpha,
+                         dgami_c128* out, void* stream);
+
+/* Matsubara part of the occupations:
+ *   occ_k[k][a][b] = (1/beta) sum_{v in [-niv, niv)} Re[G(k,v) - G_model(k,v)]_ab
+ *   G = (i v + mu - e_k - Sigma(k,v))^-1,  G_model = (i v + mu - h - smom0)^-1
+ * h = e_k (hloc NULL, get_fill_nonlocal) or hloc (get_total_fill); occ_mean (or NULL) = k-mean of occ_k.
+ * ek: [nk][nb][nb]; sigma: [sigma_nk][nb][nb][2*niv], sigma_nk = 1 or nk; smom0, hloc: [nb][nb];
+ * occ_k: [nk][nb][nb] doubles; occ_mean: [nb][nb] doubles. The density-matrix term of the model
+ * (a Fermi function of an nb x nb matrix per k) is added by the caller.
+ * Replaces the frequency sums of get_total_fill and GreensFunction.get_fill_nonlocal
+ * (greens_function.py:20-51, 212-240). */
+int dgami_fill_sums(int nk, int nb, int niv, double beta, double mu, const dgami_c128* ek, const dgami_c128* sigma,
+                    int sigma_nk, const dgami_c128* smom0, const dgami_c128* hloc, double* occ_k, double* occ_mean,
+                    void* stream);
+
+/* out[e] = (1/nk) sum_k x[k][e], e < m (the k-mean of SelfEnergy.fit_smom, self_energy.py:60). */
+int dgami_kmean(int nk, int64_t m, const dgami_c128* x, dgami_c128* out, void* stream);
+
 /* Pairing-vertex preparation for the linearised Eliashberg operator:
  *   g_hat = s fft_q Gamma_pp (q mapped irr -> FBZ), g_hat_flip = flip_k flip_v' perm(abcd->adcb) g_hat,
  * stored for the matvec as [nk][(a,b,v)][(c,d,v')] (row (a,b,v) = out, col (c,d,v') = in), n = nb^2*2np.

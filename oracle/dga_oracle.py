@@ -41,6 +41,15 @@ PARITY = {
     "sigma_direct": "brute-force loop over (k,q,w,a,d) on a tiny grid (test_oracle_sigma.py)",
     "find_lambda": "reference known answers lambda=-37.450340, 4.328781, 4.281153 + fixtures (test_oracle_golden.py)",
     "pairing_matvec": "equals the explicit dense pairing operator built element by element (test_oracle_eliashberg.py)",
+    "hartree_fock": "closed forms: local U gives k-independent Fock -U n and Hartree 2 U n; cos(q) V on a "
+                    "cos(k) occupation gives -(V/2) b cos(k) (test_oracle_selfcons.py)",
+    "fit_smom": "closed form on Sigma = m0 + m1/(i v) (test_oracle_selfcons.py)",
+    "total_fill": "reference-held mu_history.npy: update_mu on sigma_dga_iteration_1.npy reproduces mu_1 "
+                  "(test_oracle_selfcons.py); Fermi-Dirac limit at Sigma = 0",
+    "fill_nonlocal": "Sigma = 0 gives the Fermi function of e_k exactly; target filling of the golden run",
+    "update_mu": "reference-held mu_history.npy (test_oracle_selfcons.py)",
+    "sigma_assemble": "reference-held sigma_dga_iteration_1.npy equals sigma_dmft outside the core window "
+                      "(test_oracle_selfcons.py); mixing closed form",
     "auxiliary_chi_q": "A X == 1, row sums equal ladder_sum_solve (test_oracle_vertex.py)",
     "full_vertex_q": "BSE form beta^2 (K - K X K), Woodbury form beta^2 D X K, K = 0 gives 0, second term against "
                      "the reference's einsum contractions (test_oracle_vertex.py)",
@@ -521,6 +530,115 @@ def sigma_direct(kernel_irr, giwk, irrk_inv, beta, niv_core, niw):
             mat += np.einsum("aijdv,xyzadv->xyzijv", k_full[iq, ..., iw, niv_core:], g_qk)
     mat *= -0.5 / beta / ntot
     return np.concatenate([np.conj(np.flip(mat, axis=-1)), mat], axis=-1)
+
+
+# -----------------------------------------------------------------------------
+# Hartree-Fock, self-energy assembly and mixing, fillings and the chemical potential
+# (dgamore/nonlocal_sde.py:31-69, 884-893, 1145-1151; dgamore/self_energy.py:51-75, 153-168;
+#  dgamore/greens_function.py:20-80, 212-240)
+# -----------------------------------------------------------------------------
+def hartree_fock(u, v_full, occ, occ_k):
+    """Hartree [nb,nb] and Fock [nx,ny,nz,nb,nb] terms (nonlocal_sde.py:31-69).
+
+    u: [nb]*4; v_full: [nx,ny,nz,nb,nb,nb,nb] or None; occ: [nb,nb]; occ_k: [nx,ny,nz,nb,nb].
+    Fock by the reference's loop: for every q, n(k+q) is a roll of occ_k by -q.
+    """
+    nk = occ_k.shape[:3]
+    nb = u.shape[0]
+    v = np.zeros(nk + (nb,) * 4, dtype=complex) if v_full is None else v_full
+    hartree = 2 * np.einsum("abcd,dc->ab", u + v[0, 0, 0], occ)
+    fock = np.zeros(nk + (nb, nb), dtype=complex)
+    for q in np.ndindex(*nk):
+        rolled = np.roll(occ_k, [-i for i in q], axis=(0, 1, 2))
+        fock += np.einsum("abcd,xyzdc->xyzab", u + v[q], rolled)
+    return hartree, -fock / np.prod(nk)
+
+
+def fit_smom(sigma, beta):
+    """First two local moments of Sigma from the top 20% (at least 4) positive frequencies
+    (self_energy.py:51-75). sigma: [..k axes.., nb, nb, 2niv] with 3 momentum axes or none."""
+    s = sigma if sigma.ndim == 3 else sigma.reshape((-1,) + sigma.shape[-3:]).mean(axis=0)
+    niv = s.shape[-1] // 2
+    half = s[..., niv:]
+    iv = vn(niv, beta)[niv:]
+    n_fit = max(int(0.2 * niv), 4)
+    fit = half[..., niv - n_fit:]
+    return fit.real.mean(axis=-1), (fit.imag * iv[niv - n_fit:][None, None, :]).mean(axis=-1)
+
+
+def _fermi_matrix(m):
+    """Fermi function (1 + exp(M))^-1 of a batch of matrices via their eigenvalues (greens_function.py:38-45)."""
+    ev, evec = np.linalg.eig(m)
+    rho = np.empty_like(ev)
+    pos = ev > 0
+    rho[pos] = np.exp(-ev[pos]) / (1 + np.exp(-ev[pos]))
+    rho[~pos] = 1 / (1 + np.exp(ev[~pos]))
+    return evec @ (rho[..., :, None] * np.eye(m.shape[-1])) @ np.linalg.inv(evec)
+
+
+def total_fill(mu, ek, sigma_k, beta, smom0):
+    """2 Tr occ with the local model Green's function (greens_function.py:20-51).
+
+    ek: [nx,ny,nz,nb,nb]; sigma_k: [nk,nb,nb,2niv]; smom0: [nb,nb].
+    """
+    nb = sigma_k.shape[-2]
+    niv = sigma_k.shape[-1] // 2
+    eye = np.eye(nb)
+    iv = 1j * vn(niv, beta)
+    hloc = ek.mean(axis=(0, 1, 2))
+    g_model = np.linalg.inv(iv[:, None, None] * eye + mu * eye - hloc - smom0)
+    e = ek.reshape(-1, nb, nb)
+    mat = iv[None, :, None, None] * eye + mu * eye - e[:, None] - sigma_k.transpose(0, 3, 1, 2)
+    g_loc = np.linalg.inv(mat).mean(axis=0)
+    rho = _fermi_matrix(beta * (hloc.real + smom0 - mu * eye))
+    occ = rho + (g_loc.real - g_model.real).sum(axis=0) / beta
+    return 2.0 * np.trace(occ).real
+
+
+def update_mu(mu0, target, ek, sigma_k, beta, smom0):
+    """Newton (secant) root of total_fill - target with tol 1e-6 (greens_function.py:54-80)."""
+    import scipy.optimize as opt
+
+    mu = opt.newton(lambda m: total_fill(m, ek, sigma_k, beta, smom0) - target, mu0, tol=1e-6)
+    return float(np.real(mu))
+
+
+def fill_nonlocal(sigma_k, ek, mu, beta, smom0):
+    """(n, occ_mean [nb,nb], occ_k [nx,ny,nz,nb,nb]) with the k-dependent model (greens_function.py:212-240).
+
+    sigma_k: [nk or 1, nb, nb, 2niv]; ek: [nx,ny,nz,nb,nb]; smom0: [nb,nb].
+    """
+    nk = ek.shape[:3]
+    nb = ek.shape[-1]
+    niv = sigma_k.shape[-1] // 2
+    eye = np.eye(nb)
+    iv = 1j * vn(niv, beta)
+    e = ek.reshape(-1, nb, nb)
+    base = iv[None, :, None, None] * eye + mu * eye - e[:, None]
+    g = np.linalg.inv(base - sigma_k.transpose(0, 3, 1, 2))
+    g_model = np.linalg.inv(base - smom0)
+    rho = _fermi_matrix(beta * (e.real + smom0 - mu * eye))
+    occ_k = rho + (g.real - g_model.real).sum(axis=1) / beta
+    occ_k.real[np.abs(occ_k) < 1e-12] = 0.0
+    occ_mean = occ_k.mean(axis=0)
+    occ_mean.real[np.abs(occ_mean) < 1e-12] = 0.0
+    return 2.0 * np.trace(occ_mean).real, occ_mean, occ_k.reshape(nk + (nb, nb))
+
+
+def sigma_assemble(sigma_nl, hartree, fock, delta_sigma, sigma_dmft, niv_out, sigma_old=None, alpha=1.0):
+    """sigma + hartree + fock + delta_sigma on the core window, the DMFT self-energy outside it, then
+    linear mixing (nonlocal_sde.py:884-893, 1145-1151; self_energy.py:153-168).
+
+    sigma_nl: [nk,nb,nb,2niv_core]; hartree [nb,nb]; fock [nk,nb,nb]; delta_sigma [nb,nb,2niv_core];
+    sigma_dmft [nb,nb,2niv_dmft]. Returns [nk,nb,nb,2niv_out].
+    """
+    nk = sigma_nl.shape[0]
+    new = sigma_nl + hartree[None, ..., None] + fock[..., None] + delta_sigma[None]
+    nd = sigma_dmft.shape[-1] // 2
+    dmft = np.tile(sigma_dmft[..., nd - niv_out: nd + niv_out][None], (nk, 1, 1, 1))
+    diff = niv_out - new.shape[-1] // 2
+    out = np.concatenate((dmft[..., :diff], new, dmft[..., diff + new.shape[-1]:]), axis=-1)
+    return out if sigma_old is None else alpha * out + (1 - alpha) * sigma_old
 
 
 # -----------------------------------------------------------------------------

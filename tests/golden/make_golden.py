@@ -28,6 +28,8 @@ FILES = {
     "end_2_end/giwk_dga.npy": "e2e_giwk_dga.npy",
     "end_2_end/gamma_irrq_sing_pp.npy": "e2e_gamma_irrq_sing_pp.npy",
     "end_2_end/gamma_irrq_trip_pp.npy": "e2e_gamma_irrq_trip_pp.npy",
+    "end_2_end/sigma_dga_iteration_1.npy": "e2e_sigma_dga_iteration_1.npy",
+    "end_2_end/siw_dga_local.npy": "e2e_siw_dga_local.npy",
     "end_2_end/f_irrq_dens_pp.npy": "e2e_f_irrq_dens_pp.npy",
     "end_2_end/f_irrq_magn_pp.npy": "e2e_f_irrq_magn_pp.npy",
     # lambda correction: 1 band, 4x4x1, beta=12.5

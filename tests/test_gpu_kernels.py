@@ -383,4 +383,12 @@ def test_pipeline_step_matches_oracle(K, cfg, nk, niw, niv):
         kern = kern + wgt * O.kernel_r(vrg, chi, ur, zero_v, ch)
     ref = O.sigma_fft(kern, g, maps["irrk_inv"], m.beta, m.niv, m.niw)
     assert rel(pipe.last["kernel"], K.ref4_to_qf(K.to_dev(kern)).cpu().numpy()) < RTOL
-    assert rel(sig, ref) < RTOL
+    assert rel(pipe.last["sigma_nl"], ref) < RTOL
+    # Hartree-Fock from the occupations of G(sigma_dmft, mu), DMFT tails outside the core window
+    N, nb = int(np.prod(m.nk)), m.nb
+    smom0, _ = O.fit_smom(m.sigma_loc, m.beta)
+    _, occ, occ_k = O.fill_nonlocal(m.sigma_loc[None], m.ek, m.mu, m.beta, smom0)
+    hart, fock = O.hartree_fock(m.u, None, occ, occ_k)
+    new = O.sigma_assemble(ref.reshape(N, nb, nb, -1), hart, fock.reshape(N, nb, nb),
+                           np.zeros((nb, nb, 2 * m.niv)), m.sigma_loc, m.sigma_loc.shape[-1] // 2)
+    assert rel(sig, new) < RTOL
