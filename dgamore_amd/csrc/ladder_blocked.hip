@@ -77,6 +77,9 @@ struct PanelShared {
   S krow[2][NB];
   int piv[NB];
   int mdst[2 * NB], msrc[2 * NB];
+  int mdest[NB];
+  int ocount;
+  int rowval[BLK_MAX_N];  // rowval[r - p0]: the row whose former value row r holds after the interchanges
 };
 
 template <typename S, int NB, int RPT, int C>
@@ -183,68 +186,97 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
   const int nbw = (n - p0 < NB) ? n - p0 : NB;
   __shared__ PanelShared<S, NB> sh;
 
+  // complex64 rows of a full panel move as 16-byte pairs (row starts are 32-byte aligned)
+  constexpr bool pairs = std::is_same<S, float2>::value;
   S P[RPT][NB];
 #pragma unroll
   for (int m = 0; m < RPT; ++m) {
     const int r = p0 + tid + TH * m;
+    const S* src = W + static_cast<size_t>(r) * ld + p0;
+    if (pairs && r < n && nbw == NB) {
 #pragma unroll
-    for (int c = 0; c < NB; ++c) P[m][c] = (r < n && c < nbw) ? W[static_cast<size_t>(r) * ld + p0 + c] : czero<S>();
+      for (int c = 0; c < NB; c += 2) {
+        const float4 x = *reinterpret_cast<const float4*>(src + c);
+        P[m][c] = S{x.x, x.y};
+        P[m][c + 1] = S{x.z, x.w};
+      }
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; ++c) P[m][c] = (r < n && c < nbw) ? src[c] : czero<S>();
+    }
   }
   panel_columns<S, NB, RPT>(P, sh, st, b, p0, nbw, std::make_integer_sequence<int, NB>{});
 #pragma unroll
   for (int m = 0; m < RPT; ++m) {
     const int r = p0 + tid + TH * m;
+    S* dst = W + static_cast<size_t>(r) * ld + p0;
+    if (pairs && r < n && nbw == NB) {
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (r < n && c < nbw) W[static_cast<size_t>(r) * ld + p0 + c] = P[m][c];
+      for (int c = 0; c < NB; c += 2)
+        *reinterpret_cast<float4*>(dst + c) = make_float4(P[m][c].x, P[m][c].y, P[m][c + 1].x, P[m][c + 1].y);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (r < n && c < nbw) dst[c] = P[m][c];
+    }
   }
   // Interchange map of the panel. A row below the panel only ever exchanges its value with a
   // panel row, so after the NB interchanges every row below the panel holds a former panel
   // row's value, and the former values of rows below the panel end in panel rows:
   //   outer[e] = (dst, src): panel row dst receives row src (src below the panel), e < count;
   //   dest[c]:  panel row p0 + c's former value goes to row dest[c] (-1: stays).
-  if (tid == 0) {
-    int* dst = sh.mdst;
-    int* src = sh.msrc;
-    int cnt = 0;
-    for (int c = 0; c < nbw; ++c) {
-      const int j = p0 + c, p = sh.piv[c];
-      st.piv[static_cast<size_t>(b) * n + j] = p;
-      if (p == j) continue;
-      int ij = -1, ip = -1;
-      for (int e = 0; e < cnt; ++e) {
-        if (dst[e] == j) ij = e;
-        if (dst[e] == p) ip = e;
-      }
-      if (ij < 0) {
-        ij = cnt++;
-        dst[ij] = j;
-        src[ij] = j;
-      }
-      if (ip < 0) {
-        ip = cnt++;
-        dst[ip] = p;
-        src[ip] = p;
-      }
-      const int t = src[ij];
-      src[ij] = src[ip];
-      src[ip] = t;
-    }
-    int* map = st.rowmap + (static_cast<size_t>(b) * 2 + slot) * MAP_STRIDE;
-    for (int c = 0; c < NB; ++c) map[1 + 2 * NB + c] = -1;
-    int outer = 0;
-    for (int e = 0; e < cnt; ++e) {
-      if (src[e] == dst[e]) continue;
-      if (src[e] < p0 + nbw) {
-        map[1 + 2 * NB + (src[e] - p0)] = dst[e];
-      } else {
-        map[1 + outer] = dst[e];
-        map[1 + NB + outer] = src[e];
-        ++outer;
-      }
-    }
-    map[0] = outer;
+  // The interchanges are replayed on an LDS table of the touched rows (one thread, one swap per
+  // column), then every panel row and every pivot row below the panel is classified in parallel.
+  int* rv = sh.rowval;
+  const int pbase = p0 + nbw;
+  if (tid < nbw) {
+    const int p = sh.piv[tid];
+    rv[tid] = p0 + tid;
+    rv[p - p0] = p;  // a row chosen by several columns gets the same value from each
+    sh.mdest[tid] = -1;
+    st.piv[static_cast<size_t>(b) * n + p0 + tid] = p;
   }
+  if (tid == 0) sh.ocount = 0;
+  __syncthreads();
+  if (tid == 0) {
+    for (int c = 0; c < nbw; ++c) {
+      const int p = sh.piv[c] - p0;
+      if (p != c) {
+        const int t = rv[c];
+        rv[c] = rv[p];
+        rv[p] = t;
+      }
+    }
+  }
+  __syncthreads();
+  if (tid < nbw) {
+    const int r = p0 + tid, src = rv[tid];
+    if (src != r) {
+      if (src < pbase) {
+        sh.mdest[src - p0] = r;
+      } else {
+        const int e = atomicAdd(&sh.ocount, 1);
+        sh.mdst[e] = r;
+        sh.msrc[e] = src;
+      }
+    }
+    const int p = sh.piv[tid];
+    if (p >= pbase) {  // a pivot row below the panel, classified by the first column that chose it
+      bool first = true;
+      for (int c = 0; c < tid; ++c) first = first && sh.piv[c] != p;
+      const int sp = rv[p - p0];
+      if (first && sp != p) sh.mdest[sp - p0] = p;
+    }
+  }
+  __syncthreads();
+  int* map = st.rowmap + (static_cast<size_t>(b) * BLK_PANELS + slot) * MAP_STRIDE;
+  const int outer = sh.ocount;
+  if (tid < NB) map[1 + 2 * NB + tid] = tid < nbw ? sh.mdest[tid] : -1;
+  if (tid < outer) {
+    map[1 + tid] = sh.mdst[tid];
+    map[1 + NB + tid] = sh.msrc[tid];
+  }
+  if (tid == 0) map[0] = outer;
 }
 
 // ---------------------------------------------------------------------------
@@ -264,7 +296,7 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
   const int nbw = (n - p0 < NB) ? n - p0 : NB;
   __shared__ S s_l[NB][NB + 1];
   __shared__ int s_dst[NB], s_src[NB], s_dest[NB];
-  const int* map = st.rowmap + (static_cast<size_t>(b) * 2 + slot) * MAP_STRIDE;
+  const int* map = st.rowmap + (static_cast<size_t>(b) * BLK_PANELS + slot) * MAP_STRIDE;
   const int outer = map[0];
   for (int e = threadIdx.x; e < NB; e += 64) {
     s_dst[e] = e < outer ? map[1 + e] : 0;
@@ -694,15 +726,20 @@ struct BlockedSteps {
   }
 };
 
-// Right-looking LU in outer blocks of 2*NB columns, each factored as two register panels
-// of NB columns, so that the trailing matrix is read and written once per 2*NB columns:
-//   panel 1; its interchanges + U12 on panel 2's columns, panel 2's columns -= L21 U12;
-//   panel 2; its interchanges on panel 1's columns;
-//   trailing columns: interchanges 1 + U12 (rows of panel 1), interchanges 2,
-//     rows of panel 2 -= L(panel 2 rows, panel 1) U12, U12 of panel 2;
-//   rows below the block -= L21 (2*NB columns) U12 (2*NB rows).
-template <typename S, int NB, int RPT, int MAXTH>
+// Right-looking LU in outer blocks of NPAN * NB columns, each factored as NPAN register
+// panels of NB columns, so that the trailing matrix is read and written once per
+// outer block. Inside the block the panels are brought up to date left-looking: before panel
+// j, for every earlier panel i, its interchanges, the update of its rows by the panels before
+// it and its U block are applied to panel j's columns, then panel j's rows and the rows below
+// are updated by all earlier panels at once; after panel j its interchanges are applied to the
+// earlier panels' columns. The trailing columns then receive, panel by panel, the
+// interchanges, the row-block update and the U blocks, and the rows below the block one
+// trailing update of depth NPAN * NB. Complex128 uses two panels per block (its vector-FMA
+// trailing update is compute-bound, and the extra in-block updates of deeper blocks cost more
+// than they save); complex64 uses four, halving the trailing matrix's HBM traffic.
+template <typename S, int NB, int RPT, int MAXTH, int NPAN>
 static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream) {
+  static_assert(NPAN >= 1 && NPAN <= BLK_PANELS, "panels per outer block");
   const int n = st.n, ncol = st.ncol;
   DGAMI_ARG_CHECK(nb <= 65535, "ladder: batch too large");
   hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(ceil_div(n, ASM_ROWS), nb), dim3(256), 0, stream, a, st, item0);
@@ -712,24 +749,37 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
   int rc = 0;
 #define DGAMI_STEP(x) \
   if ((rc = (x)) != 0) return rc
-  for (int P = 0; P < n; P += 2 * NB) {
-    const int w1 = min(NB, n - P);
-    DGAMI_STEP(k.panel(P, 0));
-    const int P2 = P + NB;
-    if (P2 >= n) {  // last, single panel: only the right-hand-side columns remain
-      DGAMI_STEP(k.rows(P, 0, P + w1, ncol, P, BOTH));
-      break;
+  for (int P = 0; P < n; P += NPAN * NB) {
+    const int np = min(NPAN, static_cast<int>(ceil_div(n - P, NB)));
+    const int E = min(n, P + NPAN * NB);  // end of the outer block
+    for (int j = 0; j < np; ++j) {
+      const int Pj = P + j * NB, Ej = min(n, Pj + NB);
+      if (j > 0) {
+        for (int i = 0; i < j; ++i) {
+          const int Pi = P + i * NB;
+          if (i == 0) {
+            DGAMI_STEP(k.rows(Pi, i, Pj, Ej, 0, BOTH));
+          } else {
+            DGAMI_STEP(k.rows(Pi, i, Pj, Ej, 0, ROWS_SWAP));
+            DGAMI_STEP(k.gemm(Pi, Pi + NB, Pj, Ej, P, i * NB));
+            DGAMI_STEP(k.rows(Pi, i, Pj, Ej, 0, ROWS_TRSM));
+          }
+        }
+        DGAMI_STEP(k.gemm(Pj, n, Pj, Ej, P, j * NB));
+      }
+      DGAMI_STEP(k.panel(Pj, j));
+      if (j > 0) DGAMI_STEP(k.rows(Pj, j, P, Pj, 0, ROWS_SWAP));
     }
-    const int w2 = min(NB, n - P2);
-    const int E = P2 + w2;  // end of the outer block
-    DGAMI_STEP(k.rows(P, 0, P2, E, 0, BOTH));
-    DGAMI_STEP(k.gemm(P2, n, P2, E, P, NB));
-    DGAMI_STEP(k.panel(P2, 1));
-    DGAMI_STEP(k.rows(P2, 1, P, P2, 0, ROWS_SWAP));
-    DGAMI_STEP(k.rows(P, 0, E, ncol, P, BOTH));
-    DGAMI_STEP(k.rows(P2, 1, E, ncol, P, ROWS_SWAP));
-    DGAMI_STEP(k.gemm(P2, E, E, ncol, P, NB));
-    DGAMI_STEP(k.rows(P2, 1, E, ncol, 0, ROWS_TRSM));
+    for (int j = 0; j < np; ++j) {
+      const int Pj = P + j * NB, Ej = min(n, Pj + NB);
+      if (j == 0) {
+        DGAMI_STEP(k.rows(Pj, j, E, ncol, P, BOTH));
+      } else {
+        DGAMI_STEP(k.rows(Pj, j, E, ncol, P, ROWS_SWAP));
+        DGAMI_STEP(k.gemm(Pj, Ej, E, ncol, P, j * NB));
+        DGAMI_STEP(k.rows(Pj, j, E, ncol, 0, ROWS_TRSM));
+      }
+    }
     DGAMI_STEP(k.gemm(E, n, E, ncol, P, E - P));
   }
 #undef DGAMI_STEP
@@ -738,14 +788,15 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
 
 template <>
 int blocked_factor<double2>(const LadderArgs& a, BlockedState<double2>& st, int item0, int nb, hipStream_t stream) {
-  if (st.n <= 768) return blocked_factor_nb<double2, 32, 1, 768>(a, st, item0, nb, stream);
-  return blocked_factor_nb<double2, 16, 3, 512>(a, st, item0, nb, stream);
+  if (st.n <= 768) return blocked_factor_nb<double2, 32, 1, 768, 2>(a, st, item0, nb, stream);
+  return blocked_factor_nb<double2, 16, 3, 512, 2>(a, st, item0, nb, stream);
 }
 
 template <>
 int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int item0, int nb, hipStream_t stream) {
-  if (st.n <= 1024) return blocked_factor_nb<float2, 32, 1, 1024>(a, st, item0, nb, stream);
-  return blocked_factor_nb<float2, 16, 3, 512>(a, st, item0, nb, stream);
+  // three rows per thread: 256-thread panels for n <= 768, two problems per CU
+  if (st.n <= 1152) return blocked_factor_nb<float2, 32, 3, 384, 4>(a, st, item0, nb, stream);
+  return blocked_factor_nb<float2, 16, 3, 512, 4>(a, st, item0, nb, stream);
 }
 
 int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,
@@ -758,7 +809,7 @@ int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0,
 int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream) {
   const int n = a.n, ncol = n + a.T;
   const int ld = (ncol + 3) & ~3;
-  DGAMI_ARG_CHECK(n <= 1536, "ladder: compound size above 1536 is not supported");
+  DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);
   // MI355X holds 288 GB of HBM: factor up to 48 GiB of problems per pass

@@ -7,7 +7,9 @@
 namespace dgami {
 
 constexpr int BLK_MAX_NB = 64;                      // widest panel
+constexpr int BLK_MAX_N = 1536;                     // largest compound size of the blocked LU
 constexpr int BLK_MAP_STRIDE = 3 * BLK_MAX_NB + 1;  // ints of one problem's interchange map
+constexpr int BLK_PANELS = 4;                       // register panels per outer block (trailing-update depth)
 
 // Trailing-update kernels of the complex64 factorisation (complex128 always uses vector FMA).
 enum BlockedGemm { BLK_GEMM_VECTOR = 0, BLK_GEMM_F32_MFMA = 1, BLK_GEMM_BF16X3 = 2 };
@@ -17,7 +19,7 @@ struct BlockedState {
   S* W;            // [batch][n][ld] matrices (row major), factored in place
   double2* dinv;   // [batch][n] inverse pivots (FP64 copy)
   int* piv;        // [batch][n] pivot row of every elimination step
-  int* rowmap;     // [batch][2][BLK_MAP_STRIDE] interchange maps of the two panels of an outer block
+  int* rowmap;     // [batch][BLK_PANELS][BLK_MAP_STRIDE] interchange maps of the panels of an outer block
   int* info;       // [batch] first zero pivot (1-based) or 0
   int n, ld, ncol;
   int swap_left;   // also apply interchanges left of the panel (needed when L is reused)
@@ -28,14 +30,14 @@ struct BlockedState {
 template <typename S>
 inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t batch, int ws) {
   st.W = static_cast<S*>(workspace(ws, batch * n * ld * sizeof(S)));
-  const size_t aux = batch * n * sizeof(double2) + batch * n * sizeof(int) + batch * 2 * BLK_MAP_STRIDE * sizeof(int) +
+  const size_t aux = batch * n * sizeof(double2) + batch * n * sizeof(int) + batch * BLK_PANELS * BLK_MAP_STRIDE * sizeof(int) +
                      batch * sizeof(int) + 1024;
   char* p = static_cast<char*>(workspace(ws + 1, aux));
   if (!st.W || !p) return false;
   st.dinv = reinterpret_cast<double2*>(p);
   st.piv = reinterpret_cast<int*>(st.dinv + batch * n);
   st.rowmap = st.piv + batch * n;
-  st.info = st.rowmap + batch * 2 * BLK_MAP_STRIDE;
+  st.info = st.rowmap + batch * BLK_PANELS * BLK_MAP_STRIDE;
   st.n = n;
   st.ld = ld;
   st.ncol = ncol;

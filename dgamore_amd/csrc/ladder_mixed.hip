@@ -194,7 +194,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   const int max_solves = precision == DGAMI_MIXED_BF16X3_IR ? MIXED_MAX_SOLVES_BF16 : MIXED_MAX_SOLVES_FP32;
   const int n = a.n, T = a.T;
   DGAMI_ARG_CHECK(T <= 16, "ladder: at most 4 orbitals in the mixed-precision form");
-  DGAMI_ARG_CHECK(n <= 1536, "ladder: compound size above 1536 is not supported");
+  DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
   const int ld = (n + 7) & ~7;
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const int gq = a.dense ? 1 : a.nq;

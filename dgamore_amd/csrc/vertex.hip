@@ -233,7 +233,7 @@ enum VertexOut { VTX_OUT_INVERSE = 0, VTX_OUT_FULL = 1 };
 static int vertex_solve(LadderArgs a, int what, const VertexArgs& o, hipStream_t stream) {
   const int n = a.n, nr = n, ncol = 2 * n;
   const int ld = (ncol + 3) & ~3;
-  DGAMI_ARG_CHECK(n <= 1536, "vertex: compound size above 1536 is not supported");
+  DGAMI_ARG_CHECK(n <= BLK_MAX_N, "vertex: compound size above 1536 is not supported");
   a.rhs_mode = what == VTX_OUT_FULL ? LADDER_RHS_CHI0INV : LADDER_RHS_IDENTITY;
   const size_t items = static_cast<size_t>(a.nw) * a.nq;
   const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);
