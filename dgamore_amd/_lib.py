@@ -50,8 +50,11 @@ SIGNATURES = {
     "dgami_sigma_assemble": ([_i, _i, _i, _i, _c, _c, _c, _c, _c, _i, _c, _d, _c, _c], _i),
     "dgami_fill_sums": ([_i, _i, _i, _d, _d, _c, _c, _i, _c, _c, _c, _c, _c], _i),
     "dgami_kmean": ([_i, _i64, _c, _c, _c], _i),
+    "dgami_chi0_pp": ([_i, _i, _i, _i, _c, _c, _c], _i),
     "dgami_pairing_setup": ([_i, _i, _i, _i, _i, _i, _c, _c, _i, _c, _c, _c], _i),
     "dgami_pairing_matvec": ([_i, _i, _i, _i, _i, _i, _d, _c, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_pairing_setup_c64": ([_i, _i, _i, _i, _i, _i, _c, _c, _i, _c, _c, _c], _i),
+    "dgami_pairing_matvec_c64": ([_i, _i, _i, _i, _i, _i, _d, _c, _c, _c, _i, _c, _c, _c], _i),
 }
 
 _lib = None

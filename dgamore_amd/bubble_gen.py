@@ -60,10 +60,10 @@ class BubbleGenerator:
 
     @staticmethod
     def create_generalized_chi0_q_pp_w0(giwk, niv_pp, q_grid):
-        """chi0pp_{abcd}(k,v) = G_ad(k,v) conj(G_cb(k,v)) (bubble_gen.py:268-279); elementwise input of the matvec."""
+        """chi0pp_{abcd}(k,v) = G_ad(k,v) conj(G_cb(k,v)) (bubble_gen.py:268-279), the elementwise input of the
+        pairing matvec, computed by dgami_chi0_pp. Returns a FourPoint [nk, nb, nb, nb, nb, 2 niv_pp]."""
         mat = _giwk_mat(giwk)
-        niv = mat.shape[-1] // 2
-        g = mat[..., niv - niv_pp: niv + niv_pp].reshape((-1,) + mat.shape[3:5] + (2 * niv_pp,))
-        pp = g[:, :, None, None, :, :] * np.conj(g.swapaxes(1, 2))[:, None, :, :, None, :]
-        pp[(np.abs(pp.real) < 1e-12) & (np.abs(pp.imag) < 1e-12)] = 0.0
-        return FourPoint(pp, SpinChannel.NONE, tuple(q_grid.nk), 0, 1, True, True, True, FrequencyNotation.PP)
+        nb = mat.shape[3]
+        pp = K.chi0_pp(K.to_dev(mat), niv_pp).reshape((-1,) + (nb,) * 4 + (2 * niv_pp,))
+        return FourPoint(pp.cpu().numpy(), SpinChannel.NONE, tuple(q_grid.nk), 0, 1, True, True, True,
+                         FrequencyNotation.PP)

@@ -14,6 +14,7 @@ lattice = SimpleNamespace(nk=(1, 1, 1), nq=(1, 1, 1), k_grid=None, q_grid=None)
 self_consistency = SimpleNamespace(mixing=1.0, max_iter=1, epsilon=1e-4, mixing_strategy="linear",
                                    mixing_history_length=2)
 lambda_correction = SimpleNamespace(perform_lambda_correction=False, type="spch")
-eliashberg = SimpleNamespace(n_eig=4, epsilon=1e-12, symmetry="random", save_fq=False, construct_fq_cheap=False)
+eliashberg = SimpleNamespace(n_eig=4, epsilon=1e-12, symmetry="random", save_fq=False, construct_fq_cheap=False,
+                             matvec_precision="fp64")
 memory = SimpleNamespace(save_memory_for_chiq_aux=False, save_memory_for_chi0q=False, save_memory_for_sde=False)
 precision = SimpleNamespace(ladder=0)  # 0 plain FP64, 1 FP32 + FP64 refinement, 2 split-BF16 + FP64 refinement

@@ -30,9 +30,10 @@ __global__ void pair_gather_kernel(const double2* __restrict__ girr, const int32
   }
 }
 
-// hat: [nb][nb][nb][nb][np2][np2][N] (reference element order, R fastest)
-__global__ void pair_scatter_kernel(const double2* __restrict__ hat, double2* __restrict__ g_hat,
-                                    double2* __restrict__ g_flip, FftPlan3D plan, int nb, int np2) {
+// hat: [nb][nb][nb][nb][np2][np2][N] (reference element order, R fastest); M = double2 or float2
+template <typename M>
+__global__ void pair_scatter_kernel(const double2* __restrict__ hat, M* __restrict__ g_hat, M* __restrict__ g_flip,
+                                    FftPlan3D plan, int nb, int np2) {
   const int N = plan.ntot;
   const int n = nb * nb * np2;
   const size_t total = static_cast<size_t>(N) * n * n;
@@ -48,8 +49,8 @@ __global__ void pair_scatter_kernel(const double2* __restrict__ hat, double2* __
                    (plan.n[2] - z) % plan.n[2];
     const size_t e1 = (((((static_cast<size_t>(a) * nb + c) * nb + b) * nb + d) * np2 + v) * np2 + p);
     const size_t e2 = (((((static_cast<size_t>(a) * nb + d) * nb + b) * nb + c) * np2 + v) * np2 + (np2 - 1 - p));
-    g_hat[idx] = hat[e1 * N + R];
-    g_flip[idx] = hat[e2 * N + Rm];
+    g_hat[idx] = from_c128<M>(hat[e1 * N + R]);
+    g_flip[idx] = from_c128<M>(hat[e2 * N + Rm]);
   }
 }
 
@@ -75,10 +76,11 @@ __global__ void pair_bubble_kernel(const double2* __restrict__ chi0pp, const dou
 }
 
 // y[vec][row][R] = sum_col g_hat[R][row][col] x[vec][col][R] + s g_flip[R][row][col] x[vec][col][-R]
-__global__ void __launch_bounds__(256) pair_gemv_kernel(const double2* __restrict__ g_hat,
-                                                       const double2* __restrict__ g_flip, const double2* __restrict__ x,
-                                                       double2* __restrict__ y, FftPlan3D plan, int n, int nvec,
-                                                       double sign) {
+// The matrices are complex128 or complex64 (M); products and sums are complex128.
+template <typename M>
+__global__ void __launch_bounds__(256) pair_gemv_kernel(const M* __restrict__ g_hat, const M* __restrict__ g_flip,
+                                                       const double2* __restrict__ x, double2* __restrict__ y,
+                                                       FftPlan3D plan, int n, int nvec, double sign) {
   extern __shared__ double2 s_x[];  // [2][nvec][n]: x(R) and x(-R)
   const int N = plan.ntot;
   const int R = blockIdx.x;
@@ -91,15 +93,15 @@ __global__ void __launch_bounds__(256) pair_gemv_kernel(const double2* __restric
   }
   __syncthreads();
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const double2* M1 = g_hat + static_cast<size_t>(R) * n * n;
-  const double2* M2 = g_flip + static_cast<size_t>(R) * n * n;
+  const M* M1 = g_hat + static_cast<size_t>(R) * n * n;
+  const M* M2 = g_flip + static_cast<size_t>(R) * n * n;
   for (int row = wave; row < n; row += 4) {
     double2 acc[PAIR_MAXVEC];
 #pragma unroll
     for (int vv = 0; vv < PAIR_MAXVEC; ++vv) acc[vv] = make_double2(0.0, 0.0);
     for (int col = lane; col < n; col += 64) {
-      const double2 m1 = M1[static_cast<size_t>(row) * n + col];
-      const double2 m2 = cscale(M2[static_cast<size_t>(row) * n + col], sign);
+      const double2 m1 = to_c128(M1[static_cast<size_t>(row) * n + col]);
+      const double2 m2 = cscale(to_c128(M2[static_cast<size_t>(row) * n + col]), sign);
 #pragma unroll
       for (int vv = 0; vv < PAIR_MAXVEC; ++vv) {
         if (vv < nvec) {
@@ -134,15 +136,31 @@ __global__ void pair_out_kernel(const double2* __restrict__ t, double2* __restri
   }
 }
 
+// chi0pp[k][abcd][v] = G_ad(k, v) conj(G_cb(k, v)) over the centred 2*np of 2*niv_g frequencies,
+// |Re|, |Im| < 1e-12 -> 0 (bubble_gen.py:268-279, n_point_base.py:205-211).
+__global__ void pair_chi0pp_kernel(const double2* __restrict__ g, double2* __restrict__ out, int nk, int nb, int niv_g,
+                                   int np) {
+  const int T = nb * nb, P2 = 2 * np;
+  const size_t total = static_cast<size_t>(nk) * T * T * P2;
+  for (size_t idx = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int v = static_cast<int>(idx % P2);
+    const int o = static_cast<int>((idx / P2) % (T * T));
+    const int k = static_cast<int>(idx / (static_cast<size_t>(P2) * T * T));
+    const int d = o % nb, c = (o / nb) % nb, b = (o / T) % nb, a = o / (T * nb);
+    const size_t base = static_cast<size_t>(k) * T * 2 * niv_g + (niv_g - np) + v;
+    const double2 x = cmul(g[base + static_cast<size_t>(a * nb + d) * 2 * niv_g],
+                           cconj(g[base + static_cast<size_t>(c * nb + b) * 2 * niv_g]));
+    out[idx] = make_double2(fabs(x.x) < 1e-12 && fabs(x.y) < 1e-12 ? 0.0 : x.x,
+                            fabs(x.x) < 1e-12 && fabs(x.y) < 1e-12 ? 0.0 : x.y);
+  }
+}
+
 inline unsigned grid_for(size_t total) { return ceil_div(total, 256) > 262144u ? 262144u : ceil_div(total, 256); }
 
-}  // namespace dgami
-
-extern "C" int dgami_pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel,
-                                   const dgami_c128* gamma_pp_irr, const int32_t* irrk_inv, int nq_irr,
-                                   dgami_c128* g_hat, dgami_c128* g_hat_flip, void* stream_) {
-  using namespace dgami;
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
+template <typename M>
+static int pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel, const dgami_c128* gamma_pp_irr,
+                         const int32_t* irrk_inv, int nq_irr, M* g_hat, M* g_hat_flip, hipStream_t stream) {
   DGAMI_ARG_CHECK(gamma_pp_irr && irrk_inv && g_hat && g_hat_flip, "pairing_setup: null pointer");
   DGAMI_ARG_CHECK(nb >= 1 && np2 >= 1 && nq_irr >= 1, "pairing_setup: invalid sizes");
   DGAMI_ARG_CHECK(channel == 0 || channel == 1, "pairing_setup: channel must be 0 (singlet) or 1 (triplet)");
@@ -160,17 +178,16 @@ extern "C" int dgami_pairing_setup(int nx, int ny, int nz, int nb, int np2, int 
   int rc = launch_fft_grid(hat, hat, nelem, nx, ny, nz, -1, 1.0, stream);
   if (rc) return rc;
   const int n = nb * nb * np2;
-  hipLaunchKernelGGL(pair_scatter_kernel, dim3(grid_for(static_cast<size_t>(N) * n * n)), dim3(256), 0, stream, hat,
-                     reinterpret_cast<double2*>(g_hat), reinterpret_cast<double2*>(g_hat_flip), plan, nb, np2);
+  hipLaunchKernelGGL(pair_scatter_kernel<M>, dim3(grid_for(static_cast<size_t>(N) * n * n)), dim3(256), 0, stream,
+                     hat, g_hat, g_hat_flip, plan, nb, np2);
   DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
 }
 
-extern "C" int dgami_pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta,
-                                    const dgami_c128* g_hat, const dgami_c128* g_hat_flip, const dgami_c128* chi0pp,
-                                    int nvec, const dgami_c128* gap, dgami_c128* out, void* stream_) {
-  using namespace dgami;
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
+template <typename M>
+static int pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta, const M* g_hat,
+                          const M* g_hat_flip, const dgami_c128* chi0pp, int nvec, const dgami_c128* gap,
+                          dgami_c128* out, hipStream_t stream) {
   DGAMI_ARG_CHECK(g_hat && g_hat_flip && chi0pp && gap && out, "pairing_matvec: null pointer");
   DGAMI_ARG_CHECK(nvec >= 1 && nvec <= PAIR_MAXVEC, "pairing_matvec: nvec must be 1..8");
   DGAMI_ARG_CHECK(channel == 0 || channel == 1, "pairing_matvec: channel must be 0 or 1");
@@ -192,15 +209,63 @@ extern "C" int dgami_pairing_matvec(int nx, int ny, int nz, int nb, int np2, int
   DGAMI_HIP_CHECK(hipGetLastError());
   int rc = launch_fft_grid(t, t, static_cast<int64_t>(nvec) * n, nx, ny, nz, -1, 1.0, stream);
   if (rc) return rc;
-  allow_lds(pair_gemv_kernel, lds);
-  hipLaunchKernelGGL(pair_gemv_kernel, dim3(N), dim3(256), lds, stream, reinterpret_cast<const double2*>(g_hat),
-                     reinterpret_cast<const double2*>(g_hat_flip), t, yv, plan, n, nvec, channel == 0 ? 1.0 : -1.0);
+  allow_lds(pair_gemv_kernel<M>, lds);
+  hipLaunchKernelGGL(pair_gemv_kernel<M>, dim3(N), dim3(256), lds, stream, g_hat, g_hat_flip, t, yv, plan, n, nvec,
+                     channel == 0 ? 1.0 : -1.0);
   DGAMI_HIP_CHECK(hipGetLastError());
   rc = launch_fft_grid(yv, yv, static_cast<int64_t>(nvec) * n, nx, ny, nz, 1, 0.5 / (static_cast<double>(N) * beta) / N,
                        stream);
   if (rc) return rc;
   hipLaunchKernelGGL(pair_out_kernel, dim3(grid_for(vsz)), dim3(256), 0, stream, yv, reinterpret_cast<double2*>(out), N,
                      n, nvec);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
+}  // namespace dgami
+
+extern "C" int dgami_pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel,
+                                   const dgami_c128* gamma_pp_irr, const int32_t* irrk_inv, int nq_irr,
+                                   dgami_c128* g_hat, dgami_c128* g_hat_flip, void* stream) {
+  using namespace dgami;
+  return pairing_setup(nx, ny, nz, nb, np2, channel, gamma_pp_irr, irrk_inv, nq_irr, reinterpret_cast<double2*>(g_hat),
+                       reinterpret_cast<double2*>(g_hat_flip), static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_pairing_setup_c64(int nx, int ny, int nz, int nb, int np2, int channel,
+                                       const dgami_c128* gamma_pp_irr, const int32_t* irrk_inv, int nq_irr,
+                                       dgami_c64* g_hat, dgami_c64* g_hat_flip, void* stream) {
+  using namespace dgami;
+  return pairing_setup(nx, ny, nz, nb, np2, channel, gamma_pp_irr, irrk_inv, nq_irr, reinterpret_cast<float2*>(g_hat),
+                       reinterpret_cast<float2*>(g_hat_flip), static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta,
+                                    const dgami_c128* g_hat, const dgami_c128* g_hat_flip, const dgami_c128* chi0pp,
+                                    int nvec, const dgami_c128* gap, dgami_c128* out, void* stream) {
+  using namespace dgami;
+  return pairing_matvec(nx, ny, nz, nb, np2, channel, beta, reinterpret_cast<const double2*>(g_hat),
+                        reinterpret_cast<const double2*>(g_hat_flip), chi0pp, nvec, gap, out,
+                        static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_pairing_matvec_c64(int nx, int ny, int nz, int nb, int np2, int channel, double beta,
+                                        const dgami_c64* g_hat, const dgami_c64* g_hat_flip, const dgami_c128* chi0pp,
+                                        int nvec, const dgami_c128* gap, dgami_c128* out, void* stream) {
+  using namespace dgami;
+  return pairing_matvec(nx, ny, nz, nb, np2, channel, beta, reinterpret_cast<const float2*>(g_hat),
+                        reinterpret_cast<const float2*>(g_hat_flip), chi0pp, nvec, gap, out,
+                        static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_chi0_pp(int nk, int nb, int niv_g, int np, const dgami_c128* g_k, dgami_c128* chi0pp,
+                             void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(g_k && chi0pp, "chi0_pp: null pointer");
+  DGAMI_ARG_CHECK(nk >= 1 && nb >= 1 && nb <= 4 && np >= 1 && np <= niv_g, "chi0_pp: invalid sizes (np <= niv_g)");
+  const size_t total = static_cast<size_t>(nk) * nb * nb * nb * nb * 2 * np;
+  hipLaunchKernelGGL(pair_chi0pp_kernel, dim3(grid_for(total)), dim3(256), 0, static_cast<hipStream_t>(stream),
+                     reinterpret_cast<const double2*>(g_k), reinterpret_cast<double2*>(chi0pp), nk, nb, niv_g, np);
   DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
 }

@@ -81,7 +81,9 @@ def create_pairing_vertices_pp(f_dens, f_magn, niv_pp):
 class PairingOperator:
     """Device-resident linearised gap operator of one channel."""
 
-    def __init__(self, gamma_pp_irr, chi0pp, q_grid, beta, channel):
+    def __init__(self, gamma_pp_irr, chi0pp, q_grid, beta, channel, matrix_precision="fp64"):
+        """matrix_precision "fp32" stores the transformed vertex in complex64 (half the bytes per
+        application); the vectors and the sums stay complex128."""
         import torch
 
         self.nk = tuple(q_grid.nk)
@@ -89,8 +91,9 @@ class PairingOperator:
         self.beta = beta
         gam = np.asarray(gamma_pp_irr)
         self.nb, self.np2 = gam.shape[1], gam.shape[-1]
+        storage = {"fp64": torch.complex128, "fp32": torch.complex64}[matrix_precision]
         self.g_hat, self.g_flip = K.pairing_setup(K.to_dev(gam), K.to_dev(np.asarray(q_grid.irrk_inv), torch.int32),
-                                                  self.nk, channel)
+                                                  self.nk, channel, storage)
         n_k = int(np.prod(self.nk))
         self.chi0pp = K.to_dev(np.asarray(chi0pp).reshape(n_k, self.nb**4, self.np2))
         self.shape = self.nk + (self.nb, self.nb, self.np2)
@@ -113,7 +116,8 @@ def solve_eliashberg_lanczos(gamma_r_pp, gchi0_q0_pp, ranks=(0, 0), v0=None):
     Returns (lambdas sorted descending, [gap arrays [nx,ny,nz,nb,nb,2np]]).
     """
     ch = gamma_r_pp.channel.value if isinstance(gamma_r_pp.channel, SpinChannel) else str(gamma_r_pp.channel)
-    op = PairingOperator(gamma_r_pp.mat, gchi0_q0_pp.mat, config.lattice.q_grid, config.sys.beta, ch)
+    op = PairingOperator(gamma_r_pp.mat, gchi0_q0_pp.mat, config.lattice.q_grid, config.sys.beta, ch,
+                         getattr(config.eliashberg, "matvec_precision", "fp64"))
     lin = spla.LinearOperator(shape=(op.dim, op.dim), matvec=op.matvec, dtype=np.complex128)
     n_eig = config.eliashberg.n_eig
     lambdas, gaps = spla.eigsh(lin, k=n_eig, tol=config.eliashberg.epsilon, v0=v0, which="LA", maxiter=10000)

@@ -299,17 +299,27 @@ def kmean(x):
     return out
 
 
-def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel):
-    """gamma_pp_irr [nq,nb,nb,nb,nb,np2,np2] -> (g_hat, g_hat_flip) [N][n][n]."""
+def chi0_pp(giwk, niv_pp):
+    """chi0pp [N][nb^4 (abcd)][2 niv_pp] from G(k) [..., nb, nb, 2 niv_g] (REF2 or [N,nb,nb,2niv_g])."""
+    nb, nvg = giwk.shape[-2], giwk.shape[-1]
+    g = giwk.reshape(-1, nb, nb, nvg).contiguous()
+    out = torch.empty((g.shape[0], nb**4, 2 * niv_pp), dtype=C128, device=giwk.device)
+    _lib.call("dgami_chi0_pp", g.shape[0], nb, nvg // 2, int(niv_pp), _p(g), _p(out), _stream())
+    return out
+
+
+def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel, storage=C128):
+    """gamma_pp_irr [nq,nb,nb,nb,nb,np2,np2] -> (g_hat, g_hat_flip) [N][n][n] in complex128 or complex64."""
     nx, ny, nz = nk
     nq, nb, np2 = gamma_pp_irr.shape[0], gamma_pp_irr.shape[1], gamma_pp_irr.shape[-1]
     n = nb * nb * np2
     N = nx * ny * nz
-    g_hat = torch.empty((N, n, n), dtype=C128, device=gamma_pp_irr.device)
+    g_hat = torch.empty((N, n, n), dtype=storage, device=gamma_pp_irr.device)
     g_flip = torch.empty_like(g_hat)
     ch = {"sing": 0, "trip": 1}[channel]
-    _lib.call("dgami_pairing_setup", nx, ny, nz, nb, np2, ch, _p(gamma_pp_irr), _p(irrk_inv, torch.int32), nq,
-              _p(g_hat), _p(g_flip), _stream())
+    name = "dgami_pairing_setup" if storage == C128 else "dgami_pairing_setup_c64"
+    _lib.call(name, nx, ny, nz, nb, np2, ch, _p(gamma_pp_irr), _p(irrk_inv, torch.int32), nq,
+              _p(g_hat, storage), _p(g_flip, storage), _stream())
     return g_hat, g_flip
 
 
@@ -322,6 +332,7 @@ def pairing_matvec(g_hat, g_flip, chi0pp, gap, nk, channel, beta):
     g = g.contiguous()
     out = torch.empty_like(g)
     ch = {"sing": 0, "trip": 1}[channel]
-    _lib.call("dgami_pairing_matvec", nx, ny, nz, nb, np2, ch, float(beta), _p(g_hat), _p(g_flip), _p(chi0pp),
-              nvec, _p(g), _p(out), _stream())
+    name = "dgami_pairing_matvec" if g_hat.dtype == C128 else "dgami_pairing_matvec_c64"
+    _lib.call(name, nx, ny, nz, nb, np2, ch, float(beta), _p(g_hat, g_hat.dtype), _p(g_flip, g_hat.dtype),
+              _p(chi0pp), nvec, _p(g), _p(out), _stream())
     return out[0] if squeeze else out

@@ -46,6 +46,11 @@ typedef struct {
   double re, im;
 } dgami_c128;
 
+/* Complex64: interleaved (re, im) floats, the layout of numpy.complex64. */
+typedef struct {
+  float re, im;
+} dgami_c64;
+
 /* Ladder precision modes (dgami_ladder_solve_sum). */
 enum {
   DGAMI_PLAIN_FP64 = 0,    /* complex128 LU, the reference's algorithm in FP64 */
@@ -245,6 +250,12 @@ int dgami_fill_sums(int nk, int nb, int niv, double beta, double mu, const dgami
 /* out[e] = (1/nk) sum_k x[k][e], e < m (the k-mean of SelfEnergy.fit_smom, self_energy.py:60). */
 int dgami_kmean(int nk, int64_t m, const dgami_c128* x, dgami_c128* out, void* stream);
 
+/* Particle-particle bubble at w = 0: chi0pp[k][abcd][v] = G_ad(k,v) conj(G_cb(k,v)) on the centred 2*np of
+ * the 2*niv_g frequencies of g_k (REF2 [nk][nb][nb][2*niv_g]), |Re|,|Im| < 1e-12 -> 0; chi0pp: [nk][nb^4][2*np]
+ * (reference orbital order abcd), the input of dgami_pairing_matvec.
+ * Replaces BubbleGenerator.create_generalized_chi0_q_pp_w0 (bubble_gen.py:268-279). */
+int dgami_chi0_pp(int nk, int nb, int niv_g, int np, const dgami_c128* g_k, dgami_c128* chi0pp, void* stream);
+
 /* Pairing-vertex preparation for the linearised Eliashberg operator:
  *   g_hat = s fft_q Gamma_pp (q mapped irr -> FBZ), g_hat_flip = flip_k flip_v' perm(abcd->adcb) g_hat,
  * stored for the matvec as [nk][(a,b,v)][(c,d,v')] (row (a,b,v) = out, col (c,d,v') = in), n = nb^2*2np.
@@ -261,6 +272,17 @@ int dgami_pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel, co
 int dgami_pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta, const dgami_c128* g_hat,
                          const dgami_c128* g_hat_flip, const dgami_c128* chi0pp, int nvec, const dgami_c128* gap,
                          dgami_c128* out, void* stream);
+
+/* The same operator with the transformed pairing vertex stored in complex64 (half the bytes
+ * of the HBM-bound matvec); vectors, products and sums stay complex128. The vertex is
+ * transformed in complex128 and rounded once when stored.
+ * Replaces the same reference code as dgami_pairing_setup / dgami_pairing_matvec. */
+int dgami_pairing_setup_c64(int nx, int ny, int nz, int nb, int np2, int channel, const dgami_c128* gamma_pp_irr,
+                            const int32_t* irrk_inv, int nq_irr, dgami_c64* g_hat, dgami_c64* g_hat_flip,
+                            void* stream);
+int dgami_pairing_matvec_c64(int nx, int ny, int nz, int nb, int np2, int channel, double beta, const dgami_c64* g_hat,
+                             const dgami_c64* g_hat_flip, const dgami_c128* chi0pp, int nvec, const dgami_c128* gap,
+                             dgami_c128* out, void* stream);
 
 #ifdef __cplusplus
 }

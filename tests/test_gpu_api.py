@@ -169,3 +169,29 @@ def test_auxiliary_chi_and_full_vertex_mirror(api):
     sing, trip = es.create_pairing_vertices_pp(got["dens"], got["magn"], niv_pp)
     ref_s, ref_t = O.pairing_vertices(O.vertex_pp_w0(ref["dens"], niv_pp), O.vertex_pp_w0(ref["magn"], niv_pp))
     assert rel(sing.mat, ref_s) < 1e-10 and rel(trip.mat, ref_t) < 1e-10
+
+
+def test_eliashberg_lanczos_fp32_vertex_storage(api):
+    """The leading eigenvalues with the complex64-stored vertex agree with the complex128 run to 1e-6."""
+    print("[gpu-case] solve_eliashberg_lanczos end_2_end singlet, fp32 vs fp64 vertex storage", flush=True)
+    from dgamore_amd import eliashberg_solver as es
+
+    fp, cfg = api["fp"], api["config"]
+    cfg.sys.beta = 12.5
+    cfg.eliashberg.n_eig = 4
+    cfg.eliashberg.epsilon = 1e-12
+    gam = fp.FourPoint(golden("e2e_gamma_irrq_sing_pp.npy"), fp.SpinChannel.SING, (4, 4, 1), 0, 2, True,
+                       has_compressed_q_dimension=True)
+    pp = O.chi0_pp_w0(golden("e2e_giwk_dga.npy"), 10)
+    chi0pp = fp.FourPoint(pp, fp.SpinChannel.NONE, (4, 4, 1), 0, 1, True, has_compressed_q_dimension=True)
+    v0 = np.ones(16 * 4 * 20, dtype=complex)
+    lam64, _ = es.solve_eliashberg_lanczos(gam, chi0pp, v0=v0)
+    cfg.eliashberg.matvec_precision = "fp32"
+    try:
+        lam32, _ = es.solve_eliashberg_lanczos(gam, chi0pp, v0=v0)
+    finally:
+        cfg.eliashberg.matvec_precision = "fp64"
+    err = np.abs(lam32 - lam64).max() / np.abs(lam64).max()
+    print(f"[gpu-case] leading eigenvalue error fp32 storage vs fp64: {err:.2e}", flush=True)
+    assert err < 1e-6
+

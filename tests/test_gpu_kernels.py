@@ -297,6 +297,42 @@ def test_pairing_matvec_e2e(K, channel):
     assert rel(got, ref) < RTOL
 
 
+def test_chi0_pp(K):
+    say("chi0_pp end_2_end giwk_dga niv_pp=10")
+    giwk = golden("e2e_giwk_dga.npy")
+    ref = O.chi0_pp_w0(giwk, 10)                    # [N, nb, nb, nb, nb, 2np]
+    got = K.chi0_pp(K.to_dev(giwk), 10)
+    torch.cuda.synchronize()
+    assert rel(got.reshape(ref.shape), ref) < 1e-15
+
+
+# complex64 storage of the transformed vertex: every element rounded once (2^-24 relative), sums in
+# complex128, so the product keeps ~1e-7 relative accuracy.
+C64_STORAGE_RTOL = 1e-6
+
+
+@pytest.mark.parametrize("channel", ["sing", "trip"])
+def test_pairing_matvec_c64_storage(K, channel):
+    say(f"pairing matvec complex64 storage end_2_end fixture {channel}")
+    gam = golden(f"e2e_gamma_irrq_{channel}_pp.npy")
+    giwk = golden("e2e_giwk_dga.npy")
+    nk, beta = (4, 4, 1), 12.5
+    maps = O.kgrid_irr_maps(nk)
+    np2 = gam.shape[-1]
+    chi0pp = O.chi0_pp_w0(giwk, np2 // 2)
+    ghat, gflip, s = O.pairing_setup(gam, maps["irrk_inv"], nk, channel)
+    rng = np.random.default_rng(4)
+    gaps = rng.standard_normal((2,) + nk + (2, 2, np2)) + 1j * rng.standard_normal((2,) + nk + (2, 2, np2))
+    chi0pp_k = chi0pp.reshape(nk + (2, 2, 2, 2, np2))
+    ref = np.stack([O.pairing_matvec(ghat, gflip, chi0pp_k, gp, s, beta) for gp in gaps])
+    d_hat, d_flip = K.pairing_setup(K.to_dev(gam), K.to_dev(maps["irrk_inv"], torch.int32), nk, channel,
+                                    torch.complex64)
+    assert d_hat.dtype == torch.complex64
+    got = K.pairing_matvec(d_hat, d_flip, K.to_dev(chi0pp.reshape(16, 16, np2)), K.to_dev(gaps), nk, channel, beta)
+    err = rel(got, ref)
+    assert 1e-12 < err < C64_STORAGE_RTOL  # rounded, but only once
+
+
 # ----------------------------------------------------------------------------- small helpers
 def test_g_full(K):
     from dgamore_amd import synthetic as S
