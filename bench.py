@@ -54,7 +54,68 @@ def parse():
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
     ap.add_argument("--ladder-only", action="store_true",
                     help="after set-up and one warm-up step, run only the batched ladder solve (for counter runs)")
+    ap.add_argument("--eliashberg", action="store_true",
+                    help="instead of the SDE step: the linearised-gap matvec at the 2-orbital 48x48, 2np=128 "
+                         "workload with the vertex stored in complex128 and in complex64")
     return ap.parse_args()
+
+
+def eliashberg_matvec(steps, warmup):
+    """Pairing matvec of the 2-orbital e_g model on a 48x48 k-grid (325 irreducible q), 2np = 128
+    (n = nb^2 2np = 512, 2304 lattice vectors): set-up once, then `steps` applications of one and
+    of four gap vectors, the vertex stored in complex128 and in complex64. HBM roofline: every
+    application reads both transformed matrices once (2 N n^2 element sizes)."""
+    import numpy as np
+    import torch
+
+    from dgamore_amd import kernels as K
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.brillouin_zone import KGrid
+
+    np2 = 128
+    m = S.make_model(4, with_vertices=False)
+    grid = KGrid(m.nk)
+    N, nb, nq = int(np.prod(m.nk)), m.nb, grid.nk_irr
+    n = nb * nb * np2
+    dev = torch.device("cuda")
+    gen = torch.Generator(device=dev).manual_seed(4004)
+    # synthetic pairing vertex on the irreducible q, decaying in both fermionic frequencies
+    v = torch.arange(np2, device=dev, dtype=torch.float64) - np2 / 2 + 0.5
+    decay = 1.0 / (1.0 + (v[:, None] ** 2 + v[None, :] ** 2) / 400.0)
+    gam = torch.randn((nq,) + (nb,) * 4 + (np2, np2), dtype=torch.complex128, device=dev, generator=gen) * decay
+    g = K.g_full(K.to_dev(m.ek.reshape(N, nb, nb)), K.to_dev(m.sigma_loc[None]), m.mu, m.beta, np2 // 2 + 8)
+    chi0pp = K.chi0_pp(g, np2 // 2)
+    irrk_inv = K.to_dev(grid.irrk_inv, torch.int32)
+    x = torch.randn((4, N, nb, nb, np2), dtype=torch.complex128, device=dev, generator=gen)
+    out = {}
+    results = {}
+    for name, storage in (("c128", torch.complex128), ("c64", torch.complex64)):
+        g_hat, g_flip = K.pairing_setup(gam, irrk_inv, m.nk, "sing", storage)
+        es = torch.finfo(torch.float64 if storage == torch.complex128 else torch.float32).bits // 4
+        for nvec in (1, 4):
+            xv = x[:nvec].reshape((nvec,) + tuple(m.nk) + (nb, nb, np2)).contiguous()
+            for _ in range(warmup):
+                y = K.pairing_matvec(g_hat, g_flip, chi0pp, xv, m.nk, "sing", m.beta)
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for _ in range(steps):
+                y = K.pairing_matvec(g_hat, g_flip, chi0pp, xv, m.nk, "sing", m.beta)
+            e1.record()
+            torch.cuda.synchronize()
+            ms = e0.elapsed_time(e1) / steps
+            mat_bytes = 2.0 * N * n * n * es
+            results[f"{name}_nvec{nvec}"] = {"ms": ms, "matrix_GB": mat_bytes / 1e9,
+                                             "achieved_GBps": mat_bytes / (ms * 1e-3) / 1e9,
+                                             "frac_hbm_peak": mat_bytes / (ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+            out[(name, nvec)] = y
+        del g_hat, g_flip
+        torch.cuda.empty_cache()
+    a, b = out[("c64", 4)], out[("c128", 4)]
+    err = float((a - b).abs().max() / b.abs().max())
+    print(json.dumps({"workload": f"eliashberg matvec: 2 orbitals, k-grid 48x48x1 ({nq} irreducible q), 2np={np2}, "
+                                  f"n={n}, {N} lattice vectors", "results": results,
+                      "rel_err_c64_vs_c128": err}), flush=True)
 
 
 def copy_calibration():
@@ -231,6 +292,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.eliashberg:
+        eliashberg_matvec(args.steps, args.warmup)
         return
     import torch
 

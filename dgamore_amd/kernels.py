@@ -102,6 +102,15 @@ def fft_grid(x, nk, sign, scale=1.0, out=None):
     return out
 
 
+def g_full(ek, sigma, mu, beta, niv):
+    """G(k, v) [N,nb,nb,2 niv] from e_k [N,nb,nb] and Sigma [1 or N][nb][nb][2 niv_sigma] (centred cut)."""
+    N, nb = ek.shape[0], ek.shape[-1]
+    g = torch.empty((N, nb, nb, 2 * niv), dtype=C128, device=ek.device)
+    _lib.call("dgami_g_full", N, nb, int(niv), float(beta), float(mu), _p(ek), _p(sigma), sigma.shape[0],
+              sigma.shape[-1] // 2, _p(g), _stream())
+    return g
+
+
 def bubble_irrq(giwk, nk, niw, niv, beta, irrk_ind):
     """giwk: REF2 [nx,ny,nz,nb,nb,2niv_g]; irrk_ind: int32 [nq]. Returns chi0 QF [niw+1][nb^4][2niv][nq]."""
     nx, ny, nz = nk

@@ -68,12 +68,7 @@ class SDEPipeline:
     # -- pieces -------------------------------------------------------------------------------
     def g_full(self, sigma, mu, niv):
         """sigma: [1 or N][nb][nb][2 niv_sigma] device tensor."""
-        nb = self.m.nb
-        nks = sigma.shape[0]
-        g = torch.empty((self.N, nb, nb, 2 * niv), dtype=K.C128, device=sigma.device)
-        _lib.call("dgami_g_full", self.N, nb, niv, float(self.m.beta), float(mu), K._p(self.ek), K._p(sigma), nks,
-                  sigma.shape[-1] // 2, K._p(g), K._stream())
-        return g
+        return K.g_full(self.ek, sigma, mu, self.m.beta, niv)
 
     def update_occupation(self, sigma, mu):
         """n, occ and occ_k of G(sigma, mu) with the k-dependent model (greens_function.py:212-240)."""
