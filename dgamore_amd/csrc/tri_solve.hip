@@ -65,6 +65,22 @@ __global__ void __launch_bounds__(64) tri_diag_inverse_kernel(TriFactors<S> f, i
   }
 }
 
+// Two consecutive factor elements (the first at an even column of a block column: 16-byte
+// aligned for complex64, since ld is a multiple of 4).
+template <typename S>
+__device__ __forceinline__ void load_pair(const S* src, S& a, S& b);
+template <>
+__device__ __forceinline__ void load_pair<float2>(const float2* src, float2& a, float2& b) {
+  const float4 x = *reinterpret_cast<const float4*>(src);
+  a = make_float2(x.x, x.y);
+  b = make_float2(x.z, x.w);
+}
+template <>
+__device__ __forceinline__ void load_pair<double2>(const double2* src, double2& a, double2& b) {
+  a = src[0];
+  b = src[1];
+}
+
 // ---------------------------------------------------------------------------
 // One whole sweep per problem: grid (batch), 256 threads. Thread t keeps rows t + 256 m
 // (m < RPT) of the right-hand sides in registers. For every block b (forward: first to
@@ -124,10 +140,24 @@ __global__ void __launch_bounds__(256) tri_sweep_kernel(TriFactors<S> f, TriVec<
         for (int t = 0; t < T; ++t) yr[m][t] = s_x[r - c0][t];
       } else if (r < n && (LOWER ? r >= c0 + cb : r < c0)) {
         const S* frow = F + static_cast<size_t>(r) * f.ld + c0;
-        for (int k = 0; k < cb; ++k) {
-          const S l = frow[k];
+        if (cb == TRI_B) {
+          // full block: unrolled in pairs of 16-byte-aligned elements so that loads run ahead
+#pragma unroll 4
+          for (int k = 0; k < TRI_B; k += 2) {
+            S l0, l1;
+            load_pair<S>(frow + k, l0, l1);
 #pragma unroll
-          for (int t = 0; t < T; ++t) yr[m][t] = cfms(yr[m][t], l, s_x[k][t]);
+            for (int t = 0; t < T; ++t) {
+              yr[m][t] = cfms(yr[m][t], l0, s_x[k][t]);
+              yr[m][t] = cfms(yr[m][t], l1, s_x[k + 1][t]);
+            }
+          }
+        } else {
+          for (int k = 0; k < cb; ++k) {
+            const S l = frow[k];
+#pragma unroll
+            for (int t = 0; t < T; ++t) yr[m][t] = cfms(yr[m][t], l, s_x[k][t]);
+          }
         }
       }
     }
