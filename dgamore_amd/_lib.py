@@ -57,6 +57,14 @@ SIGNATURES = {
     "dgami_pairing_matvec_c64": ([_i, _i, _i, _i, _i, _i, _d, _c, _c, _c, _i, _c, _c, _c], _i),
 }
 
+class LadderModel(ctypes.Structure):
+    """dgami_ladder_model (include/dgami.h)."""
+    _fields_ = [("nchan", _i), ("nq", _i), ("nb", _i), ("nw", _i), ("niv", _i), ("beta", _d),
+                ("chi0_inv", _c), ("gamma", _c), ("u_r", _c), ("v_r", _c), ("chi_aux_sum", _c), ("info", _c)]
+
+
+SIGNATURES["dgami_ladder_solve_sum_multi"] = ([_i, _c, _c], _i)
+
 _lib = None
 
 

@@ -1,7 +1,9 @@
 // Entry point of the batched ladder solves: picks the register-resident
 // kernel for small compound sizes and the blocked workspace LU otherwise.
+#include <algorithm>
 #include <cstdlib>
 #include <string>
+#include <vector>
 
 #include "dgami.h"
 #include "dgami_internal.hpp"
@@ -15,6 +17,7 @@ int ladder_reg_launch(int variant, const LadderArgs& a, hipStream_t stream);
 int ladder_col_variant(int n, int nrhs);
 int ladder_col_launch(int variant, const LadderArgs& a, hipStream_t stream);
 int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream);
+int ladder_reg_launch_multi(int variant, const LadderMulti& mm, hipStream_t stream);
 
 // Kernel family of the plain FP64 solve: "reg" (2D register tiles) up to compound size
 // 64, the blocked workspace LU above it. Measured on 8192 matrices per call, the register
@@ -39,6 +42,61 @@ int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
 }
 
 }  // namespace dgami
+
+extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_model* models, void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(nmodels >= 0 && (nmodels == 0 || models), "ladder_multi: null model table");
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  std::vector<LadderArgs> small;
+  int rc = 0;
+  for (int m = 0; m < nmodels; ++m) {
+    const dgami_ladder_model& md = models[m];
+    DGAMI_ARG_CHECK(md.chi0_inv && md.gamma && md.u_r && md.chi_aux_sum, "ladder_multi: null pointer in a model");
+    DGAMI_ARG_CHECK(md.nchan >= 1 && md.nq >= 1 && md.nb >= 1 && md.nb <= 4 && md.nw >= 1 && md.niv >= 1 &&
+                        md.beta > 0,
+                    "ladder_multi: invalid sizes in a model");
+    LadderArgs a{};
+    a.chi0_inv = reinterpret_cast<const double2*>(md.chi0_inv);
+    a.gamma = reinterpret_cast<const double2*>(md.gamma);
+    a.u_r = reinterpret_cast<const double2*>(md.u_r);
+    a.v_r = reinterpret_cast<const double2*>(md.v_r);
+    a.out = reinterpret_cast<double2*>(md.chi_aux_sum);
+    a.info = md.info;
+    a.nchan = md.nchan;
+    a.nq = md.nq;
+    a.nw = md.nw;
+    a.T = md.nb * md.nb;
+    a.nv = 2 * md.niv;
+    a.n = a.T * a.nv;
+    a.inv_beta2 = 1.0 / (md.beta * md.beta);
+    a.inv_beta = 1.0 / md.beta;
+    a.dense = 0;
+    const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
+    DGAMI_ARG_CHECK(items < (size_t(1) << 30), "ladder_multi: too many problems in a model");
+    if (a.n <= LADDER_REG_MAX_N && ladder_reg_variant(a.n, a.T) >= 0)
+      small.push_back(a);
+    else if ((rc = ladder_solve_fp64(a, s)) != 0)
+      return rc;
+  }
+  // the register-resident models, LADDER_MULTI_MAX per launch, in the variant of the largest one
+  for (size_t g0 = 0; g0 < small.size(); g0 += LADDER_MULTI_MAX) {
+    LadderMulti mm{};
+    mm.nmodels = static_cast<int>(std::min(small.size() - g0, static_cast<size_t>(LADDER_MULTI_MAX)));
+    int variant = 0;
+    size_t total = 0;
+    for (int m = 0; m < mm.nmodels; ++m) {
+      const LadderArgs& a = small[g0 + m];
+      mm.model[m] = a;
+      mm.start[m] = static_cast<int>(total);
+      total += static_cast<size_t>(a.nchan) * a.nw * a.nq;
+      variant = std::max(variant, ladder_reg_variant(a.n, a.T));
+    }
+    DGAMI_ARG_CHECK(total < (size_t(1) << 31), "ladder_multi: too many problems in one launch");
+    mm.start[mm.nmodels] = static_cast<int>(total);
+    if ((rc = ladder_reg_launch_multi(variant, mm, s)) != 0) return rc;
+  }
+  return 0;
+}
 
 extern "C" int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double beta,
                                       const dgami_c128* chi0_inv, const dgami_c128* gamma, const dgami_c128* u_r,

@@ -74,6 +74,14 @@ __device__ __forceinline__ const double2* ladder_matrix_ptr(const LadderArgs& a,
   return a.gamma + (static_cast<size_t>(chan) * a.nw + w) * nn;
 }
 
+// Up to LADDER_MULTI_MAX models solved by one launch of the register-resident kernel.
+constexpr int LADDER_MULTI_MAX = 16;
+struct LadderMulti {
+  LadderArgs model[LADDER_MULTI_MAX];
+  int start[LADDER_MULTI_MAX + 1];  // first workgroup of each model; start[nmodels] = total
+  int nmodels;
+};
+
 int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream);
 int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream);
 

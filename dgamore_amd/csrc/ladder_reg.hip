@@ -33,8 +33,9 @@ __device__ __forceinline__ double readlane_d(double v, int lane) {
   return __longlong_as_double((static_cast<long long>(hi) << 32) | static_cast<unsigned int>(lo));
 }
 
+// Problem `item` = (chan, w, q) of model `a`, solved by the calling workgroup.
 template <int TR, int TC, int RPT, int CPT>
-__global__ void __launch_bounds__(TR* TC) ladder_reg_kernel(LadderArgs a) {
+__device__ __forceinline__ void ladder_reg_solve(const LadderArgs& a, int item) {
   constexpr int NT = TR * TC;
   constexpr int NWAVE = NT / 64;
   static_assert(64 % TC == 0, "TC must divide the wavefront");
@@ -42,8 +43,8 @@ __global__ void __launch_bounds__(TR* TC) ladder_reg_kernel(LadderArgs a) {
   const int tc = tid % TC, tr = tid / TC;
   const int lane = tid & 63, wave = tid >> 6;
 
-  const int q = blockIdx.x % a.nq;
-  const int rest = blockIdx.x / a.nq;
+  const int q = item % a.nq;
+  const int rest = item / a.nq;
   const int w = rest % a.nw;
   const int chan = rest / a.nw;
   const int n = a.n, T = a.T, ncol = n + T;
@@ -274,13 +275,32 @@ __global__ void __launch_bounds__(TR* TC) ladder_reg_kernel(LadderArgs a) {
   if (tid == 0 && a.info) a.info[(static_cast<size_t>(chan) * a.nw + w) * a.nq + q] = s_info;
 }
 
+template <int TR, int TC, int RPT, int CPT>
+__global__ void __launch_bounds__(TR* TC) ladder_reg_kernel(LadderArgs a) {
+  ladder_reg_solve<TR, TC, RPT, CPT>(a, blockIdx.x);
+}
+
+// Several models in one launch: workgroup b solves item b - start[m] of the model m with
+// start[m] <= b < start[m + 1] (the table travels in the kernel arguments, so a captured
+// graph keeps it).
+template <int TR, int TC, int RPT, int CPT>
+__global__ void __launch_bounds__(TR* TC) ladder_reg_multi_kernel(LadderMulti mm) {
+  const int b = blockIdx.x;
+  int m = 0;
+  while (m + 1 < mm.nmodels && b >= mm.start[m + 1]) ++m;
+  ladder_reg_solve<TR, TC, RPT, CPT>(mm.model[m], b - mm.start[m]);
+}
+
 struct RegVariant {
   int max_rows, max_cols, threads;
   void (*kernel)(LadderArgs);
+  void (*multi)(LadderMulti);
 };
 
-#define DGAMI_REG_VARIANT(TR, TC, RPT, CPT) \
-  RegVariant { TR * RPT, TC * CPT, TR * TC, ladder_reg_kernel<TR, TC, RPT, CPT> }
+#define DGAMI_REG_VARIANT(TR, TC, RPT, CPT)                                            \
+  RegVariant {                                                                         \
+    TR * RPT, TC * CPT, TR * TC, ladder_reg_kernel<TR, TC, RPT, CPT>, ladder_reg_multi_kernel<TR, TC, RPT, CPT> \
+  }
 
 static const RegVariant kRegVariants[] = {
     DGAMI_REG_VARIANT(16, 16, 3, 4),   // n <= 48,  n + nb^2 <= 64
@@ -294,6 +314,21 @@ int ladder_reg_variant(int n, int nrhs) {
   for (int v = 0; v < static_cast<int>(sizeof(kRegVariants) / sizeof(kRegVariants[0])); ++v)
     if (n <= kRegVariants[v].max_rows && n + nrhs <= kRegVariants[v].max_cols) return v;
   return -1;
+}
+
+int ladder_reg_launch_multi(int variant, const LadderMulti& mm, hipStream_t stream) {
+  const RegVariant& rv = kRegVariants[variant];
+  const int items = mm.start[mm.nmodels];
+  if (items == 0) return 0;
+  size_t lds = 0;
+  for (int m = 0; m < mm.nmodels; ++m) {
+    const size_t l = static_cast<size_t>(mm.model[m].n) * (mm.model[m].T + LADDER_CB) * sizeof(double2);
+    if (l > lds) lds = l;
+  }
+  allow_lds(rv.multi, lds);
+  hipLaunchKernelGGL(rv.multi, dim3(static_cast<unsigned>(items)), dim3(rv.threads), lds, stream, mm);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
 }
 
 int ladder_reg_launch(int variant, const LadderArgs& a, hipStream_t stream) {

@@ -157,6 +157,34 @@ def ladder_solve_sum(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, precision=_lib.PLA
     return out, info
 
 
+def ladder_solve_sum_multi(models, check_info=True):
+    """Plain-FP64 ladder solves of several models in one call (dgami_ladder_solve_sum_multi).
+
+    models: list of (chi0_inv QF, gamma_cmp [nchan][nw][n][n], u_r_m [nchan][nb^4], v_r_m or None, beta).
+    Returns a list of (chi_aux_sum [nchan] x QF, info [nchan][nw][nq]) per model.
+    """
+    outs, table = [], (_lib.LadderModel * max(1, len(models)))()
+    for m, (chi0_inv, gamma_cmp, u_r_m, v_r_m, beta) in enumerate(models):
+        nw, nb4, nv, nq = chi0_inv.shape
+        nb = int(round(nb4 ** 0.25))
+        nchan = gamma_cmp.shape[0]
+        n = nb * nb * nv
+        if tuple(gamma_cmp.shape[1:]) != (nw, n, n):
+            raise ValueError(f"model {m}: gamma must be [nchan,{nw},{n},{n}], got {tuple(gamma_cmp.shape)}")
+        out = torch.empty((nchan, nw, nb4, nv, nq), dtype=C128, device=chi0_inv.device)
+        info = torch.zeros((nchan, nw, nq), dtype=torch.int32, device=chi0_inv.device)
+        table[m] = _lib.LadderModel(nchan, nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp), _p(u_r_m),
+                                    _p(v_r_m), _p(out), _p(info, torch.int32))
+        outs.append((out, info))
+    _lib.call("dgami_ladder_solve_sum_multi", len(models), ctypes.addressof(table), _stream())
+    if check_info:
+        for out, info in outs:
+            bad = torch.nonzero(info).cpu().numpy()
+            if len(bad):
+                raise np.linalg.LinAlgError(f"singular ladder matrix at (chan, w, q) = {bad[:8].tolist()}")
+    return outs
+
+
 def _raise_singular(info, what):
     bad = torch.nonzero(info).cpu().numpy()
     if len(bad):

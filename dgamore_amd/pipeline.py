@@ -114,6 +114,32 @@ class SDEPipeline:
                              chi_phys=chis, kernel=kern, sigma_nl=sig, hartree=hart, fock=fock, sigma=new)
         return new
 
+    # -- HIP graph of one step ------------------------------------------------------------------
+    def capture(self, mu=None):
+        """Capture one plain-FP64 SDE step as a HIP graph (torch.cuda.CUDAGraph). The captured step
+        reads Sigma from the fixed buffer `graph_sigma` [N,nb,nb,2 niv_s] and writes the new, mixed
+        Sigma to `graph_out`; replay() runs it again without host work. mu, the occupations of the
+        Hartree-Fock terms and the model are fixed at capture time. The mixed forms read one float
+        per refinement step on the host and are not capturable."""
+        if self.precision != _lib.PLAIN_FP64:
+            raise ValueError("only the plain FP64 step can be captured as a graph")
+        mu = self.m.mu if mu is None else mu
+        nb = self.m.nb
+        self.graph_sigma = self.sigma_dmft.expand(self.N, nb, nb, 2 * self.niv_s).contiguous().clone()
+        # warm-up outside the capture: library workspaces and occupations are allocated here
+        self.step(self.graph_sigma, mu)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.graph_out = self.step(self.graph_sigma, mu)
+        torch.cuda.synchronize()
+        return self.graph
+
+    def replay(self):
+        """Run the captured step on the current contents of graph_sigma; returns graph_out."""
+        self.graph.replay()
+        return self.graph_out
+
     def iterate(self, n_iter, sigma=None, mu=None, n_target=None):
         """Self-consistency loop: SDE step, mu update to the target filling, new occupations
         (nonlocal_sde.py:777-914). Returns (sigma [N,nb,nb,2 niv_s], mu history)."""

@@ -115,6 +115,29 @@ int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double be
                            const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r, int precision,
                            dgami_c128* chi_aux_sum, int32_t* info, void* stream);
 
+/* One model of dgami_ladder_solve_sum_multi: the arguments of dgami_ladder_solve_sum (plain FP64). */
+typedef struct {
+  int nchan, nq, nb, nw, niv;
+  double beta;
+  const dgami_c128* chi0_inv;
+  const dgami_c128* gamma;
+  const dgami_c128* u_r;
+  const dgami_c128* v_r;
+  dgami_c128* chi_aux_sum;
+  int32_t* info;
+} dgami_ladder_model;
+
+/* Plain-FP64 ladder solves of several independent models (different orbital counts, frequency
+ * boxes and momentum grids) in one call. `models` is a HOST array of nmodels descriptors whose
+ * device pointers follow dgami_ladder_solve_sum. The models whose compound size takes the
+ * register-resident kernel are solved together, up to 16 models per launch: one workgroup per
+ * (model, chan, w, q) of the concatenated work list, the model table in the kernel arguments
+ * (so a captured HIP graph keeps it). The larger models go through the blocked LU, one call
+ * each. Outputs and info per model as in dgami_ladder_solve_sum. No counterpart in the
+ * reference, which loops over its models one run at a time; it replaces that loop for a
+ * batch of models. */
+int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_model* models, void* stream);
+
 /* Dense-matrix variant of the ladder solve: a_cmp holds nmat assembled compound matrices
  * [nmat][n][n] (n = nb^2 * 2*niv); out[nmat][n][nb^2] = (1/beta) sum_v' [A^-1]_{(o1,o2,v),(o4,o3,v')}
  * at row (o1,o2,v), column (o4,o3). info: [nmat].

@@ -139,6 +139,29 @@ def test_ladder_plain_fp64_large(K, nb, niv, niw, nq):
     assert rel(got[0], ref) < RTOL
 
 
+def test_ladder_multi_models(K):
+    """Five models of different orbital counts, boxes and grids in one call: four share one
+    register-kernel launch, the n = 120 one goes through the blocked LU."""
+    say("ladder_solve_sum_multi: n = 16, 30, 64, 54 (one launch) and 120 (blocked)")
+    cases = [(1, 8, 2, 3), (1, 15, 1, 4), (2, 8, 2, 2), (3, 3, 1, 2), (2, 15, 1, 2)]
+    models, refs = [], []
+    for idx, (nb, niv, niw, nq) in enumerate(cases):
+        chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=50 + idx)
+        rng = np.random.default_rng(idx)
+        gamma2 = gamma + 0.5 * rng.standard_normal(gamma.shape)
+        refs.append([O.ladder_sum_solve(chi0_inv, g, u, v, beta) for g in (gamma, gamma2)])
+        models.append((K.ref4_to_qf(K.to_dev(chi0_inv)),
+                       torch.stack([K.vertex_to_cmp(K.to_dev(g)) for g in (gamma, gamma2)]).contiguous(),
+                       torch.stack([K.orb_to_m(K.to_dev(u))] * 2).contiguous(),
+                       torch.stack([K.orb_to_m(K.to_dev(v))] * 2).contiguous(), beta))
+    outs = K.ladder_solve_sum_multi(models)
+    torch.cuda.synchronize()
+    for (nb, *_), (out, info), ref in zip(cases, outs, refs):
+        assert int(info.abs().sum()) == 0
+        for c in range(2):
+            assert rel(K.qf_to_ref4(out[c], nb), ref[c]) < RTOL
+
+
 # The mixed form (complex64 LU, FP64 iterative refinement) must agree with the FP64 solve
 # to 1e-8 relative error, the accuracy stated for it.
 MIXED_RTOL = 1e-8
