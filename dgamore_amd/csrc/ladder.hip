@@ -9,6 +9,7 @@
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
 #include "ladder_common.hpp"
+#include "ladder_tune.hpp"
 
 namespace dgami {
 
@@ -16,16 +17,25 @@ int ladder_reg_variant(int n, int nrhs);
 int ladder_reg_launch(int variant, const LadderArgs& a, hipStream_t stream);
 int ladder_col_variant(int n, int nrhs);
 int ladder_col_launch(int variant, const LadderArgs& a, hipStream_t stream);
-int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream);
+int ladder_blocked_fp64(const LadderArgs& a, int variant, hipStream_t stream);
 int ladder_reg_launch_multi(int variant, const LadderMulti& mm, hipStream_t stream);
 
-// Kernel family of the plain FP64 solve: "reg" (2D register tiles) up to compound size
-// 64, the blocked workspace LU above it. Measured on 8192 matrices per call, the register
-// kernel leads at n = 40 (0.96 vs 1.85 ms) and ties at n = 64, while the blocked LU is
-// 1.7-2.7x faster from n = 72 on. The column-cyclic kernel ("col", n <= 128) is kept as an
-// alternative. The environment variable DGAMI_LADDER_KERNEL (col, reg or blocked) forces
-// one family for comparisons.
+// Kernel of the plain FP64 solve. The environment variable DGAMI_LADDER_KERNEL (col, reg or
+// blocked) forces one family for comparisons; otherwise an autotuned choice for (n, nrhs)
+// wins; otherwise the size rule: "reg" (2D register tiles) up to compound size 64, the
+// blocked workspace LU above it. Measured on 8192 matrices per call, the register kernel
+// leads at n = 40 (0.96 vs 1.85 ms) and ties at n = 64, while the blocked LU is 1.7-2.7x
+// faster from n = 72 on. The column-cyclic kernel ("col", n <= 128) is the alternative.
 constexpr int LADDER_REG_MAX_N = 64;
+
+// Register variant a plain FP64 solve of (n, nrhs) runs in, or -1 when it takes another family.
+static int fp64_reg_choice(int n, int nrhs) {
+  const char* force = getenv("DGAMI_LADDER_KERNEL");
+  if (force && *force) return std::string(force) == "reg" ? ladder_reg_variant(n, nrhs) : -1;
+  const int tuned = ladder_tuned_kernel(n, nrhs, DGAMI_PLAIN_FP64);
+  if (tuned >= 0) return ladder_kernel_family(tuned) == LADDER_FAM_REG ? ladder_kernel_variant(tuned) : -1;
+  return n <= LADDER_REG_MAX_N ? ladder_reg_variant(n, nrhs) : -1;
+}
 
 int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
   const char* force = getenv("DGAMI_LADDER_KERNEL");
@@ -34,11 +44,17 @@ int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
     const int v = ladder_col_variant(a.n, a.T);
     if (v >= 0) return ladder_col_launch(v, a, stream);
   }
-  if (f == "reg" || (f.empty() && a.n <= LADDER_REG_MAX_N)) {
-    const int v = ladder_reg_variant(a.n, a.T);
-    if (v >= 0) return ladder_reg_launch(v, a, stream);
+  if (f.empty()) {
+    const int tuned = ladder_tuned_kernel(a.n, a.T, DGAMI_PLAIN_FP64);
+    if (tuned >= 0) {
+      const int fam = ladder_kernel_family(tuned), v = ladder_kernel_variant(tuned);
+      if (fam == LADDER_FAM_COL && ladder_col_variant_fits(v, a.n, a.T)) return ladder_col_launch(v, a, stream);
+      if (fam == LADDER_FAM_BLOCKED) return ladder_blocked_fp64(a, v, stream);
+    }
   }
-  return ladder_blocked_fp64(a, stream);
+  const int v = fp64_reg_choice(a.n, a.T);
+  if (v >= 0) return ladder_reg_launch(v, a, stream);
+  return ladder_blocked_fp64(a, -1, stream);
 }
 
 }  // namespace dgami
@@ -73,12 +89,12 @@ extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_mode
     a.dense = 0;
     const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
     DGAMI_ARG_CHECK(items < (size_t(1) << 30), "ladder_multi: too many problems in a model");
-    if (a.n <= LADDER_REG_MAX_N && ladder_reg_variant(a.n, a.T) >= 0)
+    if (fp64_reg_choice(a.n, a.T) >= 0)
       small.push_back(a);
     else if ((rc = ladder_solve_fp64(a, s)) != 0)
       return rc;
   }
-  // the register-resident models, LADDER_MULTI_MAX per launch, in the variant of the largest one
+  // the register-resident models, LADDER_MULTI_MAX per launch, in the largest variant of the group
   for (size_t g0 = 0; g0 < small.size(); g0 += LADDER_MULTI_MAX) {
     LadderMulti mm{};
     mm.nmodels = static_cast<int>(std::min(small.size() - g0, static_cast<size_t>(LADDER_MULTI_MAX)));
@@ -89,7 +105,7 @@ extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_mode
       mm.model[m] = a;
       mm.start[m] = static_cast<int>(total);
       total += static_cast<size_t>(a.nchan) * a.nw * a.nq;
-      variant = std::max(variant, ladder_reg_variant(a.n, a.T));
+      variant = std::max(variant, fp64_reg_choice(a.n, a.T));
     }
     DGAMI_ARG_CHECK(total < (size_t(1) << 31), "ladder_multi: too many problems in one launch");
     mm.start[mm.nmodels] = static_cast<int>(total);

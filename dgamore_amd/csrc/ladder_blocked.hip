@@ -21,6 +21,7 @@
 #include "dgami_launch.hpp"
 #include "ladder_blocked.hpp"
 #include "ladder_common.hpp"
+#include "ladder_tune.hpp"
 #include "tri_solve.hpp"
 
 #include <cstdlib>
@@ -788,14 +789,16 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
 
 template <>
 int blocked_factor<double2>(const LadderArgs& a, BlockedState<double2>& st, int item0, int nb, hipStream_t stream) {
-  if (st.n <= 768) return blocked_factor_nb<double2, 32, 1, 768, 2>(a, st, item0, nb, stream);
+  const bool nb32 = blocked_variant_fits(false, st.variant, st.n) ? st.variant == 0 : st.n <= 768;
+  if (nb32) return blocked_factor_nb<double2, 32, 1, 768, 2>(a, st, item0, nb, stream);
   return blocked_factor_nb<double2, 16, 3, 512, 2>(a, st, item0, nb, stream);
 }
 
 template <>
 int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int item0, int nb, hipStream_t stream) {
   // three rows per thread: 256-thread panels for n <= 768, two problems per CU
-  if (st.n <= 1152) return blocked_factor_nb<float2, 32, 3, 384, 4>(a, st, item0, nb, stream);
+  const bool nb32 = blocked_variant_fits(true, st.variant, st.n) ? st.variant == 0 : st.n <= 1152;
+  if (nb32) return blocked_factor_nb<float2, 32, 3, 384, 4>(a, st, item0, nb, stream);
   return blocked_factor_nb<float2, 16, 3, 512, 4>(a, st, item0, nb, stream);
 }
 
@@ -806,7 +809,7 @@ int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0,
   return k.gemm(r0, r1, c0, c1, k0, kw);
 }
 
-int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream) {
+int ladder_blocked_fp64(const LadderArgs& a, int variant, hipStream_t stream) {
   const int n = a.n, ncol = n + a.T;
   const int ld = (ncol + 3) & ~3;
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
@@ -819,6 +822,7 @@ int ladder_blocked_fp64(const LadderArgs& a, hipStream_t stream) {
   if (batch > 65535) batch = 65535;
   BlockedState<double2> st;
   if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
+  st.variant = variant;
   const int nblk = static_cast<int>(ceil_div(n, TRI_B));
   double2* tri = static_cast<double2*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(double2)));
   if (!tri) return -2;

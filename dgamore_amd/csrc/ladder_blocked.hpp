@@ -24,6 +24,7 @@ struct BlockedState {
   int n, ld, ncol;
   int swap_left;   // also apply interchanges left of the panel (needed when L is reused)
   int gemm_mode;   // BlockedGemm (complex64 only)
+  int variant;     // blocked-LU variant (panel width and rows per thread), -1: chosen by n
 };
 
 // Allocate the workspace of `batch` problems in slots ws and ws+1; returns false on failure.
@@ -43,6 +44,7 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
   st.ncol = ncol;
   st.swap_left = 0;
   st.gemm_mode = BLK_GEMM_F32_MFMA;
+  st.variant = -1;
   return true;
 }
 

@@ -20,6 +20,7 @@
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
 #include "ladder_common.hpp"
+#include "ladder_tune.hpp"
 
 namespace dgami {
 
@@ -250,6 +251,14 @@ static const ColVariant kColVariants[] = {
     DGAMI_COL_VARIANT(8, 2, 12),  // n <= 128, n + nb^2 <= 96
     DGAMI_COL_VARIANT(8, 2, 16),  // n <= 128, n + nb^2 <= 128
 };
+
+int ladder_col_variant_count() { return static_cast<int>(sizeof(kColVariants) / sizeof(kColVariants[0])); }
+
+bool ladder_col_variant_fits(int v, int n, int nrhs) {
+  if (v < 0 || v >= ladder_col_variant_count()) return false;
+  const size_t lds = (static_cast<size_t>(n) * (n + 1) / 2 + static_cast<size_t>(n) * nrhs) * sizeof(double2);
+  return n <= kColVariants[v].max_rows && n + nrhs <= kColVariants[v].max_cols && lds <= 140 * 1024;
+}
 
 int ladder_col_variant(int n, int nrhs) {
   for (int v = 0; v < static_cast<int>(sizeof(kColVariants) / sizeof(kColVariants[0])); ++v) {

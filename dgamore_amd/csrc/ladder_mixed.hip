@@ -22,6 +22,7 @@
 #include "dgami_launch.hpp"
 #include "ladder_blocked.hpp"
 #include "ladder_common.hpp"
+#include "ladder_tune.hpp"
 #include "tri_solve.hpp"
 #include "zgemm.hpp"
 
@@ -215,6 +216,8 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   if (!blocked_alloc(st, n, ld, n, bitems, WS_MIXED)) return -2;
   st.swap_left = 1;
   st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
+  const int tuned = ladder_tuned_kernel(n, T, precision);
+  if (tuned >= 0 && ladder_kernel_family(tuned) == LADDER_FAM_BLOCKED) st.variant = ladder_kernel_variant(tuned);
   char* vec = static_cast<char*>(workspace(WS_MIXED_VEC, 2 * bitems * n * T * sizeof(double2) + bitems * n * sizeof(int) +
                                                              (2 * bitems + 4) * sizeof(float)));
   if (!vec) return -2;

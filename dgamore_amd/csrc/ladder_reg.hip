@@ -21,6 +21,7 @@
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
 #include "ladder_common.hpp"
+#include "ladder_tune.hpp"
 
 namespace dgami {
 
@@ -308,6 +309,14 @@ static const RegVariant kRegVariants[] = {
     DGAMI_REG_VARIANT(32, 16, 3, 6),   // n <= 96,  n + nb^2 <= 96
     DGAMI_REG_VARIANT(32, 16, 4, 8),   // n <= 128, n + nb^2 <= 128
 };
+
+int ladder_reg_variant_count() { return static_cast<int>(sizeof(kRegVariants) / sizeof(kRegVariants[0])); }
+
+bool ladder_reg_variant_fits(int v, int n, int nrhs) {
+  if (v < 0 || v >= ladder_reg_variant_count()) return false;
+  const size_t lds = static_cast<size_t>(n) * (nrhs + LADDER_CB) * sizeof(double2);
+  return n <= kRegVariants[v].max_rows && n + nrhs <= kRegVariants[v].max_cols && lds <= 160 * 1024;
+}
 
 // Index of the register variant for (n, nrhs), or -1 when n is too large.
 int ladder_reg_variant(int n, int nrhs) {

@@ -85,9 +85,17 @@ class SDEPipeline:
         """One non-local SDE iteration from Sigma to the new, mixed Sigma [N,nb,nb,2 niv_s] on the device
         (the DMFT self-energy outside the core window). sigma, sigma_old: [1 or N][nb][nb][2 niv_s];
         sigma_old defaults to sigma."""
+        sigma = self.sigma_dmft if sigma is None else sigma
+        front = self.front(sigma, mu)
+        aux, info = K.ladder_solve_sum(front["chi0_inv"], self.gamma, self.u_r, None, self.m.beta, self.precision,
+                                       check_info=False)
+        return self.back(front, aux, info, sigma, sigma_old, keep)
+
+    def front(self, sigma, mu=None):
+        """G(k), the bubble, chi0^-1 with its sums and the double-counting kernel: everything the
+        ladder solves need (a dict of device tensors)."""
         m = self.m
         nb, beta = m.nb, m.beta
-        sigma = self.sigma_dmft if sigma is None else sigma
         mu = m.mu if mu is None else mu
         if self.occ is None:
             self.update_occupation(sigma, mu)
@@ -95,7 +103,19 @@ class SDEPipeline:
         chi0 = K.bubble_irrq(g, m.nk, m.niw, m.niv_full, beta, self.irrk_ind)
         inv, full, core = K.chi0_prepare(chi0, nb, m.niv, beta)
         kern = K.dc_kernel(chi0, self.u_ref, self.f_magn, nb, m.niv, beta)
-        aux, info = K.ladder_solve_sum(inv, self.gamma, self.u_r, None, beta, self.precision, check_info=False)
+        return dict(g=g, chi0=chi0, chi0_inv=inv, full=full, core=core, kernel=kern)
+
+    def ladder_model(self, front):
+        """The ladder arguments of this model for kernels.ladder_solve_sum_multi."""
+        return (front["chi0_inv"], self.gamma, self.u_r, None, self.m.beta)
+
+    def back(self, front, aux, info, sigma, sigma_old=None, keep=False):
+        """From the ladder sums to the new, mixed Sigma: vrg and chi, the kernel, the Sigma
+        convolution, Hartree-Fock and the assembly with mixing."""
+        m = self.m
+        nb, beta = m.nb, m.beta
+        g, inv, full, core, kern = (front[k] for k in ("g", "chi0_inv", "full", "core", "kernel"))
+        chi0 = front["chi0"]
         vrgs, chis = [], []
         for c, (ch, weight) in enumerate(self.CHANNELS):
             vrg, chi = K.vrg_chi_phys(inv, aux[c], full, core, self.u_r[c], None, nb, beta)
@@ -115,12 +135,14 @@ class SDEPipeline:
         return new
 
     # -- HIP graph of one step ------------------------------------------------------------------
-    def capture(self, mu=None):
+    def capture(self, mu=None, chain=False):
         """Capture one plain-FP64 SDE step as a HIP graph (torch.cuda.CUDAGraph). The captured step
         reads Sigma from the fixed buffer `graph_sigma` [N,nb,nb,2 niv_s] and writes the new, mixed
-        Sigma to `graph_out`; replay() runs it again without host work. mu, the occupations of the
-        Hartree-Fock terms and the model are fixed at capture time. The mixed forms read one float
-        per refinement step on the host and are not capturable."""
+        Sigma to `graph_out`; replay() runs it again without host work. With chain, the graph ends
+        by copying graph_out into graph_sigma, so that consecutive replays are consecutive
+        iterations. mu, the occupations of the Hartree-Fock terms and the model are fixed at
+        capture time. The mixed forms read one float per refinement step on the host and are not
+        capturable."""
         if self.precision != _lib.PLAIN_FP64:
             raise ValueError("only the plain FP64 step can be captured as a graph")
         mu = self.m.mu if mu is None else mu
@@ -132,6 +154,8 @@ class SDEPipeline:
         self.graph = torch.cuda.CUDAGraph()
         with torch.cuda.graph(self.graph):
             self.graph_out = self.step(self.graph_sigma, mu)
+            if chain:
+                self.graph_sigma.copy_(self.graph_out)
         torch.cuda.synchronize()
         return self.graph
 
@@ -170,3 +194,63 @@ class SDEPipeline:
         """Algorithmic flops of one solve: complex LU (8/3 n^3) + nb^2 triangular RHS pairs (8 nb^2 n^2)."""
         n = self.m.nb**2 * 2 * self.m.niv
         return 8.0 / 3.0 * n**3 + 8.0 * self.m.nb**2 * n**2
+
+
+class ModelBatch:
+    """Independent models advanced together. One step runs, for every model, the front part of the
+    SDE iteration (G, bubble, chi0^-1, double counting), then the ladder solves of all models in one
+    dgami_ladder_solve_sum_multi call (the small compound sizes share register-kernel launches, the
+    large ones take the blocked LU, every shape in the kernel the autotuner picked), then each
+    model's back part to its new, mixed Sigma. capture() records the whole step of all models as
+    one HIP graph; with chain, consecutive replays are consecutive self-consistency iterations at
+    fixed mu and occupations. The reference runs one model per invocation
+    (nonlocal_sde.py:777-914); this replaces a loop of such runs."""
+
+    def __init__(self, models, mixing=0.3, symmetries="two_dimensional_square"):
+        self.pipes = [SDEPipeline(m, _lib.PLAIN_FP64, symmetries, mixing) for m in models]
+
+    def tune(self, **kw):
+        """Autotune the ladder kernel of every distinct shape of the batch."""
+        from dgamore_amd import autotune
+
+        out, seen = [], set()
+        for p in self.pipes:
+            key = (p.m.nb, p.m.niv, p.nq, p.m.niw + 1)
+            if key not in seen:
+                seen.add(key)
+                out.append(autotune.tune_ladder(*key, nchan=len(p.CHANNELS), **kw))
+        return out
+
+    def step(self, sigmas=None, mus=None, keep=False):
+        sigmas = [p.sigma_dmft for p in self.pipes] if sigmas is None else sigmas
+        mus = [None] * len(self.pipes) if mus is None else mus
+        fronts = [p.front(s, mu) for p, s, mu in zip(self.pipes, sigmas, mus)]
+        outs = K.ladder_solve_sum_multi([p.ladder_model(f) for p, f in zip(self.pipes, fronts)], check_info=False)
+        return [p.back(f, aux, info, s, None, keep) for p, f, (aux, info), s in zip(self.pipes, fronts, outs, sigmas)]
+
+    def capture(self, chain=True):
+        """Record one step of every model as one HIP graph reading graph_sigmas and writing
+        graph_outs (and, with chain, copying graph_outs back into graph_sigmas)."""
+        self.graph_sigmas = [p.sigma_dmft.expand(p.N, p.m.nb, p.m.nb, 2 * p.niv_s).contiguous().clone()
+                             for p in self.pipes]
+        self.step(self.graph_sigmas)  # warm-up: workspaces and occupations
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(self.graph):
+            self.graph_outs = self.step(self.graph_sigmas)
+            if chain:
+                for s, o in zip(self.graph_sigmas, self.graph_outs):
+                    s.copy_(o)
+        torch.cuda.synchronize()
+        return self.graph
+
+    def replay(self):
+        self.graph.replay()
+        return self.graph_outs
+
+    def solves_per_step(self):
+        return sum(p.solves_per_step() for p in self.pipes)
+
+    def ladder_flops_per_step(self):
+        return sum(p.solves_per_step() * p.ladder_flops_per_solve() for p in self.pipes)
+

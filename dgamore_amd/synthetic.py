@@ -190,7 +190,21 @@ def _uncompound(m, nb, nv):
     return m.reshape(nw, nb, nb, nv, nb, nb, nv).transpose(1, 2, 5, 4, 0, 3, 6)
 
 
-def local_vertices(g_loc, niw, niv, niv_full, beta, rng, rel=0.1, inter=0.05):
+def _inverse_products(c0, chi, beta, want_f, device):
+    """beta^2 (chi^-1 - c0^-1) or beta^2 (c0^-1 - c0^-1 chi c0^-1), on a torch device when given
+    (the per-w inverses of large boxes take minutes in NumPy)."""
+    if device is None:
+        c0inv = np.linalg.inv(c0)
+        return beta**2 * (c0inv - c0inv @ chi @ c0inv) if want_f else beta**2 * (np.linalg.inv(chi) - c0inv)
+    import torch
+
+    c0d, chid = torch.as_tensor(c0, device=device), torch.as_tensor(chi, device=device)
+    c0inv = torch.linalg.inv(c0d)
+    out = c0inv - c0inv @ chid @ c0inv if want_f else torch.linalg.inv(chid) - c0inv
+    return beta**2 * out.cpu().numpy()
+
+
+def local_vertices(g_loc, niw, niv, niv_full, beta, rng, rel=0.1, inter=0.05, device=None):
     """(Gamma_dens, Gamma_magn, F_magn) from chi_r = chi0_loc + Hermitian perturbation."""
     nb = g_loc.shape[0]
     out = {}
@@ -215,13 +229,8 @@ def local_vertices(g_loc, niw, niv, niv_full, beta, rng, rel=0.1, inter=0.05):
             p = np.where(same, p, inter / rel * p)
             scale = rel * np.linalg.norm(np.diag(c0[w])) / max(np.linalg.norm(p), 1e-300)
             chi[w] = c0[w] + scale * p
-        c0inv = np.linalg.inv(c0)
-        if ch == "f_magn":
-            f = beta**2 * (c0inv - c0inv @ chi @ c0inv)
-            out["f_magn"] = _uncompound(f, nb, 2 * nvb)
-        else:
-            gam = beta**2 * (np.linalg.inv(chi) - c0inv)
-            out[key] = _uncompound(gam, nb, 2 * nvb)
+        res = _inverse_products(c0, chi, beta, ch == "f_magn", device)
+        out["f_magn" if ch == "f_magn" else key] = _uncompound(res, nb, 2 * nvb)
     return out
 
 
@@ -241,10 +250,30 @@ CONFIGS = {
 }
 
 
-def make_model(cfg, seed=None, nk=None, niw=None, niv=None, niv_shell=0, with_vertices=True):
-    """Build the synthetic model of a BASELINE config (optionally with smaller boxes/grids for tests)."""
+def batch_configs(count=16, seed=3, nb_choices=(1, 2, 3), betas=(5.0, 10.0, 20.0, 40.0), grids=(16, 24, 32)):
+    """Parameters of a batch of independent models with uneven sizes: orbital count, beta
+    (niv = niw = round(1.5 beta)), k-grid, U in [2, 8] (J = 0.15 U) and filling in [0.8, 1.0],
+    drawn from a seed. Compound sizes n = nb^2 * 2 niv range over 16..1080 for the defaults."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for i in range(count):
+        nb = int(rng.choice(nb_choices))
+        beta = float(rng.choice(betas))
+        nx = int(rng.choice(grids))
+        U = float(rng.uniform(2.0, 8.0))
+        fill = float(rng.uniform(0.8, 1.0))
+        niv = int(round(1.5 * beta))
+        disp = {1: ("1band", 1.0, -0.2, 0.1), 2: ("eg",), 3: ("t2g",)}[nb]
+        out.append(dict(name=f"batch{i}_{nb}orb_{nx}x{nx}_b{beta:g}_niv{niv}", nb=nb, nk=(nx, nx, 1), beta=beta,
+                        niw=niv, niv=niv, U=U, J=0.15 * U, n=fill, disp=disp, seed=int(rng.integers(1 << 30))))
+    return out
+
+
+def make_model(cfg, seed=None, nk=None, niw=None, niv=None, niv_shell=0, with_vertices=True, device=None):
+    """Build the synthetic model of a BASELINE config (optionally with smaller boxes/grids for tests).
+    device: torch device for the per-frequency inverses of the local vertices (None: NumPy)."""
     c = dict(CONFIGS[cfg]) if isinstance(cfg, int) else dict(cfg)
-    seed = (1000 + cfg if isinstance(cfg, int) else 1000) if seed is None else seed
+    seed = (1000 + cfg if isinstance(cfg, int) else c.get("seed", 1000)) if seed is None else seed
     rng = np.random.default_rng(seed)
     nk = tuple(nk or c["nk"])
     niw = c["niw"] if niw is None else niw
@@ -266,7 +295,7 @@ def make_model(cfg, seed=None, nk=None, niw=None, niv=None, niv_shell=0, with_ve
     if with_vertices:
         g = lattice_g(ek, sig, mu, beta, niw + niv + niv_shell + 10)
         g_loc = g.mean(axis=(0, 1, 2))
-        v = local_vertices(g_loc, niw, niv, niv + niv_shell, beta, rng)
+        v = local_vertices(g_loc, niw, niv, niv + niv_shell, beta, rng, device=device)
         m.gamma = {"dens": v["dens"], "magn": v["magn"]}
         m.f_magn = v["f_magn"]
     return m

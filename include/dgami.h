@@ -138,6 +138,19 @@ typedef struct {
  * batch of models. */
 int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_model* models, void* stream);
 
+/* Autotuner table of the ladder solves. A kernel id is family * 256 + variant, family 1 = register
+ * tiles, 2 = column-cyclic, 3 = blocked LU (variant: panel shape). candidates() writes up to `cap`
+ * ids that can run a compound size n with nrhs right-hand sides in `precision` and returns how many
+ * there are; set() stores the choice for (n, nrhs, precision) in a process-wide table that every
+ * later ladder call reads (kernel_id -1 removes it); get() returns the stored id or -1; clear()
+ * empties the table. The environment variable DGAMI_LADDER_KERNEL still overrides the table. No
+ * counterpart in the reference (NumPy picks its LAPACK kernels itself); the host-side autotuner
+ * times the candidates and fills the table. */
+int dgami_ladder_tune_candidates(int n, int nrhs, int precision, int* ids, int cap);
+int dgami_ladder_tune_set(int n, int nrhs, int precision, int kernel_id);
+int dgami_ladder_tune_get(int n, int nrhs, int precision);
+int dgami_ladder_tune_clear(void);
+
 /* Dense-matrix variant of the ladder solve: a_cmp holds nmat assembled compound matrices
  * [nmat][n][n] (n = nb^2 * 2*niv); out[nmat][n][nb^2] = (1/beta) sum_v' [A^-1]_{(o1,o2,v),(o4,o3,v')}
  * at row (o1,o2,v), column (o4,o3). info: [nmat].

@@ -1,0 +1,137 @@
+"""Uneven-size batches: every kernel the autotuner may pick agrees with the oracle, the tuner and
+its cache, and a batch of independent models advanced in one step and in one HIP graph gives
+each model's own single-model result."""
+import sys
+
+import numpy as np
+import pytest
+
+from oracle import dga_oracle as O
+from tests.conftest import ladder_case
+
+pytestmark = pytest.mark.gpu
+
+torch = pytest.importorskip("torch")
+
+RTOL = 1e-10
+
+
+def say(msg):
+    print(f"[gpu-case] {msg}", flush=True)
+    sys.stdout.flush()
+
+
+def rel(a, b):
+    a = a.detach().cpu().numpy() if hasattr(a, "detach") else np.asarray(a)
+    return np.abs(a - b).max() / np.abs(b).max()
+
+
+@pytest.fixture(scope="module")
+def K():
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dgamore_amd import kernels
+
+    return kernels
+
+
+@pytest.fixture()
+def A(K):
+    from dgamore_amd import autotune
+
+    autotune.clear()
+    yield autotune
+    autotune.clear()
+
+
+def _solve(K, case, precision):
+    chi0_inv, gamma, u, v, beta = case
+    out, info = K.ladder_solve_sum(K.ref4_to_qf(K.to_dev(chi0_inv)), K.vertex_to_cmp(K.to_dev(gamma))[None].contiguous(),
+                                   K.orb_to_m(K.to_dev(u))[None].contiguous(),
+                                   K.orb_to_m(K.to_dev(v))[None].contiguous(), beta, precision)
+    torch.cuda.synchronize()
+    assert int(info.abs().sum()) == 0
+    return K.qf_to_ref4(out[0], chi0_inv.shape[1])
+
+
+@pytest.mark.parametrize("nb,niv,niw,nq", [
+    (1, 8, 2, 3),    # n = 16
+    (2, 5, 2, 3),    # n = 40, 4 right-hand sides
+    (3, 4, 1, 2),    # n = 72, 9 right-hand sides
+    (1, 60, 1, 2),   # n = 120
+    (3, 12, 1, 1),   # n = 216
+    (3, 40, 1, 1),   # n = 720
+    (3, 60, 0, 1),   # n = 1080
+])
+def test_every_tuner_candidate_matches_oracle(K, A, nb, niv, niw, nq):
+    from dgamore_amd import _lib
+
+    n = nb * nb * 2 * niv
+    case = ladder_case(nb, niv, niw, nq, seed=900 + n)
+    ref = O.ladder_sum_solve(*case)
+    for kid in A.candidates(n, nb * nb, _lib.PLAIN_FP64):
+        say(f"tuner candidate {A.kernel_name(kid)} plain n={n} nb={nb} niw={niw} nq={nq}")
+        A.set_choice(n, nb * nb, _lib.PLAIN_FP64, kid)
+        assert rel(_solve(K, case, _lib.PLAIN_FP64), ref) < RTOL
+    if n >= 216:
+        for kid in A.candidates(n, nb * nb, _lib.MIXED_BF16X3_IR):
+            say(f"tuner candidate {A.kernel_name(kid)} bf16x3 n={n}")
+            A.set_choice(n, nb * nb, _lib.MIXED_BF16X3_IR, kid)
+            assert rel(_solve(K, case, _lib.MIXED_BF16X3_IR), ref) < 1e-8
+
+
+def test_tune_ladder_times_and_caches(K, A, tmp_path, monkeypatch):
+    monkeypatch.setenv(A.CACHE_ENV, str(tmp_path / "tune.json"))
+    say("tune_ladder n=40 (10 candidates) and n=216 (2), then from the cache")
+    first = [A.tune_ladder(2, 5, 3, 3), A.tune_ladder(3, 12, 2, 2)]
+    for r in first:
+        assert r["source"] == "timed" and r["choice"] in A.candidates(r["n"], r["nrhs"])
+        assert A.get_choice(r["n"], r["nrhs"]) == r["choice"] and len(r["times_ms"]) >= 2
+    assert (tmp_path / "tune.json").exists()
+    A.clear()
+    again = [A.tune_ladder(2, 5, 3, 3), A.tune_ladder(3, 12, 2, 2)]
+    assert [r["source"] for r in again] == ["cache", "cache"]
+    assert [r["choice"] for r in again] == [r["choice"] for r in first]
+    assert A.get_choice(216, 9) == first[1]["choice"]
+
+
+def _small_batch():
+    from dgamore_amd import synthetic as S
+
+    cfgs = S.batch_configs(count=4, seed=11, betas=(5.0, 8.0), grids=(6, 8))
+    return [S.make_model(c, niw=min(c["niw"], 5), niv=min(c["niv"], 6), device=torch.device("cuda")) for c in cfgs]
+
+
+def test_model_batch_step_matches_single_models(K, A):
+    from dgamore_amd.pipeline import ModelBatch, SDEPipeline
+
+    models = _small_batch()
+    say("model batch step vs single-model steps: " + ", ".join(f"{m.name} n={m.nb**2*2*m.niv}" for m in models))
+    mb = ModelBatch(models, mixing=0.3)
+    outs = mb.step()
+    torch.cuda.synchronize()
+    for m, out in zip(models, outs):
+        ref = SDEPipeline(m, mixing=0.3).step()
+        torch.cuda.synchronize()
+        assert rel(out, ref.cpu().numpy()) < RTOL, m.name
+
+
+def test_model_batch_graph_chain_matches_eager_iterations(K, A):
+    from dgamore_amd.pipeline import ModelBatch
+
+    models = _small_batch()
+    say("model batch HIP graph, chained replays vs eager iterations (mixing 0.3, 3 iterations)")
+    mb = ModelBatch(models, mixing=0.3)
+    mb.tune()
+    mb.capture(chain=True)
+    start = [s.clone() for s in mb.graph_sigmas]
+    sig = [s.clone() for s in start]
+    for it in range(3):
+        mb.replay()
+        torch.cuda.synchronize()
+        sig = mb.step(sig)  # eager iteration from the previous eager Sigma
+        torch.cuda.synchronize()
+        for g, e, m in zip(mb.graph_sigmas, sig, models):
+            assert rel(g, e.cpu().numpy()) < 1e-12, (it, m.name)
+    # the iteration moved Sigma
+    assert max(float((g - s).abs().max()) for g, s in zip(mb.graph_sigmas, start)) > 1e-8
