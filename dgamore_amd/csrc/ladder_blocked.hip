@@ -533,6 +533,9 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
   const int wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int l16 = lane & 15, kg = lane >> 4;
+  // tiles that lie wholly inside the update range take unpredicated 16-byte loads; 32-bit
+  // element offsets (n * ld < 2^31) keep the address arithmetic in single registers
+  const bool full = rt + 64 <= g.r1 && ct + 64 <= g.c1 && ((g.c0 | g.k0 | ld) & 1) == 0;
   // the C tile is held apart from the accumulators, so its loads overlap staging and compute
   float2 cin[2][2][4];
 #pragma unroll
@@ -543,7 +546,7 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = rt + wr + 16 * i + 4 * kg + e;
-        cin[i][j][e] = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : make_float2(0.f, 0.f);
+        cin[i][j][e] = (full || (r < g.r1 && c < g.c1)) ? W[static_cast<unsigned>(r * ld + c)] : make_float2(0.f, 0.f);
       }
     }
   V4 accr[2][2], acci[2][2];
@@ -555,23 +558,43 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
       acci[i][j] = V4{0.f, 0.f, 0.f, 0.f};
     }
   // operand chunks of KC columns of k: raw values are loaded to registers (the next chunk's
-  // loads are issued before the current chunk's products), then split and packed into LDS
-  constexpr int NP = 64 * (KC / 2) / 256;  // k pairs per thread and operand
-  float2 ra[NP][2], rb[NP][2];
+  // loads are issued before the current chunk's products), then split and packed into LDS.
+  // A: thread (row tid/16 + 16p, k pair 2 (tid % 16)); B: thread (k pair tid/32 + 8q, column
+  // pair 2 (tid % 32)), each a 16-byte load per row.
+  static_assert(KC == 32, "the chunk mapping assumes 32 complex columns of k");
+  float4 ra[4], rb[2][2];
+  const int a_r = tid >> 4, a_s = 2 * (tid & 15);
+  const int b_p = tid >> 5, b_c = 2 * (tid & 31);
   auto load_chunk = [&](int kc) {
     const int kn = min(KC, g.kw - kc);
+    if (full && kn == KC) {
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      const int e = tid + 256 * p;
-      const int r = e / (KC / 2), s2 = 2 * (e - (e / (KC / 2)) * (KC / 2));
-      const bool in = rt + r < g.r1;
-      const float2* src = W + static_cast<size_t>(rt + r) * ld + g.k0 + kc + s2;
-      ra[p][0] = (in && s2 < kn) ? src[0] : make_float2(0.f, 0.f);
-      ra[p][1] = (in && s2 + 1 < kn) ? src[1] : make_float2(0.f, 0.f);
-      const int s3 = 2 * (e / 64), c = e - (e / 64) * 64;
-      const bool inc = ct + c < g.c1;
-      rb[p][0] = (inc && s3 < kn) ? W[static_cast<size_t>(g.k0 + kc + s3) * ld + ct + c] : make_float2(0.f, 0.f);
-      rb[p][1] = (inc && s3 + 1 < kn) ? W[static_cast<size_t>(g.k0 + kc + s3 + 1) * ld + ct + c] : make_float2(0.f, 0.f);
+      for (int p = 0; p < 4; ++p)
+        ra[p] = *reinterpret_cast<const float4*>(W + static_cast<unsigned>((rt + a_r + 16 * p) * ld + g.k0 + kc + a_s));
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+          rb[q][h] = *reinterpret_cast<const float4*>(
+              W + static_cast<unsigned>((g.k0 + kc + 2 * (b_p + 8 * q) + h) * ld + ct + b_c));
+    } else {
+      auto ld1 = [&](bool in, int r, int c) { return in ? W[static_cast<unsigned>(r * ld + c)] : make_float2(0.f, 0.f); };
+#pragma unroll
+      for (int p = 0; p < 4; ++p) {
+        const int r = rt + a_r + 16 * p;
+        const float2 x0 = ld1(r < g.r1 && a_s < kn, r, g.k0 + kc + a_s);
+        const float2 x1 = ld1(r < g.r1 && a_s + 1 < kn, r, g.k0 + kc + a_s + 1);
+        ra[p] = make_float4(x0.x, x0.y, x1.x, x1.y);
+      }
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int k = 2 * (b_p + 8 * q) + h, c = ct + b_c;
+          const float2 x0 = ld1(k < kn && c < g.c1, g.k0 + kc + k, c);
+          const float2 x1 = ld1(k < kn && c + 1 < g.c1, g.k0 + kc + k, c + 1);
+          rb[q][h] = make_float4(x0.x, x0.y, x1.x, x1.y);
+        }
     }
   };
   load_chunk(0);
@@ -579,20 +602,26 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
     if (kc > 0) __syncthreads();
     typedef __bf16 B4 __attribute__((ext_vector_type(4)));
 #pragma unroll
-    for (int p = 0; p < NP; ++p) {
-      const int e = tid + 256 * p;
-      const int r = e / (KC / 2), s2 = 2 * (e - (e / (KC / 2)) * (KC / 2));
-      const Bf16Split r0 = bf16_split(ra[p][0].x), i0 = bf16_split(ra[p][0].y), r1 = bf16_split(ra[p][1].x),
-                      i1 = bf16_split(ra[p][1].y);
-      *reinterpret_cast<B4*>(&s_a[0][r][2 * s2]) = B4{-r0.hi, i0.hi, -r1.hi, i1.hi};
-      *reinterpret_cast<B4*>(&s_a[1][r][2 * s2]) = B4{-r0.lo, i0.lo, -r1.lo, i1.lo};
-      *reinterpret_cast<B4*>(&s_a[2][r][2 * s2]) = B4{-i0.hi, -r0.hi, -i1.hi, -r1.hi};
-      *reinterpret_cast<B4*>(&s_a[3][r][2 * s2]) = B4{-i0.lo, -r0.lo, -i1.lo, -r1.lo};
-      const int s3 = 2 * (e / 64), c = e - (e / 64) * 64;
-      const Bf16Split b0 = bf16_split(rb[p][0].x), j0 = bf16_split(rb[p][0].y), b1 = bf16_split(rb[p][1].x),
-                      j1 = bf16_split(rb[p][1].y);
-      *reinterpret_cast<B4*>(&s_b[0][c][2 * s3]) = B4{b0.hi, j0.hi, b1.hi, j1.hi};
-      *reinterpret_cast<B4*>(&s_b[1][c][2 * s3]) = B4{b0.lo, j0.lo, b1.lo, j1.lo};
+    for (int p = 0; p < 4; ++p) {
+      const int r = a_r + 16 * p;
+      const Bf16Split r0 = bf16_split(ra[p].x), i0 = bf16_split(ra[p].y), r1 = bf16_split(ra[p].z),
+                      i1 = bf16_split(ra[p].w);
+      *reinterpret_cast<B4*>(&s_a[0][r][2 * a_s]) = B4{-r0.hi, i0.hi, -r1.hi, i1.hi};
+      *reinterpret_cast<B4*>(&s_a[1][r][2 * a_s]) = B4{-r0.lo, i0.lo, -r1.lo, i1.lo};
+      *reinterpret_cast<B4*>(&s_a[2][r][2 * a_s]) = B4{-i0.hi, -r0.hi, -i1.hi, -r1.hi};
+      *reinterpret_cast<B4*>(&s_a[3][r][2 * a_s]) = B4{-i0.lo, -r0.lo, -i1.lo, -r1.lo};
+    }
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+      const int k2 = 2 * (2 * (b_p + 8 * q));
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const float br0 = h ? rb[q][0].z : rb[q][0].x, bi0 = h ? rb[q][0].w : rb[q][0].y;
+        const float br1 = h ? rb[q][1].z : rb[q][1].x, bi1 = h ? rb[q][1].w : rb[q][1].y;
+        const Bf16Split b0 = bf16_split(br0), j0 = bf16_split(bi0), b1 = bf16_split(br1), j1 = bf16_split(bi1);
+        *reinterpret_cast<B4*>(&s_b[0][b_c + h][k2]) = B4{b0.hi, j0.hi, b1.hi, j1.hi};
+        *reinterpret_cast<B4*>(&s_b[1][b_c + h][k2]) = B4{b0.lo, j0.lo, b1.lo, j1.lo};
+      }
     }
     if (kc + KC < g.kw) load_chunk(kc + KC);
     __syncthreads();
@@ -635,8 +664,8 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = rt + wr + 16 * i + 4 * kg + e;
-        if (r < g.r1 && c < g.c1)
-          W[static_cast<size_t>(r) * ld + c] = make_float2(cin[i][j][e].x + accr[i][j][e], cin[i][j][e].y + acci[i][j][e]);
+        if (full || (r < g.r1 && c < g.c1))
+          W[static_cast<unsigned>(r * ld + c)] = make_float2(cin[i][j][e].x + accr[i][j][e], cin[i][j][e].y + acci[i][j][e]);
       }
     }
 }
@@ -797,9 +826,13 @@ int blocked_factor<double2>(const LadderArgs& a, BlockedState<double2>& st, int 
 template <>
 int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int item0, int nb, hipStream_t stream) {
   // three rows per thread: 256-thread panels for n <= 768, two problems per CU
-  const bool nb32 = blocked_variant_fits(true, st.variant, st.n) ? st.variant == 0 : st.n <= 1152;
-  if (nb32) return blocked_factor_nb<float2, 32, 3, 384, 4>(a, st, item0, nb, stream);
-  return blocked_factor_nb<float2, 16, 3, 512, 4>(a, st, item0, nb, stream);
+  const int v = blocked_variant_fits(true, st.variant, st.n) ? st.variant : (st.n <= 1152 ? 0 : 1);
+  switch (v) {
+    case 0: return blocked_factor_nb<float2, 32, 3, 384, 4>(a, st, item0, nb, stream);
+    case 2: return blocked_factor_nb<float2, 32, 3, 384, 8>(a, st, item0, nb, stream);
+    case 3: return blocked_factor_nb<float2, 16, 3, 512, 8>(a, st, item0, nb, stream);
+    default: return blocked_factor_nb<float2, 16, 3, 512, 4>(a, st, item0, nb, stream);
+  }
 }
 
 int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,

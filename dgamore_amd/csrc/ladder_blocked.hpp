@@ -9,7 +9,7 @@ namespace dgami {
 constexpr int BLK_MAX_NB = 64;                      // widest panel
 constexpr int BLK_MAX_N = 1536;                     // largest compound size of the blocked LU
 constexpr int BLK_MAP_STRIDE = 3 * BLK_MAX_NB + 1;  // ints of one problem's interchange map
-constexpr int BLK_PANELS = 4;                       // register panels per outer block (trailing-update depth)
+constexpr int BLK_PANELS = 8;                       // most register panels per outer block (trailing-update depth)
 
 // Trailing-update kernels of the complex64 factorisation (complex128 always uses vector FMA).
 enum BlockedGemm { BLK_GEMM_VECTOR = 0, BLK_GEMM_F32_MFMA = 1, BLK_GEMM_BF16X3 = 2 };

@@ -39,7 +39,9 @@ int candidates(int n, int nrhs, int precision, int* ids, int cap) {
 bool blocked_variant_fits(bool complex64, int variant, int n) {
   if (n < 1 || n > BLK_MAX_N) return false;
   if (variant == 0) return n <= (complex64 ? 1152 : 768);
-  return variant == 1;
+  if (variant == 1) return true;
+  if (!complex64) return false;
+  return variant == 3 || (variant == 2 && n <= 1152);
 }
 
 int ladder_tuned_kernel(int n, int nrhs, int precision) {

@@ -14,9 +14,10 @@ constexpr int ladder_kernel_family(int id) { return id / 256; }
 constexpr int ladder_kernel_variant(int id) { return id % 256; }
 
 // Blocked-LU variants (panel width NB, rows per panel thread, panels per outer block):
-//   complex128: 0 = NB 32, one row per thread (n <= 768); 1 = NB 16, three rows (n <= 1536)
-//   complex64:  0 = NB 32, three rows (n <= 1152);        1 = NB 16, three rows (n <= 1536)
-constexpr int BLK_VARIANTS = 2;
+//   complex128: 0 = NB 32, one row per thread, 2 panels (n <= 768); 1 = NB 16, three rows, 2 panels
+//   complex64:  0 = NB 32, three rows, 4 panels (n <= 1152);        1 = NB 16, three rows, 4 panels
+//               2 = NB 32, three rows, 8 panels (n <= 1152);        3 = NB 16, three rows, 8 panels
+constexpr int BLK_VARIANTS = 4;
 bool blocked_variant_fits(bool complex64, int variant, int n);
 
 // Tuned kernel id of (n, nrhs, precision), or -1 when the autotuner has not set one.
