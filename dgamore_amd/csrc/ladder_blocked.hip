@@ -75,17 +75,21 @@ struct PanelShared {
   unsigned key[2][16];
   int idx[2][16];
   S prow[2][NB];
-  S krow[2][NB];
   int piv[NB];
   int mdst[2 * NB], msrc[2 * NB];
   int mdest[NB];
   int ocount;
-  int rowval[BLK_MAX_N];  // rowval[r - p0]: the row whose former value row r holds after the interchanges
+  int rowat[BLK_MAX_N];  // rowat[i]: the original row now at row p0 + i (after the interchanges so far)
+  int pos[BLK_MAX_N];    // pos[r - p0]: where original row r is now, relative to p0
 };
 
+// Rows stay in the registers of their original owner; the interchanges are only booked (one
+// thread keeps rowat/pos in LDS) and applied when the panel is written back. A row is
+// eliminated ("done") once it has been a pivot: the candidates of column C are the rows not
+// yet done, exactly the rows at positions >= p0 + C in the interchanged order.
 template <typename S, int NB, int RPT, int C>
-__device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>& sh, BlockedState<S>& st, int b,
-                                             int p0, int nbw) {
+__device__ __forceinline__ void panel_column(S (&P)[RPT][NB], unsigned& done, PanelShared<S, NB>& sh,
+                                             BlockedState<S>& st, int b, int p0, int nbw) {
   if (C >= nbw) return;
   const int n = st.n, TH = blockDim.x;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6, nwave = TH >> 6;
@@ -97,7 +101,7 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
   for (int m = 0; m < RPT; ++m) {
     const int r = p0 + tid + TH * m;
     const double v = static_cast<double>(cabs1(P[m][C]));
-    if (r >= col && r < n && v > best) {
+    if (!((done >> m) & 1u) && r < n && v > best) {
       best = v;
       bidx = r;
     }
@@ -125,26 +129,23 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
     }
   }
   best = gkey > 1u ? 1.0 : 0.0;  // nonzero pivot magnitude
-  const int p = bidx;
-  constexpr int mk = 0;  // column C's diagonal row p0 + C lies in the first register row (C < NB <= TH)
-  if constexpr (RPT == 1) {
-    if (tid == p - p0) {
+  const int p = bidx;            // original index of the pivot row
 #pragma unroll
-      for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[0][cc];
+  for (int m = 0; m < RPT; ++m)
+    if (p0 + tid + TH * m == p) {
+#pragma unroll
+      for (int cc = C; cc < NB; ++cc) sh.prow[buf][cc] = P[m][cc];
+      done |= 1u << m;
     }
-  } else {
-#pragma unroll
-    for (int m = 0; m < RPT; ++m)
-      if (p0 + tid + TH * m == p) {
-#pragma unroll
-        for (int cc = 0; cc < NB; ++cc) sh.prow[buf][cc] = P[m][cc];
-      }
+  if (tid == 0) {
+    // the interchange of this column: the pivot row moves to p0 + C, the row there to its place
+    const int o = p - p0, pc = sh.pos[o], o2 = sh.rowat[C] - p0;
+    sh.piv[C] = p0 + pc;
+    sh.rowat[pc] = o2 + p0;
+    sh.pos[o2] = pc;
+    sh.rowat[C] = p;
+    sh.pos[o] = C;
   }
-  if (p != col && tid == C) {
-#pragma unroll
-    for (int cc = 0; cc < NB; ++cc) sh.krow[buf][cc] = P[mk][cc];
-  }
-  if (tid == 0) sh.piv[C] = p;
   __syncthreads();
   const bool ok = best > 0.0;
   const S pinv = ok ? cinv(sh.prow[buf][C]) : czero<S>();
@@ -155,14 +156,7 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
 #pragma unroll
   for (int m = 0; m < RPT; ++m) {
     const int r = p0 + tid + TH * m;
-    if (r == col) {
-#pragma unroll
-      for (int cc = 0; cc < NB; ++cc) P[m][cc] = sh.prow[buf][cc];
-    } else if (r > col && r < n) {
-      if (r == p) {
-#pragma unroll
-        for (int cc = 0; cc < NB; ++cc) P[m][cc] = sh.krow[buf][cc];
-      }
+    if (!((done >> m) & 1u) && r < n) {
       const S l = cmul(P[m][C], pinv);
       P[m][C] = l;
 #pragma unroll
@@ -172,9 +166,10 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], PanelShared<S, NB>
 }
 
 template <typename S, int NB, int RPT, int... Cs>
-__device__ __forceinline__ void panel_columns(S (&P)[RPT][NB], PanelShared<S, NB>& sh, BlockedState<S>& st, int b,
-                                              int p0, int nbw, std::integer_sequence<int, Cs...>) {
-  (panel_column<S, NB, RPT, Cs>(P, sh, st, b, p0, nbw), ...);
+__device__ __forceinline__ void panel_columns(S (&P)[RPT][NB], unsigned& done, PanelShared<S, NB>& sh,
+                                              BlockedState<S>& st, int b, int p0, int nbw,
+                                              std::integer_sequence<int, Cs...>) {
+  (panel_column<S, NB, RPT, Cs>(P, done, sh, st, b, p0, nbw), ...);
 }
 
 template <typename S, int NB, int RPT, int MAXTH>
@@ -194,6 +189,10 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
   for (int m = 0; m < RPT; ++m) {
     const int r = p0 + tid + TH * m;
     const S* src = W + static_cast<size_t>(r) * ld + p0;
+    if (r < n) {
+      sh.rowat[r - p0] = r;
+      sh.pos[r - p0] = r - p0;
+    }
     if (pairs && r < n && nbw == NB) {
 #pragma unroll
       for (int c = 0; c < NB; c += 2) {
@@ -206,49 +205,24 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
       for (int c = 0; c < NB; ++c) P[m][c] = (r < n && c < nbw) ? src[c] : czero<S>();
     }
   }
-  panel_columns<S, NB, RPT>(P, sh, st, b, p0, nbw, std::make_integer_sequence<int, NB>{});
-#pragma unroll
-  for (int m = 0; m < RPT; ++m) {
-    const int r = p0 + tid + TH * m;
-    S* dst = W + static_cast<size_t>(r) * ld + p0;
-    if (pairs && r < n && nbw == NB) {
-#pragma unroll
-      for (int c = 0; c < NB; c += 2)
-        *reinterpret_cast<float4*>(dst + c) = make_float4(P[m][c].x, P[m][c].y, P[m][c + 1].x, P[m][c + 1].y);
-    } else {
-#pragma unroll
-      for (int c = 0; c < NB; ++c)
-        if (r < n && c < nbw) dst[c] = P[m][c];
-    }
-  }
+  // (the first column's barrier orders these LDS writes before thread 0's bookkeeping)
+  unsigned done = 0;
+  panel_columns<S, NB, RPT>(P, done, sh, st, b, p0, nbw, std::make_integer_sequence<int, NB>{});
+  __syncthreads();
   // Interchange map of the panel. A row below the panel only ever exchanges its value with a
   // panel row, so after the NB interchanges every row below the panel holds a former panel
   // row's value, and the former values of rows below the panel end in panel rows:
   //   outer[e] = (dst, src): panel row dst receives row src (src below the panel), e < count;
   //   dest[c]:  panel row p0 + c's former value goes to row dest[c] (-1: stays).
-  // The interchanges are replayed on an LDS table of the touched rows (one thread, one swap per
-  // column), then every panel row and every pivot row below the panel is classified in parallel.
-  int* rv = sh.rowval;
+  // rowat holds the interchanged order; every panel row and every pivot row below the panel is
+  // classified in parallel.
+  const int* rv = sh.rowat;
   const int pbase = p0 + nbw;
   if (tid < nbw) {
-    const int p = sh.piv[tid];
-    rv[tid] = p0 + tid;
-    rv[p - p0] = p;  // a row chosen by several columns gets the same value from each
     sh.mdest[tid] = -1;
-    st.piv[static_cast<size_t>(b) * n + p0 + tid] = p;
+    st.piv[static_cast<size_t>(b) * n + p0 + tid] = sh.piv[tid];
   }
   if (tid == 0) sh.ocount = 0;
-  __syncthreads();
-  if (tid == 0) {
-    for (int c = 0; c < nbw; ++c) {
-      const int p = sh.piv[c] - p0;
-      if (p != c) {
-        const int t = rv[c];
-        rv[c] = rv[p];
-        rv[p] = t;
-      }
-    }
-  }
   __syncthreads();
   if (tid < nbw) {
     const int r = p0 + tid, src = rv[tid];
@@ -267,6 +241,22 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
       for (int c = 0; c < tid; ++c) first = first && sh.piv[c] != p;
       const int sp = rv[p - p0];
       if (first && sp != p) sh.mdest[sp - p0] = p;
+    }
+  }
+  // rows are written to their interchanged places
+#pragma unroll
+  for (int m = 0; m < RPT; ++m) {
+    const int r = p0 + tid + TH * m;
+    if (r >= n) continue;
+    S* dst = W + static_cast<size_t>(p0 + sh.pos[r - p0]) * ld + p0;
+    if (pairs && nbw == NB) {
+#pragma unroll
+      for (int c = 0; c < NB; c += 2)
+        *reinterpret_cast<float4*>(dst + c) = make_float4(P[m][c].x, P[m][c].y, P[m][c + 1].x, P[m][c + 1].y);
+    } else {
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nbw) dst[c] = P[m][c];
     }
   }
   __syncthreads();
