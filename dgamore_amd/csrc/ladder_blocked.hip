@@ -35,7 +35,20 @@ constexpr int MAP_STRIDE = BLK_MAP_STRIDE;  // interchange map per problem: coun
 
 // ---------------------------------------------------------------------------
 // grid (ceil(n / ASM_ROWS), batch), 256 threads: ASM_ROWS rows of one problem per workgroup.
-constexpr int ASM_ROWS = 4;
+// Each thread owns column pairs (j, j + 1); their orbital/frequency split is computed once and
+// reused for every row, and each pair is written as one 16- or 32-byte store.
+constexpr int ASM_ROWS = 8;
+
+template <typename S>
+__device__ __forceinline__ void store_pair(S* dst, double2 x, double2 y) {
+  if constexpr (std::is_same<S, float2>::value) {
+    *reinterpret_cast<float4*>(dst) = make_float4(static_cast<float>(x.x), static_cast<float>(x.y),
+                                                  static_cast<float>(y.x), static_cast<float>(y.y));
+  } else {
+    dst[0] = x;
+    dst[1] = y;
+  }
+}
 
 template <typename S>
 __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0) {
@@ -46,15 +59,39 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
   const double2* U = a.dense ? nullptr : a.u_r + chan * a.T * a.T;
   const double2* V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
   S* Wb = st.W + static_cast<size_t>(b) * st.n * st.ld;
-  const int r1 = min(st.n, (blockIdx.x + 1) * ASM_ROWS);
-  for (int i = blockIdx.x * ASM_ROWS; i < r1; ++i) {
-    for (int j = threadIdx.x; j < st.ld; j += 256) {
-      double2 val = make_double2(0.0, 0.0);
-      if (j < st.n)
-        val = ladder_elem(a, G, U, V, w, q, i, j);
-      else if (j < st.ncol)
-        val = ladder_rhs_at(a, w, q, i, j - st.n);
-      Wb[static_cast<size_t>(i) * st.ld + j] = from_c128<S>(val);
+  const int n = st.n, ld = st.ld, ncol = st.ncol, nv = a.nv, T = a.T;
+  const int i0 = blockIdx.x * ASM_ROWS, r1 = min(n, i0 + ASM_ROWS);
+  const double2 zero = make_double2(0.0, 0.0);
+  for (int jb = 2 * threadIdx.x; jb < ld; jb += 512) {  // ld is even
+    int cj[2], vj[2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int j = jb + h;
+      cj[h] = j < n ? j / nv : 0;
+      vj[h] = j < n ? j - cj[h] * nv : -1;
+    }
+    for (int i = i0; i < r1; ++i) {
+      const int ri = i / nv, vi = i - ri * nv;
+      double2 val[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int j = jb + h;
+        val[h] = zero;
+        if (j < n) {
+          if (a.dense) {
+            val[h] = G[static_cast<size_t>(i) * n + j];
+          } else {
+            double2 uv = U[ri * T + cj[h]];
+            if (V) uv = cadd(uv, V[ri * T + cj[h]]);
+            val[h] = cscale(csub(G[static_cast<size_t>(i) * n + j], uv), a.inv_beta2);
+            if (vi == vj[h])
+              val[h] = cadd(val[h], a.chi0_inv[((static_cast<size_t>(w) * T * T + ri * T + cj[h]) * nv + vi) * a.nq + q]);
+          }
+        } else if (j < ncol) {
+          val[h] = ladder_rhs_at(a, w, q, i, j - n);
+        }
+      }
+      store_pair(Wb + static_cast<size_t>(i) * ld + jb, val[0], val[1]);
     }
   }
   if (blockIdx.x == 0 && threadIdx.x == 0) st.info[b] = 0;
