@@ -196,7 +196,9 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   const int n = a.n, T = a.T;
   DGAMI_ARG_CHECK(T <= 16, "ladder: at most 4 orbitals in the mixed-precision form");
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
-  const int ld = (n + 7) & ~7;
+  // the factorisation carries the nb^2 indicator columns, so the first solve needs only the
+  // backward sweep
+  const int ncol = n + T, ld = (ncol + 7) & ~7;
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const int gq = a.dense ? 1 : a.nq;
   const size_t groups = items / gq;
@@ -213,7 +215,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   DGAMI_ARG_CHECK(bitems <= 65535, "ladder: too many q-points in one (channel, w) group");
 
   BlockedState<float2> st;
-  if (!blocked_alloc(st, n, ld, n, bitems, WS_MIXED)) return -2;
+  if (!blocked_alloc(st, n, ld, ncol, bitems, WS_MIXED)) return -2;
   st.swap_left = 1;
   st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
   const int tuned = ladder_tuned_kernel(n, T, precision);
@@ -275,13 +277,19 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, stream, m);
         DGAMI_HIP_CHECK(hipGetLastError());
       }
-      // D = U^-1 L^-1 P R in complex64, X += D
-      hipLaunchKernelGGL(mix_gather_kernel, vgrid, dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0);
-      DGAMI_HIP_CHECK(hipGetLastError());
-      rc = tri_forward(f, nit, T, Yv, Xv, stream);
-      if (rc) return rc;
-      rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
-      if (rc) return rc;
+      // D = U^-1 L^-1 P R in complex64, X += D; the first R is B, whose L^-1 P B the
+      // factorisation left in columns [n, n + T)
+      if (s == 0) {
+        rc = tri_backward(f, nit, T, TriVec<float2>{st.W + n, static_cast<size_t>(n) * ld, ld}, Yv, nullptr, 0, stream);
+        if (rc) return rc;
+      } else {
+        hipLaunchKernelGGL(mix_gather_kernel, vgrid, dim3(256), 0, stream, m, yv, 0);
+        DGAMI_HIP_CHECK(hipGetLastError());
+        rc = tri_forward(f, nit, T, Yv, Xv, stream);
+        if (rc) return rc;
+        rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
+        if (rc) return rc;
+      }
       hipLaunchKernelGGL(mix_accumulate_kernel, dim3(nit), dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0, norms);
       DGAMI_HIP_CHECK(hipGetLastError());
       if (s == 0) continue;
