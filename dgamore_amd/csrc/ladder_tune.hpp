@@ -17,7 +17,8 @@ constexpr int ladder_kernel_variant(int id) { return id % 256; }
 //   complex128: 0 = NB 32, one row per thread, 2 panels (n <= 768); 1 = NB 16, three rows, 2 panels
 //   complex64:  0 = NB 32, three rows, 4 panels (n <= 1152);        1 = NB 16, three rows, 4 panels
 //               2 = NB 32, three rows, 8 panels (n <= 1152);        3 = NB 16, three rows, 8 panels
-constexpr int BLK_VARIANTS = 4;
+//               4 = NB 32, two rows, 4 panels (n <= 1024)
+constexpr int BLK_VARIANTS = 5;
 bool blocked_variant_fits(bool complex64, int variant, int n);
 
 // Tuned kernel id of (n, nrhs, precision), or -1 when the autotuner has not set one.
