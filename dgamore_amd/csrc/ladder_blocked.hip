@@ -564,8 +564,10 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
   const int ld = st.ld;
   float2* W = st.W + static_cast<size_t>(b) * st.n * ld;
   const int rt = g.r0 + tr * 64, ct = g.c0 + tcl * 64;
-  __shared__ __attribute__((aligned(16))) __bf16 s_a[2][64][KR];  // A' hi, A' lo
-  __shared__ __attribute__((aligned(16))) __bf16 s_b[2][64][KR];  // B' hi, B' lo, [column][k]
+  // two LDS buffers: the next chunk is split and staged while the current one feeds the
+  // matrix cores, one barrier per chunk
+  __shared__ __attribute__((aligned(16))) __bf16 s_a[2][2][64][KR];  // [buffer] A' hi, A' lo
+  __shared__ __attribute__((aligned(16))) __bf16 s_b[2][2][64][KR];  // [buffer] B' hi, B' lo, [column][k]
   const int tid = threadIdx.x;
   const int wave = tid >> 6, lane = tid & 63;
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
@@ -636,17 +638,15 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
         }
     }
   };
-  load_chunk(0);
-  for (int kc = 0; kc < g.kw; kc += KC) {
-    if (kc > 0) __syncthreads();
-    typedef __bf16 B4 __attribute__((ext_vector_type(4)));
+  typedef __bf16 B4 __attribute__((ext_vector_type(4)));
+  auto stage = [&](int buf) {
 #pragma unroll
     for (int p = 0; p < 4; ++p) {
       const int r = a_r + 16 * p;
       const Bf16Split r0 = bf16_split(ra[p].x), i0 = bf16_split(ra[p].y), r1 = bf16_split(ra[p].z),
                       i1 = bf16_split(ra[p].w);
-      *reinterpret_cast<B4*>(&s_a[0][r][2 * a_s]) = B4{-r0.hi, i0.hi, -r1.hi, i1.hi};
-      *reinterpret_cast<B4*>(&s_a[1][r][2 * a_s]) = B4{-r0.lo, i0.lo, -r1.lo, i1.lo};
+      *reinterpret_cast<B4*>(&s_a[buf][0][r][2 * a_s]) = B4{-r0.hi, i0.hi, -r1.hi, i1.hi};
+      *reinterpret_cast<B4*>(&s_a[buf][1][r][2 * a_s]) = B4{-r0.lo, i0.lo, -r1.lo, i1.lo};
     }
 #pragma unroll
     for (int q = 0; q < 2; ++q) {
@@ -656,12 +656,12 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
         const float br0 = h ? rb[q][0].z : rb[q][0].x, bi0 = h ? rb[q][0].w : rb[q][0].y;
         const float br1 = h ? rb[q][1].z : rb[q][1].x, bi1 = h ? rb[q][1].w : rb[q][1].y;
         const Bf16Split b0 = bf16_split(br0), j0 = bf16_split(bi0), b1 = bf16_split(br1), j1 = bf16_split(bi1);
-        *reinterpret_cast<B4*>(&s_b[0][b_c + h][k2]) = B4{b0.hi, j0.hi, b1.hi, j1.hi};
-        *reinterpret_cast<B4*>(&s_b[1][b_c + h][k2]) = B4{b0.lo, j0.lo, b1.lo, j1.lo};
+        *reinterpret_cast<B4*>(&s_b[buf][0][b_c + h][k2]) = B4{b0.hi, j0.hi, b1.hi, j1.hi};
+        *reinterpret_cast<B4*>(&s_b[buf][1][b_c + h][k2]) = B4{b0.lo, j0.lo, b1.lo, j1.lo};
       }
     }
-    if (kc + KC < g.kw) load_chunk(kc + KC);
-    __syncthreads();
+  };
+  auto products = [&](int buf) {
 #pragma unroll
     for (int ks = 0; ks < 2 * KC; ks += 32) {
       const int k = ks + 8 * kg;
@@ -669,8 +669,8 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
 #pragma unroll
       for (int i = 0; i < 2; ++i) {
         const int row = wr + 16 * i + l16;
-        ah[i] = *reinterpret_cast<const B8*>(&s_a[0][row][k]);
-        al[i] = *reinterpret_cast<const B8*>(&s_a[1][row][k]);
+        ah[i] = *reinterpret_cast<const B8*>(&s_a[buf][0][row][k]);
+        al[i] = *reinterpret_cast<const B8*>(&s_a[buf][1][row][k]);
         // A'' = (-im, -re) pairs from A' = (-re, im): swap the halves of each word, negate the low one
         a2h[i] = swap_negate_pairs(ah[i]);
         a2l[i] = swap_negate_pairs(al[i]);
@@ -678,8 +678,8 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
 #pragma unroll
       for (int j = 0; j < 2; ++j) {
         const int col = wc + 16 * j + l16;
-        bh[j] = *reinterpret_cast<const B8*>(&s_b[0][col][k]);
-        bl[j] = *reinterpret_cast<const B8*>(&s_b[1][col][k]);
+        bh[j] = *reinterpret_cast<const B8*>(&s_b[buf][0][col][k]);
+        bl[j] = *reinterpret_cast<const B8*>(&s_b[buf][1][col][k]);
       }
 #pragma unroll
       for (int i = 0; i < 2; ++i)
@@ -693,6 +693,17 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
           acci[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a2l[i], bh[j], acci[i][j], 0, 0, 0);
         }
     }
+  };
+  load_chunk(0);
+  stage(0);
+  __syncthreads();
+  int cur = 0;
+  for (int kc = 0; kc < g.kw; kc += KC, cur ^= 1) {
+    const bool more = kc + KC < g.kw;
+    if (more) load_chunk(kc + KC);
+    products(cur);
+    if (more) stage(cur ^ 1);
+    __syncthreads();
   }
 #pragma unroll
   for (int i = 0; i < 2; ++i)
