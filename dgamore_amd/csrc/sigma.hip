@@ -13,6 +13,8 @@
 // to a workspace; pass 2 adds the chunks in a fixed order (deterministic),
 // applies the prefactor, transforms back to k in LDS and writes both halves
 // of the fermionic range.
+#include <algorithm>
+
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
@@ -110,7 +112,78 @@ __global__ void __launch_bounds__(256) sigma_final_kernel(const double2* __restr
   }
 }
 
+// Irreducible q -> list of (irreducible q, orbital transform) entries of the full zone:
+//   out[w][(o1,o2),(o4,o3)][v][j] = sum_p U[o1,p1] U*[o2,p2] U*[o4,p4] U[o3,p3] x[w][(p1,p2),(p4,p3)][v][src[j]]
+// with x = conj(kernel) for entries flagged conj. One thread per output element, j fastest;
+// zero entries of U are skipped, so a signed permutation costs one product per element.
+__global__ void __launch_bounds__(256) kernel_orbital_expand_kernel(const double2* __restrict__ kin,
+                                                                   const int* __restrict__ src,
+                                                                   const double2* __restrict__ u,
+                                                                   const int* __restrict__ conj, double2* __restrict__ out,
+                                                                   int nq, int next, int nb, int nw, int nv) {
+  const int M = nb * nb * nb * nb;
+  const size_t total = static_cast<size_t>(nw) * M * nv * next;
+  for (size_t idx = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x; idx < total;
+       idx += static_cast<size_t>(gridDim.x) * 256) {
+    const int j = static_cast<int>(idx % next);
+    size_t rest = idx / next;
+    const int v = static_cast<int>(rest % nv);
+    rest /= nv;
+    const int m = static_cast<int>(rest % M);
+    const int w = static_cast<int>(rest / M);
+    const int o3 = m % nb, o4 = (m / nb) % nb, o2 = (m / (nb * nb)) % nb, o1 = m / (nb * nb * nb);
+    const int s = src[j];
+    if (static_cast<unsigned>(s) >= static_cast<unsigned>(nq)) {  // invalid entry: NaN, no access
+      out[idx] = make_double2(__builtin_nan(""), __builtin_nan(""));
+      continue;
+    }
+    const bool cj = conj && conj[j];
+    const double2* U = u + static_cast<size_t>(j) * nb * nb;
+    double2 acc = make_double2(0.0, 0.0);
+    for (int p1 = 0; p1 < nb; ++p1) {
+      const double2 a = U[o1 * nb + p1];
+      if (a.x == 0.0 && a.y == 0.0) continue;
+      for (int p2 = 0; p2 < nb; ++p2) {
+        const double2 b = U[o2 * nb + p2];
+        if (b.x == 0.0 && b.y == 0.0) continue;
+        const double2 ab = cmul(a, cconj(b));
+        for (int p4 = 0; p4 < nb; ++p4) {
+          const double2 d = U[o4 * nb + p4];
+          if (d.x == 0.0 && d.y == 0.0) continue;
+          const double2 abd = cmul(ab, cconj(d));
+          for (int p3 = 0; p3 < nb; ++p3) {
+            const double2 c = U[o3 * nb + p3];
+            if (c.x == 0.0 && c.y == 0.0) continue;
+            const int mi = ((p1 * nb + p2) * nb + p4) * nb + p3;
+            double2 x = kin[((static_cast<size_t>(w) * M + mi) * nv + v) * nq + s];
+            if (cj) x = cconj(x);
+            acc = cadd(acc, cmul(cmul(abd, c), x));
+          }
+        }
+      }
+    }
+    out[idx] = acc;
+  }
+}
+
 }  // namespace dgami
+
+extern "C" int dgami_kernel_orbital_expand(int nq, int next, int nb, int nw, int nv, const dgami_c128* kernel_irr,
+                                           const int32_t* src, const dgami_c128* u, const int32_t* conj,
+                                           dgami_c128* kernel_ext, void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(kernel_irr && src && u && kernel_ext, "kernel_orbital_expand: null pointer");
+  DGAMI_ARG_CHECK(nq >= 1 && next >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && nv >= 1,
+                  "kernel_orbital_expand: invalid sizes");
+  const size_t total = static_cast<size_t>(nw) * nb * nb * nb * nb * nv * next;
+  const size_t blocks = std::min<size_t>(ceil_div(total, 256), size_t(1) << 20);
+  hipLaunchKernelGGL(kernel_orbital_expand_kernel, dim3(static_cast<unsigned>(blocks)), dim3(256), 0,
+                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(kernel_irr), src,
+                     reinterpret_cast<const double2*>(u), conj, reinterpret_cast<double2*>(kernel_ext), nq, next, nb,
+                     nw, nv);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
 
 extern "C" int dgami_sigma_from_kernel(int nx, int ny, int nz, int nb, int niw, int niv_core, int niv_g, double beta,
                                        const dgami_c128* kernel_irr, const int32_t* irrk_inv, int nq_irr,

@@ -295,6 +295,42 @@ def sigma_from_kernel(kernel_irr, irrk_inv, giwk, nk, niw, beta):
     return sigma
 
 
+def orbital_expand_tables(irrk_inv, us, conjs=None):
+    """Host tables of the transformed irreducible-to-full-zone map: the distinct (irreducible q,
+    U_k, conj_k) triples of the grid become kernel entries j; returns (src [next], u [next,nb,nb],
+    conj [next] int32, irrk_inv_ext [N]) as numpy arrays, the untransformed entries first."""
+    irrk_inv = np.asarray(irrk_inv)
+    us = np.asarray(us, dtype=np.complex128)
+    conjs = np.zeros(len(irrk_inv), dtype=bool) if conjs is None else np.asarray(conjs, dtype=bool)
+    keys, src, ul, cl = {}, [], [], []
+    inv_ext = np.empty(len(irrk_inv), dtype=np.int32)
+    order = np.argsort(np.abs(us - np.eye(us.shape[-1])).reshape(len(us), -1).sum(1) + conjs, kind="stable")
+    for k in order:  # identity transforms first
+        key = (int(irrk_inv[k]), us[k].tobytes(), bool(conjs[k]))
+        j = keys.get(key)
+        if j is None:
+            j = keys[key] = len(src)
+            src.append(int(irrk_inv[k]))
+            ul.append(us[k])
+            cl.append(int(conjs[k]))
+        inv_ext[k] = j
+    return (np.array(src, dtype=np.int32), np.array(ul), np.array(cl, dtype=np.int32), inv_ext)
+
+
+def kernel_orbital_expand(kernel_irr, src, u, conj=None):
+    """QF kernel on the irreducible q -> QF kernel on the transformed entries (dgami_kernel_orbital_expand).
+    src: int32 [next], u: complex128 [next, nb, nb], conj: int32 [next] or None (device tensors)."""
+    nw, nb4, nv, nq = kernel_irr.shape
+    nb = int(round(nb4 ** 0.25))
+    nxt = int(src.shape[0])
+    if tuple(u.shape) != (nxt, nb, nb):
+        raise ValueError(f"u must be [{nxt},{nb},{nb}], got {tuple(u.shape)}")
+    out = torch.empty((nw, nb4, nv, nxt), dtype=C128, device=kernel_irr.device)
+    _lib.call("dgami_kernel_orbital_expand", nq, nxt, nb, nw, nv, _p(kernel_irr), _p(src, torch.int32), _p(u),
+              _p(conj, torch.int32), _p(out), _stream())
+    return out
+
+
 def hartree_fock(u_ref, v_full, occ, occ_k, nk):
     """(hartree [nb,nb], fock [N,nb,nb]); u_ref [nb]*4, v_full [N,nb,nb,nb,nb] or None (reference order),
     occ [nb,nb], occ_k [N,nb,nb] (complex)."""

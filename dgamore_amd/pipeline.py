@@ -43,7 +43,11 @@ class SDEPipeline:
 
     CHANNELS = (("dens", 1.0), ("magn", 3.0))
 
-    def __init__(self, model, precision=_lib.PLAIN_FP64, symmetries="two_dimensional_square", mixing=1.0):
+    def __init__(self, model, precision=_lib.PLAIN_FP64, symmetries="two_dimensional_square", mixing=1.0,
+                 orbital_transforms="model"):
+        """orbital_transforms: (us [N,nb,nb], conjs [N]) of the irreducible-to-full-zone map (the
+        reference's auto symmetry mode), None for the plain gather, or "model" to take
+        model.orbital_transforms when the model carries them."""
         self.m = model
         self.mixing = mixing
         self.precision = precision
@@ -54,6 +58,14 @@ class SDEPipeline:
         nb = model.nb
         self.irrk_ind = K.to_dev(self.grid.irrk_ind, torch.int32)
         self.irrk_inv = K.to_dev(self.grid.irrk_inv, torch.int32)
+        if isinstance(orbital_transforms, str) and orbital_transforms == "model":
+            orbital_transforms = getattr(model, "orbital_transforms", None)
+        self.expand = None
+        if orbital_transforms is not None:
+            us, conjs = orbital_transforms
+            src, u, cj, inv_ext = K.orbital_expand_tables(self.grid.irrk_inv, us, conjs)
+            self.expand = (K.to_dev(src, torch.int32), K.to_dev(u), K.to_dev(cj, torch.int32))
+            self.irrk_inv_ext = K.to_dev(inv_ext, torch.int32)
         self.ek = K.to_dev(model.ek.reshape(self.N, nb, nb))
         self.gamma = torch.stack([K.vertex_to_cmp(K.to_dev(model.gamma[ch])) for ch, _ in self.CHANNELS]).contiguous()
         self.u_r = torch.stack([K.orb_to_m(K.to_dev(u_as_channel(model.u, ch))) for ch, _ in self.CHANNELS])
@@ -122,7 +134,11 @@ class SDEPipeline:
             K.kernel_r(vrg, chi, self.u_r[c], None, nb, ch, weight, kernel=kern, accumulate=True)
             vrgs.append(vrg)
             chis.append(chi)
-        sig = K.sigma_from_kernel(kern, self.irrk_inv, g, m.nk, m.niw, beta)
+        if self.expand is None:
+            sig = K.sigma_from_kernel(kern, self.irrk_inv, g, m.nk, m.niw, beta)
+        else:
+            sig = K.sigma_from_kernel(K.kernel_orbital_expand(kern, *self.expand), self.irrk_inv_ext, g, m.nk, m.niw,
+                                      beta)
         hart, fock = K.hartree_fock(self.u_ref, None, self.occ, self.occ_k, m.nk)
         old = sigma if sigma_old is None else sigma_old
         if self.mixing != 1.0 and old.shape[0] != self.N:

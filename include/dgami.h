@@ -236,6 +236,19 @@ int dgami_kernel_r(int nq, int nb, int nw, int niv, int channel, double weight, 
 int dgami_dc_kernel(int nq, int nb, int nw, int niv_full, int niv_core, double beta, const dgami_c128* chi0,
                     const dgami_c128* u, const dgami_c128* f_magn_loc, dgami_c128* kernel, void* stream);
 
+/* Kernel entries of the full zone with the per-k orbital transformation of the reference's
+ * auto symmetry mode: entry j is the irreducible kernel src[j] transformed by U_j (and
+ * conjugated when conj[j] != 0),
+ *   ext[w][(o1,o2),(o4,o3)][v][j] = U_o1p1 U*_o2p2 U*_o4p4 U_o3p3 K[w][(p1,p2),(p4,p3)][v][src[j]].
+ * kernel_irr: QF with nq entries; src: [next] int32; u: [next][nb][nb]; conj: [next] int32 or NULL;
+ * kernel_ext: QF with next entries. dgami_sigma_from_kernel then takes kernel_ext with an
+ * irrk_inv that maps every k to its entry. Replaces the orbital step of _map_to_full_bz and
+ * exchange_and_map_irrbz_fullbz (n_point_base.py:462-507, mpi_utils.py:270-290) with
+ * apply_auto_orbital_transform (symmetry_reduction.py:791-903). */
+int dgami_kernel_orbital_expand(int nq, int next, int nb, int nw, int nv, const dgami_c128* kernel_irr,
+                                const int32_t* src, const dgami_c128* u, const int32_t* conj, dgami_c128* kernel_ext,
+                                void* stream);
+
 /* Non-local self-energy from the kernel, R-space FFT form, in the full fermionic range:
  *   Sigma_ij(k, v) = -1/(2 beta N) sum_{q, w in [-niw,niw]} sum_{ad} K_aijd(q, w, v) G_ad(k+q, v-w),
  *   v >= 0, and Sigma(k, -v) = conj(Sigma(k, v)).

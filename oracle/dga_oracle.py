@@ -26,6 +26,9 @@ import scipy.linalg as spla
 # Parity status of each oracle function: name -> how it is pinned (tests/...).
 # -----------------------------------------------------------------------------
 PARITY = {
+    "orbital_expand_fbz": "the kernel of every full-zone q computed directly equals the irreducible one expanded "
+                          "with the orbital transforms on an orbital-symmetric 2- and 3-orbital model; the plain "
+                          "gather does not (test_oracle_sigma.py)",
     "kgrid_irr_maps": "known irreducible counts 6/45/91/153/325/561 + brute-force orbit check (test_oracle_grid.py)",
     "g_full": "closed form: (iv+mu-ek-sigma) @ G == 1 (test_oracle_golden.py)",
     "bubble_fft": "reference-held gchi0_q_rank_0.npy (test_oracle_golden.py) + brute-force k-sum bubble_direct",
@@ -488,6 +491,17 @@ def to_full_niw(x, n_vn_axes):
 def map_irr_to_fbz(x_irr, irrk_inv):
     """K_FBZ[k] = K_irr[irrk_inv[k]] (n_point_base.py:462-507 without orbital transform)."""
     return x_irr[irrk_inv]
+
+
+def orbital_expand_fbz(x_irr, irrk_inv, us, conjs=None):
+    """Irreducible -> full zone with the per-k orbital transformation of the reference's auto
+    mode (n_point_base.py:462-507 with symmetry_reduction.py:791-903, four orbital axes):
+    X_abcd(k) = U_aa' U*_bb' U_cc' U*_dd' [X_a'b'c'd'(rep(k))]^(*conj_k); sigma_k^2 = 1.
+    x_irr: [nq, nb, nb, nb, nb, ...]; us: [N, nb, nb]; conjs: [N] bool or None."""
+    x = map_irr_to_fbz(x_irr, irrk_inv).copy()
+    if conjs is not None:
+        x[conjs] = np.conj(x[conjs])
+    return np.einsum("kap,kbq,kcr,kds,kpqrs...->kabcd...", us, us.conj(), us, us.conj(), x, optimize=True)
 
 
 def sigma_fft(kernel_irr, giwk, irrk_inv, beta, niv_core, niw):
