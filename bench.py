@@ -6,7 +6,9 @@ A step is one full non-local SDE iteration (G(k) -> bubble -> double-counting
 kernel -> ladder solves of both channels -> three-leg vertices and
 susceptibilities -> kernel -> Sigma(k)) on synthetic inputs already resident
 in HBM. The default workload is the 3-orbital t2g model on a 24x24 k-grid
-(91 irreducible q, niw=30, niv=40: 5642 ladder solves of n=720 per step).
+(91 irreducible q, niw=30, niv=40: 5642 ladder solves of n=720 per step), its
+local inputs averaged over the lattice's orbital symmetry and its kernel mapped
+to the full zone with the per-k orbital transforms.
 The JSON line reports (q, w) ladder solves per second of the whole step in
 the headline form (bf16x3: complex64 LU whose trailing updates run as three
 split-BF16 matrix-core products, refined in FP64; mixed: the same with FP32
@@ -60,6 +62,9 @@ def parse():
                          "ladder kernels autotuned, the whole step captured as one HIP graph and replayed "
                          "--steps times (default 10) as chained iterations with linear mixing 0.3")
     ap.add_argument("--batch-seed", type=int, default=180)
+    ap.add_argument("--no-symmetric", action="store_true",
+                    help="keep the local inputs of multi-orbital models as drawn (not averaged over the lattice's "
+                         "orbital symmetry) and map the kernel to the full zone by the plain gather")
     ap.add_argument("--eliashberg", action="store_true",
                     help="instead of the SDE step: the linearised-gap matvec at the 2-orbital 48x48, 2np=128 "
                          "workload with the vertex stored in complex128 and in complex64")
@@ -124,7 +129,7 @@ def eliashberg_matvec(steps, warmup):
                       "rel_err_c64_vs_c128": err}), flush=True)
 
 
-def batch_graph(steps, warmup, seed):
+def batch_graph(steps, warmup, seed, symmetric=True):
     """16 seeded models of uneven sizes in one ModelBatch: autotune every ladder shape, time one eager
     step, capture the step of all models as one HIP graph, check one replay against the eager step,
     then time `steps` chained replays (self-consistency iterations with mixing 0.3 at fixed mu)."""
@@ -137,7 +142,7 @@ def batch_graph(steps, warmup, seed):
     cfgs = S.batch_configs(seed=seed)
     models = []
     for c in cfgs:
-        models.append(S.make_model(c, device=torch.device("cuda")))
+        models.append(S.make_model(c, device=torch.device("cuda"), symmetric=symmetric))
         print(f"[bench] built {c['name']} (n={c['nb'] ** 2 * 2 * c['niv']}) {time.perf_counter() - t0:.1f}s",
               file=sys.stderr, flush=True)
     mb = ModelBatch(models, mixing=0.3)
@@ -375,7 +380,7 @@ def main():
         eliashberg_matvec(args.steps, args.warmup)
         return
     if args.batch:
-        batch_graph(args.steps if "--steps" in sys.argv else 10, args.warmup, args.batch_seed)
+        batch_graph(args.steps if "--steps" in sys.argv else 10, args.warmup, args.batch_seed, not args.no_symmetric)
         return
     import torch
 
@@ -385,7 +390,7 @@ def main():
 
     prec = {"plain": _lib.PLAIN_FP64, "mixed": _lib.MIXED_FP32_IR, "bf16x3": _lib.MIXED_BF16X3_IR}[args.precision]
     t_setup = time.perf_counter()
-    model = S.make_model(args.config)
+    model = S.make_model(args.config, symmetric=not args.no_symmetric)
     pipe = SDEPipeline(model, prec)
     n = model.nb**2 * 2 * model.niv
     print(f"[bench] config {args.config} ({model.name}) nq_irr={pipe.nq} niw={model.niw} niv={model.niv} "
@@ -448,7 +453,9 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": {0: "c128", 1: "c64-LU+c128-refinement", 2: "bf16x3-LU+c128-refinement"}[prec],
-        "data": "synthetic (seeded DMFT-like model, random-perturbation local vertex)",
+        "data": "synthetic (seeded DMFT-like model, random-perturbation local vertex"
+                + (")" if args.no_symmetric or model.orbital_transforms is None
+                   else ", orbital-symmetric, full-zone map with per-k orbital transforms)"),
         "config": {"workload": f"{model.name}: {model.nb} orbitals, k-grid {'x'.join(map(str, model.nk))}, "
                                f"{pipe.nq} irreducible q, niw={model.niw}, niv={model.niv}, n={n}",
                    "nq_irr": pipe.nq, "niw": model.niw, "niv": model.niv, "n_compound": n,
