@@ -81,9 +81,10 @@ def create_pairing_vertices_pp(f_dens, f_magn, niv_pp):
 class PairingOperator:
     """Device-resident linearised gap operator of one channel."""
 
-    def __init__(self, gamma_pp_irr, chi0pp, q_grid, beta, channel, matrix_precision="fp64"):
+    def __init__(self, gamma_pp_irr, chi0pp, q_grid, beta, channel, matrix_precision="fp64", orbital_transforms=None):
         """matrix_precision "fp32" stores the transformed vertex in complex64 (half the bytes per
-        application); the vectors and the sums stay complex128."""
+        application); the vectors and the sums stay complex128. orbital_transforms: per-k (us,
+        conjs) of the irreducible-zone map (the reference's auto symmetry mode) or None."""
         import torch
 
         self.nk = tuple(q_grid.nk)
@@ -93,7 +94,7 @@ class PairingOperator:
         self.nb, self.np2 = gam.shape[1], gam.shape[-1]
         storage = {"fp64": torch.complex128, "fp32": torch.complex64}[matrix_precision]
         self.g_hat, self.g_flip = K.pairing_setup(K.to_dev(gam), K.to_dev(np.asarray(q_grid.irrk_inv), torch.int32),
-                                                  self.nk, channel, storage)
+                                                  self.nk, channel, storage, orbital_transforms)
         n_k = int(np.prod(self.nk))
         self.chi0pp = K.to_dev(np.asarray(chi0pp).reshape(n_k, self.nb**4, self.np2))
         self.shape = self.nk + (self.nb, self.nb, self.np2)

@@ -381,8 +381,19 @@ def chi0_pp(giwk, niv_pp):
     return out
 
 
-def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel, storage=C128):
-    """gamma_pp_irr [nq,nb,nb,nb,nb,np2,np2] -> (g_hat, g_hat_flip) [N][n][n] in complex128 or complex64."""
+def pairing_setup(gamma_pp_irr, irrk_inv, nk, channel, storage=C128, orbital_transforms=None):
+    """gamma_pp_irr [nq,nb,nb,nb,nb,np2,np2] -> (g_hat, g_hat_flip) [N][n][n] in complex128 or complex64.
+    orbital_transforms: per-k (us [N,nb,nb], conjs [N]) of the irreducible-zone map (the reference's
+    auto symmetry mode) or None for the plain gather; with them the vertex is first expanded to
+    its distinct transformed entries (dgami_kernel_orbital_expand)."""
+    if orbital_transforms is not None:
+        nq0, nb0, np0 = gamma_pp_irr.shape[0], gamma_pp_irr.shape[1], gamma_pp_irr.shape[-1]
+        src, u, cj, inv_ext = orbital_expand_tables(np.asarray(irrk_inv.cpu() if hasattr(irrk_inv, "cpu")
+                                                               else irrk_inv), *orbital_transforms)
+        qf = ref4_to_qf(gamma_pp_irr.reshape((nq0,) + (nb0,) * 4 + (1, np0 * np0)))
+        ext = kernel_orbital_expand(qf, to_dev(src, torch.int32), to_dev(u), to_dev(cj, torch.int32))
+        gamma_pp_irr = qf_to_ref4(ext, nb0).reshape((len(src),) + (nb0,) * 4 + (np0, np0)).contiguous()
+        irrk_inv = to_dev(inv_ext, torch.int32)
     nx, ny, nz = nk
     nq, nb, np2 = gamma_pp_irr.shape[0], gamma_pp_irr.shape[1], gamma_pp_irr.shape[-1]
     n = nb * nb * np2

@@ -73,3 +73,25 @@ def test_reduced_step_with_transforms_equals_full_zone_step(K, cfg):
     assert rel(red.last["sigma"], full.last["sigma"]) < 1e-10
     # the plain gather (the reference's named-symmetry mode) misses the orbital permutation
     assert rel(plain.last["sigma_nl"], full.last["sigma_nl"]) > 1e-4
+
+
+def test_pairing_setup_with_transforms_equals_expanded_vertex(K):
+    """The Eliashberg operator built from the irreducible pp vertex with orbital transforms equals
+    the one built from the vertex expanded on the host by the oracle (every k its own entry)."""
+    say("pairing_setup with orbital transforms, nb=2 nk=6x6 np2=6")
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.brillouin_zone import KGrid, orbital_transforms
+
+    nk, nb, np2 = (6, 6, 1), 2, 6
+    grid = KGrid(nk)
+    us, conjs = orbital_transforms(S.dispersion_eg(nk), grid)
+    rng = np.random.default_rng(8)
+    shape = (grid.nk_irr,) + (nb,) * 4 + (np2, np2)
+    gam = rng.standard_normal(shape) + 1j * rng.standard_normal(shape)
+    full = O.orbital_expand_fbz(gam, grid.irrk_inv, us, conjs)
+    ident = K.to_dev(np.arange(grid.nk_tot), torch.int32)
+    for ch in ("sing", "trip"):
+        a = K.pairing_setup(K.to_dev(gam), K.to_dev(grid.irrk_inv, torch.int32), nk, ch, orbital_transforms=(us, conjs))
+        b = K.pairing_setup(K.to_dev(full), ident, nk, ch)
+        torch.cuda.synchronize()
+        assert rel(a[0], b[0]) < 1e-13 and rel(a[1], b[1]) < 1e-13
