@@ -20,13 +20,14 @@ def apply_lambda(chi_r, lambda_):
     return 1.0 / (1.0 / chi_r + lambda_)
 
 
-def find_lambda(chi_r_mat, chi_r_loc_sum, delta=0.1, eps=1e-7, maxiter=1000):
-    """Newton-like iteration with step halving (lambda_correction.py:28-60), weights irrk_count."""
+def find_lambda_weighted(chi_r_mat, chi_r_loc_sum, counts, beta, nk_tot, delta=0.1, eps=1e-7, maxiter=1000):
+    """Newton-like iteration with step halving (lambda_correction.py:28-60): the lambda with
+    sum_q counts[q] chi/(1 + lambda chi) / (beta nk_tot) = chi_r_loc_sum; chi_r_mat [q_irr, 2niw+1]."""
     chi_r_mat = np.asarray(chi_r_mat, dtype=np.complex128)
-    counts = np.asarray(config.lattice.q_grid.irrk_count)[:, None]
+    counts = np.asarray(counts)[:, None]
     lambda_start = get_lambda_start(chi_r_mat)
     lambda_ = lambda_start + delta
-    factor = 1.0 / config.sys.beta / config.lattice.q_grid.nk_tot
+    factor = 1.0 / beta / nk_tot
     for _ in range(maxiter):
         chi_lam = apply_lambda(chi_r_mat, lambda_)
         f_lam = (counts * chi_lam).sum() * factor - chi_r_loc_sum
@@ -40,6 +41,12 @@ def find_lambda(chi_r_mat, chi_r_loc_sum, delta=0.1, eps=1e-7, maxiter=1000):
         else:
             lambda_ = lambda_new
     return lambda_
+
+
+def find_lambda(chi_r_mat, chi_r_loc_sum, delta=0.1, eps=1e-7, maxiter=1000):
+    """find_lambda_weighted with the irreducible weights, beta and grid size of the module config."""
+    return find_lambda_weighted(chi_r_mat, chi_r_loc_sum, config.lattice.q_grid.irrk_count, config.sys.beta,
+                                config.lattice.q_grid.nk_tot, delta, eps, maxiter)
 
 
 def perform_single_lambda_correction(chi_r, chi_r_loc_sum):

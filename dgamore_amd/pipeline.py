@@ -44,10 +44,24 @@ class SDEPipeline:
     CHANNELS = (("dens", 1.0), ("magn", 3.0))
 
     def __init__(self, model, precision=_lib.PLAIN_FP64, symmetries="two_dimensional_square", mixing=1.0,
-                 orbital_transforms="model"):
+                 orbital_transforms="model", lambda_correction=None, chi_loc_sums=None):
         """orbital_transforms: (us [N,nb,nb], conjs [N]) of the irreducible-to-full-zone map (the
         reference's auto symmetry mode), None for the plain gather, or "model" to take
-        model.orbital_transforms when the model carries them."""
+        model.orbital_transforms when the model carries them.
+        lambda_correction: None, "spch" (both channels) or "sp" (magnetic channel) correction of the
+        physical susceptibility before the kernel (single band, nonlocal_sde.py:319-320, 339-401);
+        chi_loc_sums: {"dens": ..., "magn": ...} sums of the local physical susceptibilities over the
+        full bosonic range."""
+        if lambda_correction is not None:
+            if lambda_correction not in ("spch", "sp"):
+                raise ValueError("Lambda correction type must be either 'spch' or 'sp'.")
+            if model.nb != 1:
+                raise ValueError("the lambda correction is defined for single-band models")
+            if chi_loc_sums is None:
+                raise ValueError("the lambda correction needs the local susceptibility sums")
+        self.lambda_correction = lambda_correction
+        self.chi_loc_sums = chi_loc_sums
+        self.lambdas = {}
         self.m = model
         self.mixing = mixing
         self.precision = precision
@@ -131,6 +145,8 @@ class SDEPipeline:
         vrgs, chis = [], []
         for c, (ch, weight) in enumerate(self.CHANNELS):
             vrg, chi = K.vrg_chi_phys(inv, aux[c], full, core, self.u_r[c], None, nb, beta)
+            if self.lambda_correction == "spch" or (self.lambda_correction == "sp" and ch == "magn"):
+                self.lambdas[ch] = self._lambda_correct(ch, chi, chis)
             K.kernel_r(vrg, chi, self.u_r[c], None, nb, ch, weight, kernel=kern, accumulate=True)
             vrgs.append(vrg)
             chis.append(chi)
@@ -149,6 +165,26 @@ class SDEPipeline:
             self.last = dict(g=g, chi0=chi0, chi0_inv=inv, full=full, core=core, aux=aux, info=info, vrg=vrgs,
                              chi_phys=chis, kernel=kern, sigma_nl=sig, hartree=hart, fock=fock, sigma=new)
         return new
+
+    def _lambda_correct(self, ch, chi, chis):
+        """Replace chi (QW [nw][1][nq], in place) by 1/(1/chi + lambda); lambda from the FP64 root
+        find on the host (nonlocal_sde.py:339-401, lambda_correction.py:28-73)."""
+        from dgamore_amd.lambda_correction import apply_lambda, find_lambda_weighted
+
+        def full_range(x):  # [nq, 2niw+1], chi(-w) = conj chi(w)
+            half = x[:, 0, :].T.cpu().numpy()
+            return np.concatenate([np.conj(half[:, :0:-1]), half], axis=1)
+
+        counts, beta = self.grid.irrk_count, self.m.beta
+        if self.lambda_correction == "spch":
+            target = self.chi_loc_sums[ch] / beta
+        else:
+            dens = full_range(chis[0])
+            target = (self.chi_loc_sums["dens"] + self.chi_loc_sums["magn"]
+                      - (counts[:, None] * dens).sum() / self.N) / beta
+        lam = find_lambda_weighted(full_range(chi), target, counts, beta, self.N)
+        chi.copy_(K.to_dev(apply_lambda(chi.cpu().numpy(), lam)))
+        return lam
 
     # -- HIP graph of one step ------------------------------------------------------------------
     def capture(self, mu=None, chain=False):

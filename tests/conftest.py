@@ -62,3 +62,25 @@ def ladder_case(nb, niv, niw, nq, seed, gamma=None, beta=10.0, with_v=True):
     v = 0.3 * (rng.standard_normal((nq,) + (nb,) * 4) + 1j * rng.standard_normal((nq,) + (nb,) * 4)) if with_v \
         else np.zeros((nq,) + (nb,) * 4)
     return chi0_inv, gamma, u, v, beta
+
+
+def oracle_kernel(m, g, irrk_ind, lambdas=None):
+    """K(q) = -K_dc + K_dens + 3 K_magn of the listed q-points by the oracle (the chain of
+    nonlocal_sde.py:777-857); lambdas: {channel: lambda} applied to the physical susceptibility."""
+    from oracle import dga_oracle as O
+
+    nq, nb = len(irrk_ind), m.nb
+    chi0 = O.bubble_fft(g, m.niw, m.niv, irrk_ind, m.beta)
+    inv = O.block_invert_vn1(chi0)
+    kern = -O.dc_kernel(m.f_magn, chi0, m.u, m.beta, m.niv)
+    zero_v = np.zeros((nq,) + (nb,) * 4)
+    for ch, wgt in (("dens", 1.0), ("magn", 3.0)):
+        ur = O.u_as_channel(m.u, ch)
+        aux = O.ladder_sum_solve(inv, m.gamma[ch], ur, zero_v, m.beta)
+        vrg = O.vrg(inv, aux, m.beta)
+        chi = O.chi_phys_shell(O.sum_over_vn(aux, m.beta), chi0.sum(-1) / m.beta**2, chi0.sum(-1) / m.beta**2, ur,
+                               zero_v)
+        if lambdas and ch in lambdas:
+            chi = O.apply_lambda(chi, lambdas[ch])
+        kern = kern + wgt * O.kernel_r(vrg, chi, ur, zero_v, ch)
+    return kern

@@ -115,3 +115,50 @@ def test_pipeline_iterate_conserves_filling(K):
     smom0, _ = O.fit_smom(s, m.beta)
     assert abs(O.total_fill(hist[-1], m.ek, s, m.beta, smom0) - n0) < 1e-5
     assert abs(gf.get_total_fill(hist[-1], m.ek, s, m.beta, fit_smom(s, m.beta)[0]) - n0) < 1e-5
+
+
+@pytest.mark.parametrize("kind", ["spch", "sp"])
+def test_lambda_corrected_step_matches_oracle(K, kind):
+    """The step with the lambda correction of the physical susceptibility: lambda equals the
+    oracle's root of the same sum rule, and Sigma equals the oracle chain with chi_lambda."""
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import SDEPipeline
+    from tests.conftest import oracle_kernel
+
+    say(f"lambda-corrected SDE step ({kind}), config 0 model 8x8 niw=4 niv=6")
+    m = S.make_model(0, nk=(8, 8, 1), niw=4, niv=6)
+    p0 = SDEPipeline(m)
+    p0.step(keep=True)
+    torch.cuda.synchronize()
+    counts, N = p0.grid.irrk_count, p0.N
+
+    def full(x):
+        half = x[:, 0, :].T.cpu().numpy()
+        return np.concatenate([np.conj(half[:, :0:-1]), half], axis=1)
+
+    chi_d, chi_m = (full(c) for c in p0.last["chi_phys"])
+    s_d = (counts[:, None] * chi_d).sum() / (m.beta * N)
+    s_m = (counts[:, None] * chi_m).sum() / (m.beta * N)
+    # local sums with a root below the uncorrected lattice sums
+    sums = {"dens": 0.9 * s_d * m.beta, "magn": 0.8 * s_m * m.beta}
+    p1 = SDEPipeline(m, lambda_correction=kind, chi_loc_sums=sums)
+    p1.step(keep=True)
+    torch.cuda.synchronize()
+    ref_l = {}
+    if kind == "spch":
+        ref_l["dens"] = O.find_lambda(chi_d, sums["dens"] / m.beta, counts, m.beta, N)
+    target_m = sums["magn"] / m.beta if kind == "spch" else \
+        (sums["dens"] + sums["magn"] - (counts[:, None] * chi_d).sum() / N) / m.beta
+    ref_l["magn"] = O.find_lambda(chi_m, target_m, counts, m.beta, N)
+    assert set(p1.lambdas) == set(ref_l)
+    for ch, lam in ref_l.items():
+        assert abs(p1.lambdas[ch] - lam) < 1e-10 * max(1.0, abs(lam))
+    # the corrected magnetic susceptibility satisfies its sum rule
+    chi_ml = full(p1.last["chi_phys"][1])
+    assert abs((counts[:, None] * chi_ml).sum().real / (m.beta * N) - target_m.real) < 1e-6
+    maps = O.kgrid_irr_maps(m.nk)
+    g = m.giwk()
+    kern = oracle_kernel(m, g, maps["irrk_ind"], ref_l)
+    sig_ref = O.sigma_fft(kern, g, maps["irrk_inv"], m.beta, m.niv, m.niw)
+    assert rel(p1.last["sigma_nl"], sig_ref) < 1e-9
+    assert rel(p0.last["sigma_nl"], sig_ref) > 1e-6

@@ -3,6 +3,7 @@ import numpy as np
 import pytest
 
 from oracle import dga_oracle as O
+from tests.conftest import oracle_kernel
 
 
 def test_dc_kernel_brute_force():
@@ -96,23 +97,6 @@ def test_to_full_niw_symmetry():
     assert np.allclose(full[:, 0, :], np.conj(x[:, 2, ::-1]))
 
 
-def _kernel_oracle(m, g, irrk_ind):
-    """K(q) = -K_dc + K_dens + 3 K_magn of the listed q-points (the chain of nonlocal_sde.py:777-857)."""
-    nq, nb = len(irrk_ind), m.nb
-    chi0 = O.bubble_fft(g, m.niw, m.niv, irrk_ind, m.beta)
-    inv = O.block_invert_vn1(chi0)
-    kern = -O.dc_kernel(m.f_magn, chi0, m.u, m.beta, m.niv)
-    zero_v = np.zeros((nq,) + (nb,) * 4)
-    for ch, wgt in (("dens", 1.0), ("magn", 3.0)):
-        ur = O.u_as_channel(m.u, ch)
-        aux = O.ladder_sum_solve(inv, m.gamma[ch], ur, zero_v, m.beta)
-        vrg = O.vrg(inv, aux, m.beta)
-        chi = O.chi_phys_shell(O.sum_over_vn(aux, m.beta), chi0.sum(-1) / m.beta**2, chi0.sum(-1) / m.beta**2, ur,
-                               zero_v)
-        kern = kern + wgt * O.kernel_r(vrg, chi, ur, zero_v, ch)
-    return kern
-
-
 @pytest.mark.parametrize("cfg,nk", [(4, (4, 4, 1)), (2, (4, 4, 1))])
 def test_orbital_expansion_of_the_kernel_matches_full_zone(cfg, nk):
     """On an orbital-symmetric e_g (2 orbitals) or t2g (3 orbitals) model the kernel computed at
@@ -124,8 +108,8 @@ def test_orbital_expansion_of_the_kernel_matches_full_zone(cfg, nk):
     us, conjs = m.orbital_transforms
     g = m.giwk()
     maps = O.kgrid_irr_maps(nk)
-    k_all = _kernel_oracle(m, g, np.arange(int(np.prod(nk))))
-    k_irr = _kernel_oracle(m, g, maps["irrk_ind"])
+    k_all = oracle_kernel(m, g, np.arange(int(np.prod(nk))))
+    k_irr = oracle_kernel(m, g, maps["irrk_ind"])
     expanded = O.orbital_expand_fbz(k_irr, maps["irrk_inv"], us, conjs)
     scale = np.abs(k_all).max()
     assert np.abs(expanded - k_all).max() / scale < 1e-10
