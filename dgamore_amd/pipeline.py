@@ -83,6 +83,15 @@ class SDEPipeline:
         self.ek = K.to_dev(model.ek.reshape(self.N, nb, nb))
         self.gamma = torch.stack([K.vertex_to_cmp(K.to_dev(model.gamma[ch])) for ch, _ in self.CHANNELS]).contiguous()
         self.u_r = torch.stack([K.orb_to_m(K.to_dev(u_as_channel(model.u, ch))) for ch, _ in self.CHANNELS])
+        # non-local V(q) [nx,ny,nz,nb,nb,nb,nb] (reference order) when the model has one: V_dens = 2V,
+        # V_magn = 0 on the irreducible q (interaction.py:160-182), the full grid for Fock
+        v = getattr(model, "v", None)
+        self.v_full = self.v_r = None
+        if v is not None:
+            v = np.asarray(v).reshape((self.N,) + (nb,) * 4)
+            self.v_full = K.to_dev(v)
+            v_irr = K.orb_to_m(K.to_dev(v[self.grid.irrk_ind]))
+            self.v_r = torch.stack([2.0 * v_irr, torch.zeros_like(v_irr)]).contiguous()
         self.u_ref = K.to_dev(model.u)
         self.f_magn = K.to_dev(model.f_magn)
         self.sigma_dmft = K.to_dev(model.sigma_loc[None])  # local Sigma as [1][nb][nb][2 niv_s]
@@ -113,7 +122,7 @@ class SDEPipeline:
         sigma_old defaults to sigma."""
         sigma = self.sigma_dmft if sigma is None else sigma
         front = self.front(sigma, mu)
-        aux, info = K.ladder_solve_sum(front["chi0_inv"], self.gamma, self.u_r, None, self.m.beta, self.precision,
+        aux, info = K.ladder_solve_sum(front["chi0_inv"], self.gamma, self.u_r, self.v_r, self.m.beta, self.precision,
                                        check_info=False)
         return self.back(front, aux, info, sigma, sigma_old, keep)
 
@@ -133,7 +142,7 @@ class SDEPipeline:
 
     def ladder_model(self, front):
         """The ladder arguments of this model for kernels.ladder_solve_sum_multi."""
-        return (front["chi0_inv"], self.gamma, self.u_r, None, self.m.beta)
+        return (front["chi0_inv"], self.gamma, self.u_r, self.v_r, self.m.beta)
 
     def back(self, front, aux, info, sigma, sigma_old=None, keep=False):
         """From the ladder sums to the new, mixed Sigma: vrg and chi, the kernel, the Sigma
@@ -144,10 +153,11 @@ class SDEPipeline:
         chi0 = front["chi0"]
         vrgs, chis = [], []
         for c, (ch, weight) in enumerate(self.CHANNELS):
-            vrg, chi = K.vrg_chi_phys(inv, aux[c], full, core, self.u_r[c], None, nb, beta)
+            v_c = None if self.v_r is None else self.v_r[c]
+            vrg, chi = K.vrg_chi_phys(inv, aux[c], full, core, self.u_r[c], v_c, nb, beta)
             if self.lambda_correction == "spch" or (self.lambda_correction == "sp" and ch == "magn"):
                 self.lambdas[ch] = self._lambda_correct(ch, chi, chis)
-            K.kernel_r(vrg, chi, self.u_r[c], None, nb, ch, weight, kernel=kern, accumulate=True)
+            K.kernel_r(vrg, chi, self.u_r[c], v_c, nb, ch, weight, kernel=kern, accumulate=True)
             vrgs.append(vrg)
             chis.append(chi)
         if self.expand is None:
@@ -155,7 +165,7 @@ class SDEPipeline:
         else:
             sig = K.sigma_from_kernel(K.kernel_orbital_expand(kern, *self.expand), self.irrk_inv_ext, g, m.nk, m.niw,
                                       beta)
-        hart, fock = K.hartree_fock(self.u_ref, None, self.occ, self.occ_k, m.nk)
+        hart, fock = K.hartree_fock(self.u_ref, self.v_full, self.occ, self.occ_k, m.nk)
         old = sigma if sigma_old is None else sigma_old
         if self.mixing != 1.0 and old.shape[0] != self.N:
             old = old.expand(self.N, -1, -1, -1).contiguous()

@@ -40,6 +40,7 @@ class Model:
     f_magn: np.ndarray = None  # [nb]*4+[niw+1, 2niv_full, 2niv_full]
     seed: int = 0
     orbital_transforms: tuple = None  # (us [N,nb,nb], conjs [N]) of the irreducible-zone map, or None
+    v: np.ndarray = None  # non-local V(q) [nx,ny,nz,nb,nb,nb,nb] (reference order), or None
 
     @property
     def niv_full(self):

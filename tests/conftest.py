@@ -66,21 +66,24 @@ def ladder_case(nb, niv, niw, nq, seed, gamma=None, beta=10.0, with_v=True):
 
 def oracle_kernel(m, g, irrk_ind, lambdas=None):
     """K(q) = -K_dc + K_dens + 3 K_magn of the listed q-points by the oracle (the chain of
-    nonlocal_sde.py:777-857); lambdas: {channel: lambda} applied to the physical susceptibility."""
+    nonlocal_sde.py:777-857); lambdas: {channel: lambda} applied to the physical susceptibility;
+    m.v (if set): the non-local V(q) on the full grid."""
     from oracle import dga_oracle as O
 
     nq, nb = len(irrk_ind), m.nb
+    v_q = None if getattr(m, "v", None) is None else m.v.reshape((-1,) + (nb,) * 4)[irrk_ind]
     chi0 = O.bubble_fft(g, m.niw, m.niv, irrk_ind, m.beta)
     inv = O.block_invert_vn1(chi0)
     kern = -O.dc_kernel(m.f_magn, chi0, m.u, m.beta, m.niv)
     zero_v = np.zeros((nq,) + (nb,) * 4)
     for ch, wgt in (("dens", 1.0), ("magn", 3.0)):
         ur = O.u_as_channel(m.u, ch)
-        aux = O.ladder_sum_solve(inv, m.gamma[ch], ur, zero_v, m.beta)
+        vr = zero_v if v_q is None else O.v_as_channel(v_q, ch)
+        aux = O.ladder_sum_solve(inv, m.gamma[ch], ur, vr, m.beta)
         vrg = O.vrg(inv, aux, m.beta)
         chi = O.chi_phys_shell(O.sum_over_vn(aux, m.beta), chi0.sum(-1) / m.beta**2, chi0.sum(-1) / m.beta**2, ur,
-                               zero_v)
+                               vr)
         if lambdas and ch in lambdas:
             chi = O.apply_lambda(chi, lambdas[ch])
-        kern = kern + wgt * O.kernel_r(vrg, chi, ur, zero_v, ch)
+        kern = kern + wgt * O.kernel_r(vrg, chi, ur, vr, ch)
     return kern

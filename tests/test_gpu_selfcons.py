@@ -162,3 +162,31 @@ def test_lambda_corrected_step_matches_oracle(K, kind):
     sig_ref = O.sigma_fft(kern, g, maps["irrk_inv"], m.beta, m.niv, m.niw)
     assert rel(p1.last["sigma_nl"], sig_ref) < 1e-9
     assert rel(p0.last["sigma_nl"], sig_ref) > 1e-6
+
+
+def test_step_with_nonlocal_interaction_matches_oracle(K):
+    """Extended Hubbard model: V(q) = V0 (cos qx + cos qy) enters the ladder (V_dens = 2V), the
+    susceptibility, the kernel and the Fock term."""
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import SDEPipeline
+    from tests.conftest import oracle_kernel
+
+    say("SDE step with non-local V(q), config 0 model 8x8 niw=4 niv=6")
+    m = S.make_model(0, nk=(8, 8, 1), niw=4, niv=6)
+    qx = 2 * np.pi * np.arange(8) / 8
+    m.v = (0.6 * (np.cos(qx)[:, None] + np.cos(qx)[None, :])).reshape(8, 8, 1, 1, 1, 1, 1).astype(complex)
+    p = SDEPipeline(m)
+    p.step(keep=True)
+    torch.cuda.synchronize()
+    maps = O.kgrid_irr_maps(m.nk)
+    g = m.giwk()
+    sig_ref = O.sigma_fft(oracle_kernel(m, g, maps["irrk_ind"]), g, maps["irrk_inv"], m.beta, m.niv, m.niw)
+    assert rel(p.last["sigma_nl"], sig_ref) < 1e-10
+    occ_k = p.occ_k.cpu().numpy().reshape(8, 8, 1, 1, 1)
+    h_ref, f_ref = O.hartree_fock(m.u, m.v, p.occ.cpu().numpy(), occ_k)
+    assert rel(p.last["hartree"], h_ref) < 1e-12
+    assert rel(p.last["fock"], f_ref.reshape(64, 1, 1)) < 1e-12
+    m.v = None
+    p0 = SDEPipeline(m)
+    p0.step(keep=True)
+    assert rel(p0.last["sigma_nl"], sig_ref) > 1e-6
