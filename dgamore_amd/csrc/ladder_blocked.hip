@@ -772,21 +772,29 @@ struct BlockedSteps {
     if (r1 <= r0 || c1 <= c0 || kw <= 0) return 0;
     int mode = BLK_GEMM_VECTOR;
     if constexpr (std::is_same<S, float2>::value) mode = use_vector_gemm() ? BLK_GEMM_VECTOR : st.gemm_mode;
-    // complex128 vector tiles are 128 x 64 (8 rows per thread), the others 64 x 64
-    constexpr bool tall = std::is_same<S, double2>::value;
-    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw, tall && mode == BLK_GEMM_VECTOR ? 128 : 64);
+    // complex128 vector tiles are 128 x 64 (8 rows per thread) when the update has more than 96
+    // rows, 64 x 64 otherwise (a 128-row tile over the short updates of small n is mostly empty);
+    // the complex64 tiles are 64 x 64
+    constexpr bool c128 = std::is_same<S, double2>::value;
+    const bool tall = c128 && mode == BLK_GEMM_VECTOR && r1 - r0 > 96;
+    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw, tall ? 128 : 64);
     const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
     DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
     hipEvent_t start = nullptr;
     const bool prof = profile_on();
     if (prof && profile_begin(stream, &start)) return -2;
     if (mode == BLK_GEMM_VECTOR) {
-      if constexpr (tall)
-        hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
-                           nb);
-      else
+      if constexpr (c128) {
+        if (tall)
+          hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st,
+                             g, nb);
+        else
+          hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 4>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st,
+                             g, nb);
+      } else {
         hipLaunchKernelGGL((blk_gemm_kernel<S, 32, 4>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
                            nb);
+      }
     } else if constexpr (std::is_same<S, float2>::value) {
       if (mode == BLK_GEMM_BF16X3)
         hipLaunchKernelGGL(blk_gemm_bf16x3_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
