@@ -25,6 +25,7 @@ struct BlockedState {
   int swap_left;   // also apply interchanges left of the panel (needed when L is reused)
   int gemm_mode;   // BlockedGemm (complex64 only)
   int variant;     // blocked-LU variant (panel width and rows per thread), -1: chosen by n
+  int tall_tiles;  // complex128: 128-row update tiles for updates of more than 96 rows (0: 64-row tiles)
 };
 
 // Allocate the workspace of `batch` problems in slots ws and ws+1; returns false on failure.
@@ -45,6 +46,7 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
   st.swap_left = 0;
   st.gemm_mode = BLK_GEMM_F32_MFMA;
   st.variant = -1;
+  st.tall_tiles = 0;
   return true;
 }
 
