@@ -232,6 +232,51 @@ def test_ladder_two_channels_realistic_vertex(K):
     assert rel(got[1], O.ladder_sum_solve(chi0_inv, gm, um, vm, beta)) < RTOL
 
 
+@pytest.mark.parametrize("precision", [0, 2])
+def test_ladder_largest_compound_size(K, precision):
+    """n = 1536, the largest size of the blocked LU (2 orbitals, niv = 192), one (q, w) problem."""
+    say(f"ladder n=1536 (maximum), precision {precision}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(2, 192, 0, 1, seed=1536)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=precision)
+    assert int(info.abs().sum()) == 0
+    assert rel(got[0], ref) < (RTOL if precision == 0 else 1e-8)
+
+
+def test_ladder_above_largest_size_is_refused(K):
+    say("ladder n=1552 (above the maximum) -> ValueError, no launch")
+    nb, niv = 2, 194
+    n = nb * nb * 2 * niv
+    chi0_inv = torch.zeros((1, nb**4, 2 * niv, 1), dtype=torch.complex128, device="cuda")
+    gamma = torch.zeros((1, 1, n, n), dtype=torch.complex128, device="cuda")
+    u = torch.zeros((1, nb**4), dtype=torch.complex128, device="cuda")
+    for precision in (0, 2):
+        with pytest.raises(ValueError):
+            K.ladder_solve_sum(chi0_inv, gamma, u, None, 10.0, precision)
+
+
+def test_ladder_four_orbitals(K):
+    """nb = 4 (16 right-hand sides), the most orbitals the kernels take, in both forms."""
+    say("ladder nb=4 niv=4 (n=128, 16 right-hand sides), plain and split-BF16")
+    chi0_inv, gamma, u, v, beta = _ladder_case(4, 4, 1, 2, seed=44)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    for precision, tol in ((0, RTOL), (2, 1e-8)):
+        got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=precision)
+        assert int(info.abs().sum()) == 0
+        assert rel(got[0], ref) < tol
+
+
+def test_ladder_five_orbitals_refused(K):
+    say("ladder nb=5 -> ValueError")
+    nb, niv = 5, 1
+    n = nb * nb * 2 * niv
+    chi0_inv = torch.zeros((1, nb**4, 2 * niv, 1), dtype=torch.complex128, device="cuda")
+    gamma = torch.zeros((1, 1, n, n), dtype=torch.complex128, device="cuda")
+    u = torch.zeros((1, nb**4), dtype=torch.complex128, device="cuda")
+    with pytest.raises(ValueError):
+        K.ladder_solve_sum(chi0_inv, gamma, u, None, 10.0)
+
+
 def test_ladder_singular_raises(K):
     say("ladder singular matrix -> LinAlgError")
     nb, niv, nw, nq = 1, 4, 1, 2
