@@ -40,7 +40,7 @@ bool blocked_variant_fits(bool complex64, int variant, int n) {
   if (n < 1 || n > BLK_MAX_N) return false;
   if (variant == 0) return n <= (complex64 ? 1152 : 768);
   if (variant == 1) return true;
-  if (!complex64) return variant == 3 || (variant == 2 && n <= 768);
+  if (!complex64) return variant == 3 || variant == 4 || (variant == 2 && n <= 768);
   return variant == 3 || (variant == 2 && n <= 1152) || (variant == 4 && n <= 1024);
 }
 

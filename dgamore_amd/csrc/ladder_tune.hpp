@@ -16,6 +16,7 @@ constexpr int ladder_kernel_variant(int id) { return id % 256; }
 // Blocked-LU variants (panel width NB, rows per panel thread, panels per outer block):
 //   complex128: 0 = NB 32, one row per thread, 2 panels (n <= 768); 1 = NB 16, three rows, 2 panels
 //               2, 3 = 0, 1 with 128-row update tiles for long updates (64-row tiles otherwise)
+//               4 = NB 16, three rows, 4 panels (64-column outer blocks for n > 768)
 //   complex64:  0 = NB 32, three rows, 4 panels (n <= 1152);        1 = NB 16, three rows, 4 panels
 //               2 = NB 32, three rows, 8 panels (n <= 1152);        3 = NB 16, three rows, 8 panels
 //               4 = NB 32, two rows, 4 panels (n <= 1024)

@@ -45,9 +45,10 @@ def test_ladder_tune_table_without_gpu():
     A.clear()
     names = lambda n, t, p: [A.kernel_name(k) for k in A.candidates(n, t, p)]  # noqa: E731
     assert names(16, 1, _lib.PLAIN_FP64)[:2] == ["reg0", "reg1"] and "blocked0" not in names(16, 1, 0)
-    assert names(120, 1, _lib.PLAIN_FP64) == ["reg3", "col3", "blocked0", "blocked1", "blocked2", "blocked3"]
-    assert names(720, 9, _lib.PLAIN_FP64) == ["blocked0", "blocked1", "blocked2", "blocked3"]
-    assert names(1080, 9, _lib.PLAIN_FP64) == ["blocked1", "blocked3"]  # one row per panel thread stops at 768
+    assert names(120, 1, _lib.PLAIN_FP64) == ["reg3", "col3", "blocked0", "blocked1", "blocked2", "blocked3",
+                                              "blocked4"]
+    assert names(720, 9, _lib.PLAIN_FP64) == ["blocked0", "blocked1", "blocked2", "blocked3", "blocked4"]
+    assert names(1080, 9, _lib.PLAIN_FP64) == ["blocked1", "blocked3", "blocked4"]  # one row per thread: n <= 768
     assert names(720, 9, _lib.MIXED_BF16X3_IR) == ["blocked0", "blocked1", "blocked2", "blocked3", "blocked4"]
     assert names(1080, 9, _lib.MIXED_BF16X3_IR) == ["blocked0", "blocked1", "blocked2", "blocked3"]
     assert names(1600, 1, _lib.PLAIN_FP64) == []
