@@ -26,6 +26,34 @@ import scipy.linalg as spla
 # Parity status of each oracle function: name -> how it is pinned (tests/...).
 # -----------------------------------------------------------------------------
 PARITY = {
+    # helpers (layouts, grids, channel algebra): pinned through the functions that use them
+    "vn": "closed form pi/beta (2n+1); with g_full/bubble_fft reproduces the reference-held gchi0_q_rank_0.npy",
+    "vn_int": "index helper of vn (same pins)",
+    "wn_int": "index helper of the bosonic axis (pinned with bubble_fft)",
+    "read_hk": "reads the reference's wannier.hk; with bubble_fft reproduces gchi0_q_rank_0.npy (test_oracle_golden.py)",
+    "filter_small": "the reference's 1e-12 filter (n_point_base.py:205-211); part of the bubble_fft pin",
+    "cut_niv_g": "frequency window helper of the bubble (pinned with bubble_fft)",
+    "u_as_channel": "the reference's expectations of tests/test_interaction.py:93-108 (test_oracle_interaction.py); "
+                    "with vrg/chi_phys_shell it reproduces the reference chain vrg_q_* -> gchi_aux_q_*_sum",
+    "v_as_channel": "the reference's expectations of tests/test_interaction.py:166-200 (test_oracle_interaction.py)",
+    "sum_over_vn": "(1/beta) sum over nu; part of the vrg -> gchi_aux_q_*_sum reference chain",
+    "to_full_niw": "F(-w) = conj F(w); reproduces the reference's lambda-correction files (test_oracle_golden.py)",
+    "apply_lambda": "closed form; with find_lambda reproduces the reference's corrected chi files to 1e-5 "
+                    "(the complex64 limit, test_oracle_golden.py)",
+    "map_irr_to_fbz": "K_FBZ[k] = K_irr[irrk_inv[k]]; sigma_fft through it equals the direct q-sum form",
+    "ladder_matrix": "ladder_sum_inv/_solve pins (v1 = v3 = element-by-element inverse, residual, beta scaling)",
+    "ladder_rhs": "the nb^2 indicator columns of four_point.py:661-668; v1 = v3 pin",
+    "block_invert_vn0": "textbook nb^2 x nb^2 inverse; part of the chi_phys_shell reference chain",
+    "identity_vn1": "helper (block identity), pinned with block_invert_vn1",
+    "orb_matrix": "layout helper of auxiliary_chi_q / full_vertex_q (pinned with them)",
+    "chi0_inv_compound": "layout helper of the compound matrix (pinned with auxiliary_chi_q)",
+    "compound_local_vertex": "layout helper (rows (o1,o2,v), cols (o4,o3,v')), pinned with ladder_matrix",
+    "compound_to_ref8": "layout helper of full_vertex_q (pinned with it)",
+    "chi0_pp_w0": "one band: |G(k,nu)|^2, real and non-negative on the central 2 niv_pp frequencies "
+                  "(test_oracle_eliashberg.py); the multi-orbital index order only through the pairing eigenvalue "
+                  "chain from the reference's own 2-orbital vertex - partially pinned",
+    "pairing_setup": "part of pairing_matvec (equals the dense pairing operator)",
+    "dense_solve_reference": "numpy.linalg.solve (the reference's own call), used as a reference inside tests",
     "orbital_expand_fbz": "the kernel of every full-zone q computed directly equals the irreducible one expanded "
                           "with the orbital transforms on an orbital-symmetric 2- and 3-orbital model; the plain "
                           "gather does not (test_oracle_sigma.py)",

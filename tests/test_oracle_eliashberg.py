@@ -47,3 +47,17 @@ def test_pairing_eigenvalues_from_reference_vertex():
     ev = np.linalg.eigvals(M)
     top = np.sort(ev.real)[::-1][:4]
     assert np.allclose(top, [3.9108, 3.7144, 3.5533, 3.4945], atol=2e-4)
+
+
+def test_chi0_pp_one_band_closed_form():
+    """One band: chi0_pp(k, nu) = G(k, nu) conj G(k, nu) = |G|^2, real and non-negative, on the
+    central 2 niv_pp frequencies of the G box (bubble_gen.py:268-279, no -beta factor)."""
+    rng = np.random.default_rng(3)
+    nk, ng, npp = (4, 4, 1), 12, 5
+    g = rng.standard_normal(nk + (1, 1, 2 * ng)) + 1j * rng.standard_normal(nk + (1, 1, 2 * ng))
+    c = O.chi0_pp_w0(g, npp)
+    want = np.abs(g[..., ng - npp: ng + npp]) ** 2
+    got = c.reshape(want.shape)
+    assert np.abs(got.imag).max() == 0.0 or np.abs(got.imag).max() < 1e-15 * np.abs(got).max()
+    assert np.abs(got.real - want).max() < 1e-14 * want.max()
+    assert (got.real >= 0).all()
