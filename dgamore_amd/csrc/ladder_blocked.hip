@@ -894,6 +894,7 @@ int blocked_factor<float2>(const LadderArgs& a, BlockedState<float2>& st, int it
     case 2: return blocked_factor_nb<float2, 32, 3, 384, 8>(a, st, item0, nb, stream);
     case 3: return blocked_factor_nb<float2, 16, 3, 512, 8>(a, st, item0, nb, stream);
     case 4: return blocked_factor_nb<float2, 32, 2, 512, 4>(a, st, item0, nb, stream);
+    case 5: return blocked_factor_nb<float2, 32, 3, 384, 6>(a, st, item0, nb, stream);
     default: return blocked_factor_nb<float2, 16, 3, 512, 4>(a, st, item0, nb, stream);
   }
 }

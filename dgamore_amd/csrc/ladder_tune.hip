@@ -41,7 +41,7 @@ bool blocked_variant_fits(bool complex64, int variant, int n) {
   if (variant == 0) return n <= (complex64 ? 1152 : 768);
   if (variant == 1) return true;
   if (!complex64) return variant == 3 || variant == 4 || (variant == 2 && n <= 768);
-  return variant == 3 || (variant == 2 && n <= 1152) || (variant == 4 && n <= 1024);
+  return variant == 3 || ((variant == 2 || variant == 5) && n <= 1152) || (variant == 4 && n <= 1024);
 }
 
 int ladder_tuned_kernel(int n, int nrhs, int precision) {

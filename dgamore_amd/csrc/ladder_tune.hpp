@@ -19,8 +19,8 @@ constexpr int ladder_kernel_variant(int id) { return id % 256; }
 //               4 = NB 16, three rows, 4 panels (64-column outer blocks for n > 768)
 //   complex64:  0 = NB 32, three rows, 4 panels (n <= 1152);        1 = NB 16, three rows, 4 panels
 //               2 = NB 32, three rows, 8 panels (n <= 1152);        3 = NB 16, three rows, 8 panels
-//               4 = NB 32, two rows, 4 panels (n <= 1024)
-constexpr int BLK_VARIANTS = 5;
+//               4 = NB 32, two rows, 4 panels (n <= 1024);      5 = NB 32, three rows, 6 panels (n <= 1152)
+constexpr int BLK_VARIANTS = 6;
 bool blocked_variant_fits(bool complex64, int variant, int n);
 
 // Tuned kernel id of (n, nrhs, precision), or -1 when the autotuner has not set one.

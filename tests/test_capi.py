@@ -49,8 +49,9 @@ def test_ladder_tune_table_without_gpu():
                                               "blocked4"]
     assert names(720, 9, _lib.PLAIN_FP64) == ["blocked0", "blocked1", "blocked2", "blocked3", "blocked4"]
     assert names(1080, 9, _lib.PLAIN_FP64) == ["blocked1", "blocked3", "blocked4"]  # one row per thread: n <= 768
-    assert names(720, 9, _lib.MIXED_BF16X3_IR) == ["blocked0", "blocked1", "blocked2", "blocked3", "blocked4"]
-    assert names(1080, 9, _lib.MIXED_BF16X3_IR) == ["blocked0", "blocked1", "blocked2", "blocked3"]
+    assert names(720, 9, _lib.MIXED_BF16X3_IR) == ["blocked0", "blocked1", "blocked2", "blocked3", "blocked4",
+                                                   "blocked5"]
+    assert names(1080, 9, _lib.MIXED_BF16X3_IR) == ["blocked0", "blocked1", "blocked2", "blocked3", "blocked5"]
     assert names(1600, 1, _lib.PLAIN_FP64) == []
     kid = A.candidates(720, 9, 0)[1]
     A.set_choice(720, 9, 0, kid)
