@@ -207,6 +207,8 @@ class SDEPipeline:
         capturable."""
         if self.precision != _lib.PLAIN_FP64:
             raise ValueError("only the plain FP64 step can be captured as a graph")
+        if self.lambda_correction is not None:
+            raise ValueError("the lambda correction finds lambda on the host and cannot be captured as a graph")
         mu = self.m.mu if mu is None else mu
         nb = self.m.nb
         self.graph_sigma = self.sigma_dmft.expand(self.N, nb, nb, 2 * self.niv_s).contiguous().clone()

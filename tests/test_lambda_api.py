@@ -30,3 +30,15 @@ def test_sp_known_answer():
                     has_compressed_q_dimension=True)
     _, lam = lc.perform_single_lambda_correction(chi, s / 12.5)
     assert np.isclose(lam, 4.281153)  # test_lambda_correction.py:102
+
+
+def test_lambda_corrected_step_is_not_capturable():
+    """The lambda correction reads chi on the host: capture() refuses it before touching the GPU."""
+    import pytest
+
+    from dgamore_amd.pipeline import SDEPipeline
+
+    pipe = SDEPipeline.__new__(SDEPipeline)
+    pipe.precision, pipe.lambda_correction = 0, "sp"
+    with pytest.raises(ValueError):
+        pipe.capture()
