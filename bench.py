@@ -37,6 +37,8 @@ PEAK_HBM_GBS = 8000.0
 # HBM bytes per trailing-update launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE
 # (scaled by the copy-kernel factors), one "precision workload GB_per_launch" line each
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "trailing_update_traffic.txt")
+# the headline metric, as the benchmark definition names it
+METRIC = "(q,\u03c9) ladder solves/s + \u03a3 wall time: mixed vs FP64 on GPU at \u22641e-8 rel err"
 
 
 def parse():
@@ -183,7 +185,7 @@ def batch_graph(steps, warmup, seed, symmetric=True):
     solves = mb.solves_per_step()
     sizes = [p.m.nb ** 2 * 2 * p.m.niv for p in mb.pipes]
     print(json.dumps({
-        "metric": "(q,w) ladder solves/s + Sigma wall time (full non-local SDE iteration)",
+        "metric": METRIC,
         "value": solves / (ms * 1e-3), "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
         "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
         "data": "synthetic (16 seeded DMFT-like models, random-perturbation local vertices)",
@@ -442,7 +444,7 @@ def main():
 
     cpu = None if args.no_cpu_baseline else cpu_baseline(model, args.cpu_seconds)
     line = {
-        "metric": "(q,w) ladder solves/s + Sigma wall time (full non-local SDE iteration)",
+        "metric": METRIC,
         "value": solves / (ms_step * 1e-3),
         "unit": "solves/s",
         "n_gpus": 1,
