@@ -49,10 +49,14 @@ PARITY = {
     "chi0_inv_compound": "layout helper of the compound matrix (pinned with auxiliary_chi_q)",
     "compound_local_vertex": "layout helper (rows (o1,o2,v), cols (o4,o3,v')), pinned with ladder_matrix",
     "compound_to_ref8": "layout helper of full_vertex_q (pinned with it)",
-    "chi0_pp_w0": "one band: |G(k,nu)|^2, real and non-negative on the central 2 niv_pp frequencies "
-                  "(test_oracle_eliashberg.py); the multi-orbital index order only through the pairing eigenvalue "
-                  "chain from the reference's own 2-orbital vertex - partially pinned",
-    "pairing_setup": "part of pairing_matvec (equals the dense pairing operator)",
+    "chi0_pp_w0": "one band: |G(k,nu)|^2, real and non-negative on the central 2 niv_pp frequencies; two orbitals: "
+                  "an orbital-diagonal complex G puts g_a conj(g_b) exactly at (a, b, b, a) and zero elsewhere, and a "
+                  "unitary orbital rotation of a general G transforms the result as W W conj(W) conj(W) "
+                  "(test_oracle_eliashberg.py)",
+    "pairing_setup": "with pairing_matvec equals the explicit k-space double sum of the linearised gap equation "
+                     "(q = k'-k and the flipped q = -k'-k term) on a 4x4, 2-orbital, 2np=4 case without symmetries; "
+                     "a missing roll, a flip of the wrong frequency axis, a wrong orbital permutation or the wrong "
+                     "sign of the flipped term fails it (test_oracle_eliashberg.py)",
     "dense_solve_reference": "numpy.linalg.solve (the reference's own call), used as a reference inside tests",
     "orbital_expand_fbz": "the kernel of every full-zone q computed directly equals the irreducible one expanded "
                           "with the orbital transforms on an orbital-symmetric 2- and 3-orbital model; the plain "
@@ -66,12 +70,14 @@ PARITY = {
     "ladder_sum_solve": "same as ladder_sum_inv",
     "vrg": "reference-held chain vrg_q -> gchi_aux_q_*_sum (test_oracle_golden.py)",
     "chi_phys_shell": "reference-held gchi_aux_q_{dens,magn}_sum_rank_0.npy (test_oracle_golden.py)",
-    "kernel_r": "closed form for nb=1: K = u*(g - g*u*chi - [magn]2/3) (test_oracle_sigma.py)",
+    "kernel_r": "closed form for nb=1: K = u*(g - g*u*chi - [magn]2/3); nb = 2, 3: the same as products of "
+                "nb^2 x nb^2 compound matrices in the (a,b) x (d,c) order of the reference's matmuls "
+                "(test_oracle_sigma.py)",
     "dc_kernel": "brute-force loops on a tiny case (test_oracle_sigma.py)",
     "sigma_fft": "equals the direct q-sum sigma_direct (nonlocal_sde.py:404-427) on small grids (test_oracle_sigma.py)",
     "sigma_direct": "brute-force loop over (k,q,w,a,d) on a tiny grid (test_oracle_sigma.py)",
     "find_lambda": "reference known answers lambda=-37.450340, 4.328781, 4.281153 + fixtures (test_oracle_golden.py)",
-    "pairing_matvec": "equals the explicit dense pairing operator built element by element (test_oracle_eliashberg.py)",
+    "pairing_matvec": "equals the explicit k-space double sum (see pairing_setup; test_oracle_eliashberg.py)",
     "hartree_fock": "closed forms: local U gives k-independent Fock -U n and Hartree 2 U n; cos(q) V on a "
                     "cos(k) occupation gives -(V/2) b cos(k) (test_oracle_selfcons.py)",
     "fit_smom": "closed form on Sigma = m0 + m1/(i v) (test_oracle_selfcons.py)",

@@ -49,6 +49,43 @@ def test_kernel_r_single_band_closed_form():
         assert np.abs(got[:, 0, 0, 0, 0] - ref).max() < 1e-13
 
 
+def _m(x):
+    """Compound matrix of a 4-orbital object in the reference's product order: rows (a, b),
+    columns (d, c), so that sum_cd X_abcd Y_dcef (four_point.py:537-541) is the matrix product."""
+    nb = x.shape[0]
+    return x.transpose(0, 1, 3, 2, *range(4, x.ndim)).reshape(nb * nb, nb * nb, *x.shape[4:])
+
+
+def _unm(m, nb):
+    return m.reshape(nb, nb, nb, nb, *m.shape[2:]).transpose(0, 1, 3, 2, *range(4, m.ndim + 2))
+
+
+@pytest.mark.parametrize("nb", [2, 3])
+@pytest.mark.parametrize("channel", ["dens", "magn"])
+def test_kernel_r_equals_compound_matrix_form(nb, channel):
+    """K_r = u (gamma - gamma u chi - [magn] 2/3 1), u = U_r + V_r(q) (nonlocal_sde.py:201-215), as
+    products of nb^2 x nb^2 compound matrices per (q, w, v): the reference's FourPoint/Interaction
+    matmuls (four_point.py:497-557) contract the left object's last orbital pair (c, d) with the
+    right object's first pair in reversed order (d, c), which is the matrix product in the
+    (a, b) x (d, c) compound index. A transposed operand or a swapped orbital pair fails here."""
+    rng = np.random.default_rng(10 + nb)
+    nq, nw, nv = 2, 2, 3
+    g = rng.standard_normal((nq,) + (nb,) * 4 + (nw, nv)) + 1j * rng.standard_normal((nq,) + (nb,) * 4 + (nw, nv))
+    chi = rng.standard_normal((nq,) + (nb,) * 4 + (nw,)) + 1j * rng.standard_normal((nq,) + (nb,) * 4 + (nw,))
+    u = rng.standard_normal((nb,) * 4)
+    v = rng.standard_normal((nq,) + (nb,) * 4) + 1j * rng.standard_normal((nq,) + (nb,) * 4)
+    got = O.kernel_r(g, chi, u, v, channel)
+    eye = np.eye(nb * nb)
+    for q in range(nq):
+        um = _m(u + v[q])
+        for w in range(nw):
+            cm = _m(chi[q, ..., w])
+            for iv in range(nv):
+                gm = _m(g[q, ..., w, iv])
+                km = um @ (gm - gm @ um @ cm - (2.0 / 3.0 if channel == "magn" else 0.0) * eye)
+                assert np.abs(_unm(km, nb) - got[q, ..., w, iv]).max() < 1e-12 * np.abs(km).max()
+
+
 @pytest.mark.parametrize("nb,nk,niw,niv", [(1, (6, 6, 1), 2, 3), (2, (4, 4, 1), 1, 2)])
 def test_sigma_fft_equals_direct(nb, nk, niw, niv):
     rng = np.random.default_rng(2)
