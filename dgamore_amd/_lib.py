@@ -69,6 +69,8 @@ SIGNATURES["dgami_ladder_tune_candidates"] = ([_i, _i, _i, _c, _i], _i)
 SIGNATURES["dgami_ladder_tune_set"] = ([_i, _i, _i, _i], _i)
 SIGNATURES["dgami_ladder_tune_get"] = ([_i, _i, _i], _i)
 SIGNATURES["dgami_ladder_tune_clear"] = ([], _i)
+SIGNATURES["dgami_workspace_hold"] = ([], _i)
+SIGNATURES["dgami_workspace_release"] = ([], _i)
 
 _lib = None
 
@@ -119,3 +121,22 @@ def profile_read():
     ms, by, fl, n = ctypes.c_double(), ctypes.c_double(), ctypes.c_double(), ctypes.c_int64()
     call("dgami_profile_read", ctypes.addressof(ms), ctypes.addressof(by), ctypes.addressof(fl), ctypes.addressof(n))
     return {"ms": ms.value, "bytes": by.value, "flops": fl.value, "launches": n.value}
+
+
+def hold_workspaces(graph):
+    """Keep the library's workspace buffers alive while `graph` (a captured HIP graph that names
+    them) exists: outgrown buffers are retired instead of freed until the graph is collected."""
+    import weakref
+
+    n = load().dgami_workspace_hold()
+    if n < 1:
+        raise RuntimeError(f"dgami_workspace_hold: {last_error()}")
+    fin = weakref.finalize(graph, _release_workspaces)
+    fin.atexit = False
+    return fin
+
+
+def _release_workspaces():
+    lib = load()
+    if lib.dgami_workspace_release() < 0:
+        raise RuntimeError(f"dgami_workspace_release: {last_error()}")

@@ -17,6 +17,10 @@ struct Slot {
 };
 std::mutex g_ws_mutex;
 std::vector<Slot> g_slots;
+// While captured graphs are alive (g_holds > 0) an outgrown buffer may still be named by a
+// graph node, so it is kept here instead of freed, until the last hold is released.
+int g_holds = 0;
+std::vector<void*> g_retired;
 }  // namespace
 
 void set_error(const std::string& msg) { g_last_error = msg; }
@@ -71,8 +75,10 @@ void* workspace(int slot, size_t bytes) {
   Slot& s = g_slots[slot];
   if (s.ptr != nullptr && s.bytes >= bytes && s.device == dev) return s.ptr;
   if (s.ptr != nullptr) {
-    // the previous buffer may still be in use by queued work on any stream
-    if (hipDeviceSynchronize() != hipSuccess || hipFree(s.ptr) != hipSuccess) {
+    if (g_holds > 0) {
+      g_retired.push_back(s.ptr);  // a live graph may still use it
+    } else if (hipDeviceSynchronize() != hipSuccess || hipFree(s.ptr) != hipSuccess) {
+      // the previous buffer may still be in use by queued work on any stream
       set_error("workspace: could not release the previous buffer");
       return nullptr;
     }
@@ -119,6 +125,24 @@ int dgami_profile_read(double* ms, double* bytes, double* flops, int64_t* launch
   *flops = g_prof.flops;
   *launches = static_cast<int64_t>(g_prof.events.size());
   return 0;
+}
+
+int dgami_workspace_hold(void) {
+  std::lock_guard<std::mutex> lock(dgami::g_ws_mutex);
+  ++dgami::g_holds;
+  return dgami::g_holds;
+}
+
+int dgami_workspace_release(void) {
+  using namespace dgami;
+  std::lock_guard<std::mutex> lock(g_ws_mutex);
+  DGAMI_ARG_CHECK(g_holds > 0, "workspace_release: no hold to release");
+  if (--g_holds == 0 && !g_retired.empty()) {
+    DGAMI_HIP_CHECK(hipDeviceSynchronize());
+    for (void* p : g_retired) DGAMI_HIP_CHECK(hipFree(p));
+    g_retired.clear();
+  }
+  return g_holds;
 }
 
 int dgami_version(void) { return 1; }

@@ -166,7 +166,9 @@ __device__ __forceinline__ void panel_column(S (&P)[RPT][NB], unsigned& done, Pa
     }
   }
   best = gkey > 1u ? 1.0 : 0.0;  // nonzero pivot magnitude
-  const int p = bidx;            // original index of the pivot row
+  // no finite candidate (every magnitude NaN): pivot on the row already at p0 + C, which is not
+  // yet done; the zero inverse pivot below marks the problem singular in info
+  const int p = gkey == 0u ? sh.rowat[C] : bidx;  // original index of the pivot row
 #pragma unroll
   for (int m = 0; m < RPT; ++m)
     if (p0 + tid + TH * m == p) {

@@ -103,6 +103,7 @@ __global__ void __launch_bounds__(NW * 64) ladder_col_kernel(LadderArgs a) {
         const unsigned long long hit = __ballot(key == wkey);
         bidx = __builtin_amdgcn_readlane(bidx, __ffsll(static_cast<long long>(hit)) - 1);
         best = wkey > 1u ? 1.0 : 0.0;  // nonzero pivot magnitude
+        if (wkey == 0u) bidx = k;      // no finite candidate (NaN column): keep row k, flagged singular below
       }
       const int p = bidx;
       const int pl = p & 63, pm = p >> 6;

@@ -11,11 +11,17 @@
 //                    U_r + V_r term are applied per item);
 //   D = (LU)^-1 P R  in FP32 (blocked triangular sweeps with inverse diagonal blocks);
 //   X += D           in FP64.
-// Starting from X = 0 (R = B), solves continue until the predicted relative error is below
-// 1e-10 (see MIXED_TOL). Each refinement step multiplies the error by about cond(A) * 2^-24
-// (FP32 factors) or cond(A) * 2^-16 (split-BF16 updates); the number of solves is data
-// dependent, so the host reads one float per solve.
+// Starting from X = 0 (R = B), every item is refined until its predicted relative error is
+// below 1e-10 (see MIXED_TOL). Each refinement step multiplies the error by about
+// cond(A) * 2^-24 (FP32 factors) or cond(A) * 2^-16 (split-BF16 updates), so the number of
+// solves differs between items: the host issues a fixed maximum number of refinement steps,
+// each item's status lives on the device, and the kernels of a step skip items that have
+// converged (and the vertex GEMM skips (channel, w) groups with no active item). No value
+// is read back by the host, so a call can be captured in a HIP graph. An item that has not
+// converged after the last step, or whose correction became non-finite, is reported in
+// info (DGAMI_INFO_REFINE_STALLED, DGAMI_INFO_REFINE_NONFINITE).
 #include <algorithm>
+#include <cstdlib>
 
 #include "dgami.h"
 #include "dgami_internal.hpp"
@@ -35,6 +41,16 @@ constexpr int MIXED_MAX_SOLVES_FP32 = 4;
 constexpr int MIXED_MAX_SOLVES_BF16 = 8;
 constexpr double MIXED_TOL = 1e-10;
 
+// Refinement status of an item (device array, one int per item of the batch).
+enum MixStatus { MIX_ACTIVE = 0, MIX_CONVERGED = 1, MIX_STALLED = 2, MIX_NONFINITE = 3 };
+
+// Tolerance on the predicted relative error; DGAMI_MIXED_TOL overrides it (for measurements).
+static double mixed_tol() {
+  const char* v = getenv("DGAMI_MIXED_TOL");
+  const double t = v ? atof(v) : 0.0;
+  return t > 0.0 ? t : MIXED_TOL;
+}
+
 // Item layout of the refinement vectors: group g = (chan, w) (or the item itself in
 // dense mode) holds gq items; X and R are [group][n][gq][T] so that Gamma(g) X(g) is
 // one GEMM with N = gq * T.
@@ -45,14 +61,23 @@ struct MixedArgs {
   const double2* dinv;  // [item][n] inverse pivots
   double2* X;           // [group][n][gq][T]
   double2* R;           // same
+  int* status;          // [item of the batch] MixStatus
+  float* prev;          // [item of the batch] relative correction of the previous refinement step
+  int* gactive;         // [group of the batch] 1 while the group has an active item
   int ld, gq, group0, item0;
 };
 
 // ---------------------------------------------------------------------------
 // Row permutation of each factorisation from its sequence of interchanges.
-__global__ void __launch_bounds__(64) mix_perm_kernel(const int* __restrict__ piv, int* __restrict__ perm, int n) {
+// Also starts the item's refinement (status active, no previous correction).
+__global__ void __launch_bounds__(64) mix_perm_kernel(const int* __restrict__ piv, int* __restrict__ perm, int n,
+                                                      int* __restrict__ status, float* __restrict__ prev) {
   extern __shared__ int s_perm[];
   const int b = blockIdx.x;
+  if (threadIdx.x == 0) {
+    status[b] = MIX_ACTIVE;
+    prev[b] = 0.f;
+  }
   for (int i = threadIdx.x; i < n; i += 64) s_perm[i] = i;
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -73,6 +98,7 @@ __global__ void __launch_bounds__(64) mix_perm_kernel(const int* __restrict__ pi
 __global__ void __launch_bounds__(256) mix_residual_kernel(MixedArgs m) {
   const LadderArgs& a = m.a;
   const int lb = blockIdx.x;                  // item within the batch
+  if (m.status[lb] != MIX_ACTIVE) return;
   const int gl = lb / m.gq, q = lb - gl * m.gq;  // local group, q within the group
   const int item = m.item0 + lb;
   const int n = a.n, T = a.T, nv = a.nv, gq = m.gq;
@@ -124,6 +150,7 @@ __global__ void __launch_bounds__(256) mix_residual_kernel(MixedArgs m) {
 __global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __restrict__ Y, int first) {
   const LadderArgs& a = m.a;
   const int lb = blockIdx.y;
+  if (m.status[lb] != MIX_ACTIVE) return;
   const int gl = lb / m.gq, q = lb - gl * m.gq;
   const int n = a.n, T = a.T, gq = m.gq;
   const int idx = blockIdx.x * 256 + threadIdx.x;
@@ -135,12 +162,17 @@ __global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __
       first ? from_c128<float2>(ladder_rhs(a, src, t)) : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
 }
 
-// X += D (FP64) and out = X / beta; per item, the largest |D| and |X| (|Re|+|Im|) go to
-// norms[2*item], norms[2*item+1]. One workgroup per item.
-__global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const float2* __restrict__ D, int first,
-                                                             float* __restrict__ norms) {
+// X += D (FP64) and out = X / beta for the active items, then the item's convergence test
+// from its largest |D| and |X| (|Re|+|Im|): r = |D|/|X| estimates the relative error before this
+// correction and r / r_prev the contraction of one refinement step, so the error after it is
+// predicted as r * contraction; the item converges when that is below tol. A non-finite D or X
+// ends the item's refinement as MIX_NONFINITE; an item still active after the last step is
+// MIX_STALLED. One workgroup per item; step 0 (the first solve, X = D) only records r.
+__global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const float2* __restrict__ D, int step,
+                                                             int last, double tol) {
   const LadderArgs& a = m.a;
   const int lb = blockIdx.x;
+  if (m.status[lb] != MIX_ACTIVE) return;
   const int gl = lb / m.gq, q = lb - gl * m.gq;
   const int n = a.n, T = a.T, gq = m.gq;
   const int item = m.item0 + lb;
@@ -148,42 +180,70 @@ __global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const 
   double2* X = m.X + static_cast<size_t>(gl) * n * gq * T;
   const float2* Db = D + static_cast<size_t>(lb) * n * T;
   float dmax = 0.f, xmax = 0.f;
+  unsigned bad = 0;
   for (int idx = threadIdx.x; idx < n * T; idx += 256) {
     const int i = idx / T, t = idx - (idx / T) * T;
     const size_t o = (static_cast<size_t>(i) * gq + q) * T + t;
     const double2 d = to_c128(Db[idx]);
-    const double2 x = first ? d : cadd(X[o], d);
+    const double2 x = step == 0 ? d : cadd(X[o], d);
     X[o] = x;
     a.out[ladder_out_index(a, chan, w, qt, i, t)] = cscale(x, a.inv_beta);
-    dmax = fmaxf(dmax, static_cast<float>(cabs1(d)));
-    xmax = fmaxf(xmax, static_cast<float>(cabs1(x)));
+    const float dv = static_cast<float>(cabs1(d)), xv = static_cast<float>(cabs1(x));
+    bad |= !(dv <= 3.0e38f) || !(xv <= 3.0e38f);  // NaN or infinite (fmaxf would drop a NaN)
+    dmax = fmaxf(dmax, dv);
+    xmax = fmaxf(xmax, xv);
   }
   __shared__ float s_d[4], s_x[4];
+  __shared__ unsigned s_bad[4];
   const unsigned dk = wave_max_u32(__float_as_uint(dmax)), xk = wave_max_u32(__float_as_uint(xmax));
+  const unsigned bk = wave_max_u32(bad);
   if ((threadIdx.x & 63) == 0) {
     s_d[threadIdx.x >> 6] = __uint_as_float(dk);
     s_x[threadIdx.x >> 6] = __uint_as_float(xk);
+    s_bad[threadIdx.x >> 6] = bk;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    norms[2 * static_cast<size_t>(lb)] = fmaxf(fmaxf(s_d[0], s_d[1]), fmaxf(s_d[2], s_d[3]));
-    norms[2 * static_cast<size_t>(lb) + 1] = fmaxf(fmaxf(s_x[0], s_x[1]), fmaxf(s_x[2], s_x[3]));
+    const float d = fmaxf(fmaxf(s_d[0], s_d[1]), fmaxf(s_d[2], s_d[3]));
+    const float x = fmaxf(fmaxf(s_x[0], s_x[1]), fmaxf(s_x[2], s_x[3]));
+    if (s_bad[0] | s_bad[1] | s_bad[2] | s_bad[3]) {
+      m.status[lb] = MIX_NONFINITE;
+      return;
+    }
+    const double r = x > 0.f ? static_cast<double>(d) / x : (d > 0.f ? 1.0 : 0.0);
+    if (step > 0) {
+      const double pr = m.prev[lb];
+      const double contraction = (step >= 2 && pr > 0.0) ? r / pr : r;
+      if (r * contraction < tol)
+        m.status[lb] = MIX_CONVERGED;
+      else if (last)
+        m.status[lb] = MIX_STALLED;
+    }
+    m.prev[lb] = static_cast<float>(r);
   }
 }
 
-// Largest relative correction max_item(|D|_max / |X|_max) of the batch into *ratio.
-__global__ void __launch_bounds__(256) mix_ratio_kernel(const float* __restrict__ norms, int nitems,
-                                                        float* __restrict__ ratio) {
-  float r = 0.f;
-  for (int b = threadIdx.x; b < nitems; b += 256) {
-    const float d = norms[2 * b], x = norms[2 * b + 1];
-    r = fmaxf(r, x > 0.f ? d / x : (d > 0.f ? 1.f : 0.f));
-  }
-  __shared__ float s_r[4];
-  const unsigned k = wave_max_u32(__float_as_uint(r));
-  if ((threadIdx.x & 63) == 0) s_r[threadIdx.x >> 6] = __uint_as_float(k);
-  __syncthreads();
-  if (threadIdx.x == 0) *ratio = fmaxf(fmaxf(s_r[0], s_r[1]), fmaxf(s_r[2], s_r[3]));
+// gactive[g] = 1 iff an item of group g of the batch is still active (one thread per group).
+__global__ void __launch_bounds__(256) mix_group_active_kernel(const int* __restrict__ status, int* __restrict__ gactive,
+                                                               int gq, int ng) {
+  const int g = blockIdx.x * 256 + threadIdx.x;
+  if (g >= ng) return;
+  int any = 0;
+  for (int i = 0; i < gq; ++i) any |= status[static_cast<size_t>(g) * gq + i] == MIX_ACTIVE;
+  gactive[g] = any;
+}
+
+// info of the batch's items: a zero pivot of the factorisation (k + 1) first, else the outcome
+// of the refinement (0, DGAMI_INFO_REFINE_STALLED or DGAMI_INFO_REFINE_NONFINITE).
+__global__ void __launch_bounds__(256) mix_info_kernel(const int* __restrict__ st_info, const int* __restrict__ status,
+                                                       int32_t* __restrict__ info, int item0, int nitems) {
+  const int b = blockIdx.x * 256 + threadIdx.x;
+  if (b >= nitems) return;
+  const int st = status[b];
+  info[item0 + b] = st_info[b] != 0 ? st_info[b]
+                    : st == MIX_STALLED ? DGAMI_INFO_REFINE_STALLED
+                    : st == MIX_NONFINITE ? DGAMI_INFO_REFINE_NONFINITE
+                                          : 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -193,6 +253,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   // the split-BF16 factors carry ~2^-16 relative error per product instead of 2^-24, so
   // each refinement step gains fewer digits and more steps may be needed
   const int max_solves = precision == DGAMI_MIXED_BF16X3_IR ? MIXED_MAX_SOLVES_BF16 : MIXED_MAX_SOLVES_FP32;
+  const double tol = mixed_tol();
   const int n = a.n, T = a.T;
   DGAMI_ARG_CHECK(T <= 16, "ladder: at most 4 orbitals in the mixed-precision form");
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
@@ -221,7 +282,8 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   const int tuned = ladder_tuned_kernel(n, T, precision);
   if (tuned >= 0 && ladder_kernel_family(tuned) == LADDER_FAM_BLOCKED) st.variant = ladder_kernel_variant(tuned);
   char* vec = static_cast<char*>(workspace(WS_MIXED_VEC, 2 * bitems * n * T * sizeof(double2) + bitems * n * sizeof(int) +
-                                                             (2 * bitems + 4) * sizeof(float)));
+                                                             2 * bitems * sizeof(int) + bitems * sizeof(float) +
+                                                             gb * sizeof(int)));
   if (!vec) return -2;
   MixedArgs m;
   m.a = a;
@@ -230,7 +292,9 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   m.R = m.X + bitems * n * T;
   int* perm = reinterpret_cast<int*>(m.R + bitems * n * T);
   m.perm = perm;
-  float* norms = reinterpret_cast<float*>(perm + bitems * n);  // [bitems][2] + the batch ratio
+  m.status = perm + bitems * n;
+  m.gactive = m.status + bitems;
+  m.prev = reinterpret_cast<float*>(m.gactive + gb);
   m.dinv = st.dinv;
   m.ld = ld;
   m.gq = gq;
@@ -248,15 +312,19 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     m.item0 = item0;
     int rc = blocked_factor<float2>(a, st, item0, nit, stream);
     if (rc) return rc;
-    hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), stream, st.piv, perm, n);
+    hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), stream, st.piv, perm, n, m.status,
+                       m.prev);
     DGAMI_HIP_CHECK(hipGetLastError());
-    const TriFactors<float2> f{st.W, tri, st.dinv, n, ld, nblk};
+    TriFactors<float2> f{st.W, tri, st.dinv, n, ld, nblk};
     rc = tri_diag_inverse(f, nit, true, stream);
     if (rc) return rc;
+    f.skip = m.status;  // the sweeps leave converged items alone
     const dim3 vgrid(ceil_div(static_cast<size_t>(n) * T, 256), nit);
-    double prev = 0.0;
     for (int s = 0; s < max_solves; ++s) {
       if (s > 0) {
+        hipLaunchKernelGGL(mix_group_active_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, stream, m.status, m.gactive,
+                           gq, ng);
+        DGAMI_HIP_CHECK(hipGetLastError());
         ZgemmArgs g;
         g.A = a.gamma + static_cast<size_t>(g0) * n * n;
         g.B = m.X;
@@ -272,6 +340,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         g.sc = g.sb;
         g.alpha = make_double2(1.0, 0.0);
         g.beta = make_double2(0.0, 0.0);
+        g.active = m.gactive;
         rc = launch_zgemm(g, ng, stream);
         if (rc) return rc;
         hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, stream, m);
@@ -290,21 +359,15 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
         if (rc) return rc;
       }
-      hipLaunchKernelGGL(mix_accumulate_kernel, dim3(nit), dim3(256), 0, stream, m, yv, s == 0 ? 1 : 0, norms);
+      hipLaunchKernelGGL(mix_accumulate_kernel, dim3(nit), dim3(256), 0, stream, m, yv, s,
+                         s == max_solves - 1 ? 1 : 0, tol);
       DGAMI_HIP_CHECK(hipGetLastError());
-      if (s == 0) continue;
-      hipLaunchKernelGGL(mix_ratio_kernel, dim3(1), dim3(256), 0, stream, norms, nit, norms + 2 * bitems);
-      DGAMI_HIP_CHECK(hipGetLastError());
-      float ratio = 0.f;
-      DGAMI_HIP_CHECK(hipMemcpyAsync(&ratio, norms + 2 * bitems, sizeof(float), hipMemcpyDeviceToHost, stream));
-      DGAMI_HIP_CHECK(hipStreamSynchronize(stream));
-      const double r = ratio;
-      const double contraction = (s >= 2 && prev > 0.0) ? r / prev : r;
-      prev = r;
-      if (r * contraction < MIXED_TOL || !(r == r)) break;
     }
-    rc = blocked_copy_info(st.info, a.info, item0, nit, stream);
-    if (rc) return rc;
+    if (a.info) {
+      hipLaunchKernelGGL(mix_info_kernel, dim3(ceil_div(nit, 256)), dim3(256), 0, stream, st.info, m.status, a.info,
+                         item0, nit);
+      DGAMI_HIP_CHECK(hipGetLastError());
+    }
   }
   return 0;
 }

@@ -136,7 +136,7 @@ __device__ __forceinline__ void ladder_reg_solve(const LadderArgs& a, int item) 
         bval = s_pv[buf][wv];
       }
     }
-    const int p = bidx;
+    const int p = bidx == 0x7fffffff ? k : bidx;  // no finite candidate (NaN column): keep row k
     const bool ok = best > 0.0;
     const double2 pinv = ok ? cinv(bval) : make_double2(0.0, 0.0);
     if (tid == 0) {

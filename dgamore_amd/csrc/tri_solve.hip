@@ -91,6 +91,7 @@ template <typename S, int T, bool LOWER, bool TO_LADDER, int RPT>
 __global__ void __launch_bounds__(256) tri_sweep_kernel(TriFactors<S> f, TriVec<S> y, TriVec<S> out, LadderArgs la,
                                                         int item0) {
   const int b = blockIdx.x;
+  if (f.skip && f.skip[b] != 0) return;
   const int tid = threadIdx.x;
   const int n = f.n;
   __shared__ S s_d[TRI_B][TRI_B + 1];

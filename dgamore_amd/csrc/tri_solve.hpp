@@ -23,6 +23,7 @@ struct TriFactors {
   S* dinvblk;          // [batch][nblk][2][TRI_B][TRI_B]: inverse of L_bb, inverse of U_bb
   const double2* dinv; // [batch][n] inverse pivots (diagonal of U^-1)
   int n, ld, nblk;
+  const int* skip = nullptr;  // [batch] or null: the sweeps leave problems with skip[b] != 0 untouched
 };
 
 // Right-hand-side block [batch][n][T] addressed with a row stride (elements) and an item stride.

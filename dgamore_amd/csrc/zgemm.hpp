@@ -14,6 +14,7 @@ struct ZgemmArgs {
   int lda, ldb, ldc;
   long long sa, sb, sc;  // batch strides
   double2 alpha, beta;
+  const int* active = nullptr;  // [batch] or null: batch entries with active[b] == 0 are skipped
 };
 
 constexpr int ZG_TM = 64, ZG_TN = 64, ZG_TK = 16;
@@ -22,6 +23,7 @@ constexpr int ZG_TM = 64, ZG_TN = 64, ZG_TK = 16;
 static __global__ void __launch_bounds__(256) zgemm_kernel(ZgemmArgs g, int batch, int tiles_n, int tiles_m) {
   int b, t;
   if (!xcd_tile(blockIdx.x, batch, tiles_m * tiles_n, b, t)) return;
+  if (g.active && g.active[b] == 0) return;
   const double2* A = g.A + b * g.sa;
   const double2* B = g.B + b * g.sb;
   double2* C = g.C + b * g.sc;

@@ -134,11 +134,21 @@ def chi0_prepare(chi0, nb, niv_core, beta):
     return inv, full, core
 
 
+def capturing():
+    """True while the current stream records a HIP graph (no host reads are allowed then)."""
+    return torch.cuda.is_current_stream_capturing()
+
+
 def ladder_solve_sum(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, precision=_lib.PLAIN_FP64, check_info=True):
     """Batched ladder solves for nchan channels.
 
     chi0_inv: QF [nw][nb^4][2niv][nq]; gamma_cmp: [nchan][nw][n][n]; u_r_m: [nchan][nb^4];
     v_r_m: [nchan][nq][nb^4] or None. Returns (chi_aux_sum [nchan] x QF, info int32 [nchan][nw][nq]).
+
+    With check_info (outside graph capture), the mixed forms' items whose refinement stalled or
+    turned non-finite (negative info) are solved again with the plain FP64 LU, and a singular
+    matrix raises numpy.linalg.LinAlgError as in the reference (four_point.py:636-638). Under
+    capture, or with check_info=False, info is returned as the kernels left it.
     """
     nw, nb4, nv, nq = chi0_inv.shape
     nb = int(round(nb4 ** 0.25))
@@ -150,11 +160,31 @@ def ladder_solve_sum(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, precision=_lib.PLA
     info = torch.zeros((nchan, nw, nq), dtype=torch.int32, device=chi0_inv.device)
     _lib.call("dgami_ladder_solve_sum", nchan, nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp),
               _p(u_r_m), _p(v_r_m), int(precision), _p(out), _p(info, torch.int32), _stream())
-    if check_info:
+    if check_info and not capturing():
+        if precision != _lib.PLAIN_FP64:
+            _resolve_fp64(out, info, chi0_inv, gamma_cmp, u_r_m, v_r_m, beta)
         bad = torch.nonzero(info).cpu().numpy()
         if len(bad):
             raise np.linalg.LinAlgError(f"singular ladder matrix at (chan, w, q) = {bad[:8].tolist()}")
     return out, info
+
+
+def _resolve_fp64(out, info, chi0_inv, gamma_cmp, u_r_m, v_r_m, beta):
+    """Solve the items whose mixed-precision refinement failed (info < 0) again with the plain
+    FP64 LU, one call per (channel, w) with the failed q-points, and put the results and their
+    info in place."""
+    bad = torch.nonzero(info < 0).cpu().numpy()
+    if not len(bad):
+        return
+    for c, w in sorted({(int(c), int(w)) for c, w, _ in bad}):
+        qs = sorted(int(q) for cc, ww, q in bad if cc == c and ww == w)
+        qi = torch.tensor(qs, dtype=torch.long, device=out.device)
+        ci = chi0_inv[w:w + 1].index_select(3, qi).contiguous()
+        v = None if v_r_m is None else v_r_m[c:c + 1].index_select(1, qi).contiguous()
+        o, inf = ladder_solve_sum(ci, gamma_cmp[c:c + 1, w:w + 1].contiguous(), u_r_m[c:c + 1].contiguous(), v, beta,
+                                  _lib.PLAIN_FP64, check_info=False)
+        out[c, w][..., qi] = o[0, 0]
+        info[c, w, qi] = inf[0, 0]
 
 
 def ladder_solve_sum_multi(models, check_info=True):
@@ -177,7 +207,7 @@ def ladder_solve_sum_multi(models, check_info=True):
                                     _p(v_r_m), _p(out), _p(info, torch.int32))
         outs.append((out, info))
     _lib.call("dgami_ladder_solve_sum_multi", len(models), ctypes.addressof(table), _stream())
-    if check_info:
+    if check_info and not capturing():
         for out, info in outs:
             bad = torch.nonzero(info).cpu().numpy()
             if len(bad):

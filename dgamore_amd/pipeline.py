@@ -119,11 +119,14 @@ class SDEPipeline:
     def step(self, sigma=None, mu=None, keep=False, sigma_old=None):
         """One non-local SDE iteration from Sigma to the new, mixed Sigma [N,nb,nb,2 niv_s] on the device
         (the DMFT self-energy outside the core window). sigma, sigma_old: [1 or N][nb][nb][2 niv_s];
-        sigma_old defaults to sigma."""
+        sigma_old defaults to sigma. Eager steps check the ladder's info (mixed-precision items that
+        did not converge are solved again in FP64; a singular matrix raises LinAlgError); a captured
+        step leaves info in self.last_info for the caller to read after a replay."""
         sigma = self.sigma_dmft if sigma is None else sigma
         front = self.front(sigma, mu)
         aux, info = K.ladder_solve_sum(front["chi0_inv"], self.gamma, self.u_r, self.v_r, self.m.beta, self.precision,
-                                       check_info=False)
+                                       check_info=not K.capturing())
+        self.last_info = info
         return self.back(front, aux, info, sigma, sigma_old, keep)
 
     def front(self, sigma, mu=None):
@@ -203,10 +206,10 @@ class SDEPipeline:
         Sigma to `graph_out`; replay() runs it again without host work. With chain, the graph ends
         by copying graph_out into graph_sigma, so that consecutive replays are consecutive
         iterations. mu, the occupations of the Hartree-Fock terms and the model are fixed at
-        capture time. The mixed forms read one float per refinement step on the host and are not
-        capturable."""
-        if self.precision != _lib.PLAIN_FP64:
-            raise ValueError("only the plain FP64 step can be captured as a graph")
+        capture time. Every precision can be captured: the mixed forms keep their refinement
+        status on the device, and after a replay `last_info` holds the ladder's info (nonzero
+        entries: singular matrices or refinements that did not converge, which a captured step
+        cannot re-solve)."""
         if self.lambda_correction is not None:
             raise ValueError("the lambda correction finds lambda on the host and cannot be captured as a graph")
         mu = self.m.mu if mu is None else mu
@@ -216,6 +219,7 @@ class SDEPipeline:
         self.step(self.graph_sigma, mu)
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
+        _lib.hold_workspaces(self.graph)
         with torch.cuda.graph(self.graph):
             self.graph_out = self.step(self.graph_sigma, mu)
             if chain:
@@ -289,7 +293,8 @@ class ModelBatch:
         sigmas = [p.sigma_dmft for p in self.pipes] if sigmas is None else sigmas
         mus = [None] * len(self.pipes) if mus is None else mus
         fronts = [p.front(s, mu) for p, s, mu in zip(self.pipes, sigmas, mus)]
-        outs = K.ladder_solve_sum_multi([p.ladder_model(f) for p, f in zip(self.pipes, fronts)], check_info=False)
+        outs = K.ladder_solve_sum_multi([p.ladder_model(f) for p, f in zip(self.pipes, fronts)],
+                                        check_info=not K.capturing())
         return [p.back(f, aux, info, s, None, keep) for p, f, (aux, info), s in zip(self.pipes, fronts, outs, sigmas)]
 
     def capture(self, chain=True):
@@ -300,6 +305,7 @@ class ModelBatch:
         self.step(self.graph_sigmas)  # warm-up: workspaces and occupations
         torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
+        _lib.hold_workspaces(self.graph)
         with torch.cuda.graph(self.graph):
             self.graph_outs = self.step(self.graph_sigmas)
             if chain:

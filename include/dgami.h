@@ -58,11 +58,30 @@ enum {
   DGAMI_MIXED_BF16X3_IR = 2 /* split-BF16 (3-term) MFMA trailing updates + FP64 IR */
 };
 
+/* Negative per-matrix codes in the `info` arrays of the mixed-precision ladder forms: the FP64
+ * refinement of the item did not reach its tolerance within the maximum number of refinement
+ * steps (STALLED), or its correction became NaN or infinite (NONFINITE). Positive codes k + 1
+ * still report an exactly singular pivot at elimination step k. The reference raises
+ * numpy.linalg.LinAlgError for a singular ladder matrix (four_point.py:636-638); the Python layer
+ * re-solves STALLED / NONFINITE items with the plain FP64 LU and raises for singular ones. */
+#define DGAMI_INFO_REFINE_STALLED (-1)
+#define DGAMI_INFO_REFINE_NONFINITE (-2)
+
 /* Text of the last error on the calling host thread ("" if none). */
 const char* dgami_last_error(void);
 
 /* ABI version of the library (incremented on incompatible changes). */
 int dgami_version(void);
+
+/* Workspace holds for captured HIP graphs. A captured graph names the library's workspace
+ * buffers by address; a later call that needs a larger buffer would normally free the old one.
+ * While at least one hold is taken, outgrown buffers are kept instead and freed only when the
+ * last hold is released. hold() returns the number of holds after taking one; release()
+ * returns the number left (-1 if none was held). No counterpart in the reference, which
+ * captures nothing; the Python layer takes a hold per captured graph and releases it when the
+ * graph is dropped. */
+int dgami_workspace_hold(void);
+int dgami_workspace_release(void);
 
 /* Kernel timing of the trailing-matrix updates of the blocked ladder LU (the dominant
  * kernel for compound sizes above 128). No counterpart in the reference: it is the
@@ -107,7 +126,9 @@ int dgami_chi0_prepare(int nq, int nb, int nw, int niv_full, int niv_core, doubl
  * computed as an LU solve with the nb^2 indicator right-hand sides.
  * chi0_inv: QF (nv = 2*niv); gamma: [nchan] x CMP; u_r: [nchan][nb^4] (M order);
  * v_r: [nchan] x QO or NULL (V = 0); chi_aux_sum: [nchan] x QF; info: [nchan][nw][nq] int32,
- * 0 or k+1 for an exactly singular pivot at elimination step k.
+ * 0 or k+1 for an exactly singular pivot at elimination step k; the mixed forms also report
+ * DGAMI_INFO_REFINE_STALLED / DGAMI_INFO_REFINE_NONFINITE per item. Nothing is read back to the
+ * host in any precision, so every form can be captured in a HIP graph.
  * precision: DGAMI_PLAIN_FP64 / DGAMI_MIXED_FP32_IR / DGAMI_MIXED_BF16X3_IR.
  * Replaces create_auxiliary_chi_r_q_sum_v1/_v3 (nonlocal_sde.py:85-156) with
  * FourPoint.invert_and_sum_over_last_vn/_v2 (four_point.py:620-684). */
