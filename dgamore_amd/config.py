@@ -12,7 +12,8 @@ sys = SimpleNamespace(beta=1.0, mu=0.0, n=0.0, n_bands=1, occ=None, occ_k=None)
 box = SimpleNamespace(niw_core=0, niv_core=0, niv_shell=0, niv_full=0, niv_dmft=0)
 lattice = SimpleNamespace(nk=(1, 1, 1), nq=(1, 1, 1), k_grid=None, q_grid=None)
 self_consistency = SimpleNamespace(mixing=1.0, max_iter=1, epsilon=1e-4, mixing_strategy="linear",
-                                   mixing_history_length=2)
+                                   mixing_history_length=2, previous_sc_path="", restrict_chi_phys=False)
+output = SimpleNamespace(output_path="", eliashberg_path="")
 lambda_correction = SimpleNamespace(perform_lambda_correction=False, type="spch")
 eliashberg = SimpleNamespace(n_eig=4, epsilon=1e-12, symmetry="random", save_fq=False, construct_fq_cheap=False,
                              matvec_precision="fp64")

@@ -181,6 +181,76 @@ class FourPoint:
     def __truediv__(self, other):
         return self.__mul__(1.0 / other)
 
+    def __neg__(self):
+        return self.__mul__(-1.0)
+
+    # -- sums of objects with the same momentum and frequency structure (four_point.py:49-71, 332-440);
+    # the reference also broadcasts a num_vn=1 operand against a num_vn=2 one by its diagonal, which no
+    # call of the hot path needs: mismatched frequency structures raise ValueError here.
+    def _operand(self, other):
+        if isinstance(other, (int, float, complex, np.ndarray)):
+            return other
+        if not isinstance(other, FourPoint):
+            raise ValueError(f"Operations are only supported with FourPoint objects, numbers or arrays, "
+                             f"not {type(other)}.")
+        if other._num_vn_dimensions != self._num_vn_dimensions or other._num_wn_dimensions != self._num_wn_dimensions:
+            raise ValueError("FourPoint objects with different frequency dimensions cannot be added here.")
+        if other._full_niw_range != self._full_niw_range:
+            other = other._like(other.mat.copy())
+            other.to_full_niw_range() if self._full_niw_range else other.to_half_niw_range()
+        if other._has_compressed_q_dimension != self._has_compressed_q_dimension:
+            other = other._like(other.mat)
+            other.compress_q_dimension() if self._has_compressed_q_dimension else other.decompress_q_dimension()
+        if other.mat.shape != self.mat.shape:
+            raise ValueError(f"Shapes {self.mat.shape} and {other.mat.shape} do not match.")
+        return other.mat
+
+    def add(self, other):
+        return self._like(self.mat + self._operand(other))
+
+    def sub(self, other):
+        return self._like(self.mat - self._operand(other))
+
+    __add__ = add
+    __radd__ = add
+    __sub__ = sub
+
+    def __rsub__(self, other):
+        return self._like(self._operand(other) - self.mat)
+
+    def __getitem__(self, item):
+        return self.mat[item]
+
+    def __setitem__(self, key, value):
+        self.mat[key] = value
+
+    def permute_orbitals(self, permutation="abcd->abcd", copy=True):
+        """Orbital permutation given as 'abcd->...' (four_point.py:303-327)."""
+        split = permutation.split("->")
+        if len(split) != 2 or len(split[0]) != 4 or len(split[1]) != 4:
+            raise ValueError("Invalid permutation.")
+        if split[0] == split[1]:
+            return self
+        lead = "i" if self._has_compressed_q_dimension else "ijk"
+        mat = np.einsum(f"{lead}{split[0]}...->{lead}{split[1]}...", self.mat)
+        if not copy:
+            self.mat = mat
+            return self
+        return self._like(np.ascontiguousarray(mat))
+
+    def save(self, output_dir="./", name="please_give_me_a_name"):
+        """np.save of the half bosonic range as {output_dir}/{name}.npy (local_n_point.py:296-303)."""
+        import os
+
+        full = self._full_niw_range
+        np.save(os.path.join(output_dir, f"{name}.npy"), self.to_half_niw_range().mat, allow_pickle=False)
+        if full:
+            self.to_full_niw_range()
+
+    def free(self, trim=False):
+        """Drop the array (n_point_base.py:174-185)."""
+        self.mat = None
+
     # -- GPU-backed methods -------------------------------------------------------------------
     def invert(self, copy=True):
         """Inverse in compound indices, returned in the half bosonic range (four_point.py:590-618).
@@ -248,3 +318,49 @@ class FourPoint:
              frequency_notation=FrequencyNotation.PH):
         return FourPoint(np.load(filename, allow_pickle=False), channel, nq, num_wn_dimensions, num_vn_dimensions,
                          full_niw_range, full_niv_range, has_compressed_q_dimension, frequency_notation)
+
+
+class LocalFourPoint(FourPoint):
+    """Local four-point object [o1, o2, o3, o4, (w), (v), (v')] (local_four_point.py; the constructor
+    arguments of LocalFourPoint.load, local_four_point.py:671-691)."""
+
+    def __init__(self, mat, channel=SpinChannel.NONE, num_wn_dimensions=1, num_vn_dimensions=2,
+                 full_niw_range=True, full_niv_range=True, frequency_notation=FrequencyNotation.PH):
+        super().__init__(mat, channel, (1, 1, 1), num_wn_dimensions, num_vn_dimensions, full_niw_range,
+                         full_niv_range, False, frequency_notation)
+
+    @property
+    def n_bands(self):
+        return self.mat.shape[0]
+
+    @property
+    def nq_tot(self):
+        return 1
+
+    def _like(self, mat):
+        return LocalFourPoint(mat, self.channel, self._num_wn_dimensions, self._num_vn_dimensions,
+                              self._full_niw_range, self._full_niv_range, self.frequency_notation)
+
+    def compress_q_dimension(self):
+        return self
+
+    def decompress_q_dimension(self):
+        return self
+
+    def permute_orbitals(self, permutation="abcd->abcd", copy=True):
+        split = permutation.split("->")
+        if len(split) != 2 or len(split[0]) != 4 or len(split[1]) != 4:
+            raise ValueError("Invalid permutation.")
+        if split[0] == split[1]:
+            return self
+        mat = np.einsum(f"{split[0]}...->{split[1]}...", self.mat)
+        if not copy:
+            self.mat = mat
+            return self
+        return self._like(np.ascontiguousarray(mat))
+
+    @staticmethod
+    def load(filename, channel=SpinChannel.NONE, num_wn_dimensions=1, num_vn_dimensions=2, full_niw_range=False,
+             full_niv_range=True, frequency_notation=FrequencyNotation.PH):
+        return LocalFourPoint(np.load(filename, allow_pickle=False), channel, num_wn_dimensions, num_vn_dimensions,
+                              full_niw_range, full_niv_range, frequency_notation)

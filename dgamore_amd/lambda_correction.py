@@ -56,3 +56,38 @@ def perform_single_lambda_correction(chi_r, chi_r_loc_sum):
     lam = find_lambda(chi_mat, chi_r_loc_sum)
     chi_r.mat = apply_lambda(chi_mat, lam)[:, None, None, None, None, :]
     return chi_r.to_half_niw_range(), lam
+
+
+def perform_lambda_correction(chi_phys_q_r):
+    """lambda correction of one channel's physical susceptibility with the local sums read from the
+    files the reference reads (nonlocal_sde.py:339-401): chi_{dens,magn}_loc.npy and, for 'sp',
+    chi_phys_q_dens.npy from config.output.output_path; lambda is appended to lambda_{type}.txt.
+    'spch' corrects both channels, 'sp' only the magnetic one."""
+    import os
+
+    from dgamore_amd.four_point import FourPoint, LocalFourPoint, SpinChannel
+
+    kind = config.lambda_correction.type.lower()
+    if kind not in ("spch", "sp"):
+        raise ValueError("Lambda correction type must be either 'spch' or 'sp'.")
+    out = config.output.output_path
+    ch = chi_phys_q_r.channel
+
+    def loc(channel):
+        return LocalFourPoint.load(os.path.join(out, f"chi_{channel.value}_loc.npy"), channel,
+                                   num_vn_dimensions=0).to_full_niw_range()
+
+    if kind == "spch":
+        target = loc(ch).mat.sum() / config.sys.beta
+    elif ch != SpinChannel.MAGN:
+        return chi_phys_q_r
+    else:
+        dens = FourPoint.load(os.path.join(out, "chi_phys_q_dens.npy"), SpinChannel.DENS,
+                              num_vn_dimensions=0).to_full_niw_range()
+        counts = np.asarray(config.lattice.q_grid.irrk_count)[:, None, None, None, None, None]
+        total = (loc(SpinChannel.DENS).mat + loc(SpinChannel.MAGN).mat).sum()
+        target = (total - (counts * dens.mat).sum() / config.lattice.q_grid.nk_tot) / config.sys.beta
+    chi_phys_q_r, lam = perform_single_lambda_correction(chi_phys_q_r, target)
+    with open(os.path.join(out, f"lambda_{config.lambda_correction.type}.txt"), "a") as f:
+        f.write(f"lambda_{ch.value}: {lam}\n")
+    return chi_phys_q_r

@@ -12,6 +12,10 @@ Interactions: ``u_loc`` is a [nb]*4 array or an object with ``.mat``;
 ``.mat``, or None for V = 0.
 """
 
+import glob
+import os
+import re
+
 import numpy as np
 
 from dgamore_amd import config
@@ -172,13 +176,17 @@ def calculate_kernel_r_q(vrg_q_r, gchi_aux_q_r_sum, v_nonloc, u_loc):
 
 
 def calculate_sigma_from_kernel_fft_auto(mpi_distributor, kernel, giwk):
-    """Sigma(k, v) in the full fermionic range from the irreducible kernel.
+    """Sigma from the irreducible kernel, returned as the reference returns it: a SelfEnergy in real
+    space R, nu >= 0, compressed momentum axis (nonlocal_sde.py:544-594, 650-674), which the caller
+    gathers, ifft's and extends to the full fermionic range (nonlocal_sde.py:874-879).
 
-    Fuses calculate_sigma_from_kernel_fft_* (nonlocal_sde.py:544-674), the irr->FBZ map
-    (mpi_utils.py:123-131) and the caller's ifft + to_full_niv_range (nonlocal_sde.py:874-879):
-    the return value is the k-space self-energy array [nx,ny,nz,nb,nb,2niv_core].
+    The irr->FBZ map (mpi_utils.py:123-131) is fused into the Sigma kernel, which produces
+    Sigma(k, nu); its nu >= 0 half is brought to R by the device FFT, so that SelfEnergy.ifft()
+    returns Sigma(k, nu) exactly.
     """
     import torch
+
+    from dgamore_amd.self_energy import SelfEnergy
 
     beta = config.sys.beta
     grid = config.lattice.q_grid
@@ -187,14 +195,42 @@ def calculate_sigma_from_kernel_fft_auto(mpi_distributor, kernel, giwk):
         raise ValueError("kernel must be given on the irreducible q-points of config.lattice.q_grid")
     g = _mat(giwk)
     niw = ker.mat.shape[5] - 1
+    nk = tuple(grid.nk)
     sig = K.sigma_from_kernel(K.ref4_to_qf(K.to_dev(ker.mat)), K.to_dev(np.asarray(grid.irrk_inv), torch.int32),
-                              K.to_dev(g), tuple(grid.nk), niw, beta)
-    return sig.cpu().numpy()
+                              K.to_dev(g), nk, niw, beta)  # [nx,ny,nz,nb,nb,2niv] in k
+    nb, nv = sig.shape[3], sig.shape[-1]
+    half = sig[..., nv // 2:].reshape(-1, nb * nb * (nv // 2)).T.contiguous()  # [rest][N]
+    K.fft_grid(half, nk, -1, 1.0)  # k -> R, exp(-i k.R), unnormalised (scipy.fft.fftn)
+    mat = half.T.contiguous().cpu().numpy().reshape(-1, nb, nb, nv // 2)
+    return SelfEnergy(mat, config.lattice.nk, full_niv_range=False, has_compressed_q_dimension=True, calc_smom=False)
 
 
 def calculate_sigma_from_kernel_auto(mpi_distributor, kernel, giwk, my_full_q_list=None):
-    """Direct q-sum form of the reference (nonlocal_sde.py:404-541); same values by the FFT route."""
-    return calculate_sigma_from_kernel_fft_auto(mpi_distributor, kernel, giwk)
+    """Direct q-sum form of the reference (nonlocal_sde.py:404-541); same values by the FFT route.
+    The reference's direct form returns Sigma(k, nu >= 0) on the momentum grid already: the same
+    object after ifft."""
+    return calculate_sigma_from_kernel_fft_auto(mpi_distributor, kernel, giwk).ifft()
+
+
+def calculate_sigma_kernel_r_q(gamma_r, gchi0_q_inv, gchi0_q_full_sum, gchi0_q_core_sum, u_loc, v_nonloc,
+                               mpi_dist_irrq=None):
+    """The kernel of one spin channel (nonlocal_sde.py:282-336): ladder sums, three-leg vertex,
+    shell-corrected physical susceptibility (lambda-corrected if config.lambda_correction asks for
+    it; single band), then K_r. With config.output.output_path set, chi_phys_q_{channel} is saved
+    in the half bosonic range as the reference saves it."""
+    aux = create_auxiliary_chi_r_q_sum_v1(gamma_r, gchi0_q_inv, u_loc, v_nonloc)
+    vrg = create_vrg_r_q(aux, gchi0_q_inv)
+    chi = create_generalized_chi_q_with_shell_correction(aux.sum_over_all_vn(config.sys.beta), gchi0_q_full_sum,
+                                                         gchi0_q_core_sum, u_loc, v_nonloc)
+    aux.free()
+    if getattr(config.lambda_correction, "perform_lambda_correction", False):
+        from dgamore_amd.lambda_correction import perform_lambda_correction
+
+        chi = perform_lambda_correction(chi)
+    out = getattr(getattr(config, "output", None), "output_path", None)
+    if out:
+        chi.save(name=f"chi_phys_q_{chi.channel.value}", output_dir=out)
+    return calculate_kernel_r_q(vrg, chi, v_nonloc, u_loc)
 
 
 def get_hartree_fock(u_loc, v_nonloc, q_list=None):
@@ -233,57 +269,111 @@ def assemble_sigma(sigma_nl, hartree, fock, delta_sigma, sigma_dmft, niv_out, si
     return out.cpu().numpy()
 
 
-def apply_mixing_strategy(sigma_new, sigma_old, sigma_dmft, current_iter, history=None):
-    """Mixing of the self-energy (nonlocal_sde.py:979-1151). Arrays [nk,nb,nb,2niv].
+def get_starting_sigma(output_path, default_sigma):
+    """(Sigma, iteration) of the last sigma_dga_iteration_N.npy in output_path, or (default_sigma, 0)
+    when there is none (nonlocal_sde.py:677-696)."""
+    from dgamore_amd.self_energy import SelfEnergy
 
-    linear: alpha sigma_new + (1 - alpha) sigma_old on the device. pulay / anderson (after
-    mixing_history_length iterations) follow the reference's truncated-SVD least squares on the
-    core window in FP64 on the host; `history` holds the previous iterations' self-energies
-    (the reference reads them from its output files), oldest first.
+    if output_path == "" or output_path is None or not os.path.exists(output_path):
+        return default_sigma, 0
+    found = _sigma_files([output_path])
+    if not found:
+        return default_sigma, 0
+    it, path = found[-1]
+    return SelfEnergy(np.load(path, allow_pickle=False), config.lattice.nk, True, True, False), it
+
+
+def read_last_n_sigmas_from_files(n, output_path="./", previous_sc_path="./"):
+    """The last n saved self-energies, oldest first, from output_path and previous_sc_path together
+    (nonlocal_sde.py:699-716)."""
+    dirs = [output_path]
+    if previous_sc_path != "" and previous_sc_path is not None and os.path.exists(previous_sc_path):
+        dirs.append(previous_sc_path)
+    return [np.load(path, allow_pickle=False) for _, path in _sigma_files(dirs)[-n:]]
+
+
+def _sigma_files(dirs):
+    """[(iteration, path)] of every sigma_dga_iteration_N.npy in dirs, by iteration (a directory
+    given twice is listed twice, as in the reference)."""
+    out = []
+    for d in dirs:
+        for f in glob.glob(os.path.join(d, "sigma_dga_iteration_*.npy")):
+            m = re.search(r"sigma_dga_iteration_(\d+)\.npy$", f)
+            if m:
+                out.append((int(m.group(1)), f))
+    return sorted(out, key=lambda x: x[0])
+
+
+def apply_mixing_strategy(sigma_new, sigma_old, sigma_dmft, current_iter, history=None):
+    """Mixing of the self-energy (nonlocal_sde.py:979-1151). sigma_*: SelfEnergy objects (or arrays
+    [nk,nb,nb,2niv]; sigma_dmft also local).
+
+    linear: alpha sigma_new + (1 - alpha) sigma_old on the device, a new object. pulay / anderson
+    (after mixing_history_length iterations): the reference's truncated-SVD least squares on the
+    core window in FP64 on the host, written into sigma_new, which is returned (the frequencies
+    outside the core window stay sigma_new's). The previous self-energies are read with
+    read_last_n_sigmas_from_files from config.output.output_path and
+    config.self_consistency.previous_sc_path, as in the reference, unless `history` (oldest
+    first) is given. A singular history falls back to linear mixing.
     """
+    from dgamore_amd.self_energy import SelfEnergy
+
     n_hist = config.self_consistency.mixing_history_length
     alpha = config.self_consistency.mixing
     strategy = config.self_consistency.mixing_strategy.lower()
-    new = np.asarray(sigma_new)
-    old = np.asarray(sigma_old)
-    if strategy in ("pulay", "anderson") and current_iter > n_hist and history is not None:
-        return _history_mixing(strategy, new, old, np.asarray(sigma_dmft), list(history)[-n_hist:], n_hist, alpha)
+    is_obj = isinstance(sigma_new, SelfEnergy)
+    new = sigma_new.mat if is_obj else np.asarray(sigma_new)
+    old = np.asarray(_mat(sigma_old))
+    if strategy in ("pulay", "anderson") and current_iter > n_hist:
+        if history is None:
+            history = read_last_n_sigmas_from_files(n_hist, config.output.output_path,
+                                                    config.self_consistency.previous_sc_path)
+        core = _history_mixing(strategy, new, np.asarray(_mat(sigma_dmft)), list(history)[-n_hist:], n_hist, alpha)
+        if core is not None:
+            niv, nc = new.shape[-1] // 2, config.box.niv_core
+            out = new if is_obj else new.copy()
+            out[..., niv - nc:niv + nc] = core
+            return sigma_new if is_obj else out
     nb, nv = new.shape[-2], new.shape[-1]
-    out = K.sigma_assemble(K.to_dev(new.reshape(-1, nb, nb, nv)), None, None, None,
-                           K.to_dev(new.reshape(-1, nb, nb, nv)[0]), nv // 2,
-                           K.to_dev(old.reshape(-1, nb, nb, nv)), alpha)
-    return out.cpu().numpy().reshape(new.shape)
+    flat = new.reshape(-1, nb, nb, nv)
+    old = np.broadcast_to(old.reshape(-1, nb, nb, nv), flat.shape)  # a local sigma_old applies to every k
+    mixed = K.sigma_assemble(K.to_dev(flat), None, None, None, K.to_dev(flat[0]), nv // 2, K.to_dev(old), alpha)
+    mixed = mixed.cpu().numpy().reshape(new.shape)
+    if is_obj:
+        return SelfEnergy(mixed, sigma_new.nq, sigma_new.full_niv_range, sigma_new.has_compressed_q_dimension)
+    return mixed
 
 
-def _history_mixing(strategy, new, old, sigma_dmft, hist, n_hist, alpha):
-    """Pulay / Anderson step on the core window (nonlocal_sde.py:990-1143), FP64 host least squares."""
+def _history_mixing(strategy, new, sigma_dmft, hist, n_hist, alpha):
+    """Pulay / Anderson step on the core window (nonlocal_sde.py:990-1143), FP64 host least squares.
+    Returns the new core window [..., 2 niv_core], or None when every singular value of the history
+    is below the cutoff (the reference then mixes linearly)."""
     niv = new.shape[-1] // 2
     nc = config.box.niv_core
     sl = slice(niv - nc, niv + nc)
     dmft = np.broadcast_to(sigma_dmft.reshape((1,) + sigma_dmft.shape[-3:]), new.shape)
-    proposals = [np.asarray(dmft)] + hist
-    results = hist + [new]
-    proposals = [p[..., sl].reshape(-1) for p in proposals]
-    results = [r[..., sl].reshape(-1) for r in results]
+    proposals = [np.asarray(dmft)] + [np.asarray(h).reshape(new.shape) for h in hist]
+    results = [np.asarray(h).reshape(new.shape) for h in hist] + [new]
+    shape = new[..., sl].shape
+    proposals = [p[..., sl].astype(np.complex128).reshape(-1) for p in proposals]
+    results = [r[..., sl].astype(np.complex128).reshape(-1) for r in results]
     ri = lambda z: np.concatenate([z.real, z.imag])  # noqa: E731
     f_i = ri(results[-1] - proposals[-1])
     norm_f = np.linalg.norm(f_i)
-    out = new.copy()
     if strategy == "pulay":
         r_m = np.column_stack([ri(proposals[-1 - i] - proposals[-2 - i]) for i in range(n_hist)])
         f_m = np.column_stack([ri(results[-1 - i] - results[-2 - i]) for i in range(n_hist)]) - r_m
         u, s, vh = np.linalg.svd(f_m, full_matrices=False)
         mask = s > 1e-5 * (s[0] if s.size else 1.0)
         if not np.any(mask):
-            return alpha * new + (1 - alpha) * old
+            return None
         coeffs = vh[mask].T @ ((u[:, mask].T @ f_i) / s[mask])
         update = alpha * f_i - (r_m + alpha * f_m) @ coeffs
         norm_u = np.linalg.norm(update)
         if norm_f > 0 and norm_u > 10.0 * norm_f:
             update *= 10.0 * norm_f / norm_u
         n = update.size // 2
-        out[..., sl] = (proposals[-1] + update[:n] + 1j * update[n:]).reshape(out[..., sl].shape)
-        return out
+        return (proposals[-1] + update[:n] + 1j * update[n:]).reshape(shape)
     dx = np.column_stack([ri(proposals[-1 - i] - proposals[-2 - i]) for i in range(n_hist)])
     df = np.column_stack([ri((results[-1 - i] - proposals[-1 - i]) - (results[-2 - i] - proposals[-2 - i]))
                           for i in range(n_hist)])
@@ -291,7 +381,7 @@ def _history_mixing(strategy, new, old, sigma_dmft, hist, n_hist, alpha):
     cutoff = 1e-5 * (s[0] if s.size else 1.0)
     mask = s > cutoff
     if not np.any(mask):
-        return alpha * new + (1 - alpha) * old
+        return None
     coeffs = vh[mask].T @ ((s[mask] / (s[mask] ** 2 + cutoff**2)) * (u[:, mask].T @ f_i))
     x_n = proposals[-1]
     xa = ri(x_n) + f_i - (dx + df) @ coeffs
@@ -301,6 +391,4 @@ def _history_mixing(strategy, new, old, sigma_dmft, hist, n_hist, alpha):
     norm_u = np.linalg.norm(upd)
     if norm_f > 0 and norm_u > 3.0 * norm_f:
         cand = x_n + upd * (3.0 * norm_f / norm_u)
-    out[..., sl] = cand.reshape(out[..., sl].shape)
-    return out
-
+    return cand.reshape(shape)

@@ -24,3 +24,176 @@ def fit_smom(sigma_mat, beta):
     fit = loc[..., nv - n_fit:]
     v = np.pi / beta * (2 * np.arange(niv - n_fit, niv) + 1)
     return fit.real.mean(axis=-1), (fit.imag * v[None, None, :]).mean(axis=-1)
+
+
+def _momentum_fft(mat, nk, sign, scale):
+    """FFT over the leading [nx, ny, nz] axes of a host array on the device (dgami_fft_grid)."""
+    nx, ny, nz = nk
+    rest = mat.shape[3:]
+    x = K.to_dev(mat.reshape(nx * ny * nz, -1)).T.contiguous()  # [rest][N]
+    K.fft_grid(x, (nx, ny, nz), sign, scale)
+    return x.T.contiguous().cpu().numpy().reshape((nx, ny, nz) + rest)
+
+
+class SelfEnergy:
+    """The reference's SelfEnergy container (self_energy.py:20-168, with the IHaveMat arithmetic of
+    n_point_base.py:107-187 and the momentum FFTs of n_point_base.py:509-551).
+
+    mat: [q, nb, nb, nv] (compressed momentum) or [nx, ny, nz, nb, nb, nv], complex128 (the
+    reference casts to complex64, n_point_base.py:64-74). full_niv_range=False holds nu >= 0 only.
+    The momentum FFTs run on the GPU; the local moments (smom) are fitted on first use instead of
+    at construction, so building a container needs no device."""
+
+    def __init__(self, mat, nk=(1, 1, 1), full_niv_range=True, has_compressed_q_dimension=False,
+                 estimate_niv_core=False, calc_smom=True):
+        self.mat = np.asarray(mat, dtype=np.complex128)
+        self._nq = tuple(nk)
+        self._full_niv_range = full_niv_range
+        self._has_compressed_q_dimension = has_compressed_q_dimension
+        self._smom = None
+
+    # -- shape bookkeeping (local_n_point.py:84-108, n_point_base.py:306-404) ----------------------
+    @property
+    def nq(self):
+        return self._nq
+
+    @property
+    def nq_tot(self):
+        return int(np.prod(self._nq))
+
+    @property
+    def has_compressed_q_dimension(self):
+        return self._has_compressed_q_dimension
+
+    @property
+    def full_niv_range(self):
+        return self._full_niv_range
+
+    @property
+    def current_shape(self):
+        return self.mat.shape
+
+    @property
+    def n_bands(self):
+        return self.mat.shape[-2]
+
+    @property
+    def niv(self):
+        return self.mat.shape[-1] // 2
+
+    def compress_q_dimension(self):
+        if not self._has_compressed_q_dimension:
+            self.mat = self.mat.reshape((-1,) + self.mat.shape[3:])
+            self._has_compressed_q_dimension = True
+        return self
+
+    def decompress_q_dimension(self):
+        if self._has_compressed_q_dimension:
+            self.mat = self.mat.reshape(tuple(self._nq) + self.mat.shape[1:])
+            self._has_compressed_q_dimension = False
+        return self
+
+    def _like(self, mat, full=None):
+        return SelfEnergy(mat, self._nq, self._full_niv_range if full is None else full,
+                          self._has_compressed_q_dimension)
+
+    # -- arithmetic (n_point_base.py:107-136, self_energy.py:122-151) ----------------------------
+    def __mul__(self, other):
+        if not isinstance(other, (int, float, complex)):
+            raise ValueError("Multiplication only supported with numbers or numpy arrays.")
+        return self._like(self.mat * other)
+
+    __rmul__ = __mul__
+
+    def __neg__(self):
+        return self.__mul__(-1.0)
+
+    def __truediv__(self, other):
+        if not isinstance(other, (int, float, complex)):
+            raise ValueError("Division only supported with numbers.")
+        return self.__mul__(1.0 / other)
+
+    def add(self, other):
+        if isinstance(other, np.ndarray):
+            return self._like(self.mat + other)
+        if not isinstance(other, SelfEnergy):
+            raise ValueError(f"Can not add {type(other)} to {type(self)}.")
+        if other.has_compressed_q_dimension != self._has_compressed_q_dimension:
+            other = SelfEnergy(other.mat, other.nq, other.full_niv_range, other.has_compressed_q_dimension)
+            other = other.compress_q_dimension() if self._has_compressed_q_dimension else other.decompress_q_dimension()
+        return self._like(self.mat + other.mat)
+
+    def sub(self, other):
+        return self.add(-other)
+
+    __add__ = add
+    __sub__ = sub
+
+    def __getitem__(self, item):
+        return self.mat[item]
+
+    def __setitem__(self, key, value):
+        self.mat[key] = value
+
+    # -- frequency ranges (self_energy.py:109-120, 153-168; local_n_point.py:134-154) -------------
+    def to_full_niv_range(self):
+        if self._full_niv_range:
+            return self
+        self.mat = np.concatenate((np.conj(np.flip(self.mat, axis=-1)), self.mat), axis=-1)
+        self._full_niv_range = True
+        return self
+
+    def cut_niv(self, niv_cut):
+        if niv_cut > self.niv:
+            return self
+        s = slice(self.niv - niv_cut, self.niv + niv_cut) if self._full_niv_range else slice(0, niv_cut)
+        return self._like(np.ascontiguousarray(self.mat[..., s]))
+
+    def concatenate_self_energies(self, other):
+        """This self-energy inside, `other` (local or on the same grid) outside, up to other.niv."""
+        if self.niv > other.niv:
+            raise ValueError("Can not concatenate with a self-energy that has less frequencies.")
+        d = other.niv - self.niv
+        self.compress_q_dimension()
+        om = other.mat.reshape((-1,) + other.mat.shape[-3:])
+        if om.shape[0] == 1:
+            om = np.broadcast_to(om, (self.mat.shape[0],) + om.shape[1:])
+        mat = np.concatenate((om[..., :d], self.mat, om[..., d + 2 * self.niv:]), axis=-1)
+        return SelfEnergy(mat, self._nq, self._full_niv_range, True)
+
+    # -- momentum FFTs on the device (n_point_base.py:509-551) -----------------------------------
+    def _fft(self, sign, scale, copy):
+        obj = self._like(self.mat.copy()) if copy else self
+        compress = obj._has_compressed_q_dimension
+        obj.decompress_q_dimension()
+        obj.mat = _momentum_fft(obj.mat, obj._nq, sign, scale)
+        return obj.compress_q_dimension() if compress else obj
+
+    def fft(self, copy=True):
+        """Forward, unnormalised: sum_k x(k) exp(-i k.R) (scipy.fft.fftn)."""
+        return self._fft(-1, 1.0, copy)
+
+    def ifft(self, copy=True):
+        """Backward with 1/N (scipy.fft.ifftn)."""
+        return self._fft(+1, 1.0 / self.nq_tot, copy)
+
+    # -- moments, files, memory -------------------------------------------------------------------
+    def fit_smom(self):
+        from dgamore_amd import config
+
+        return fit_smom(self.mat.reshape((-1,) + self.mat.shape[-3:]), config.sys.beta)
+
+    @property
+    def smom(self):
+        if self._smom is None:
+            self._smom = self.fit_smom()
+        return self._smom
+
+    def save(self, output_dir="./", name="please_give_me_a_name"):
+        """np.save of mat as {output_dir}/{name}.npy (local_n_point.py:296-303)."""
+        import os
+
+        np.save(os.path.join(output_dir, f"{name}.npy"), self.mat, allow_pickle=False)
+
+    def free(self, trim=False):
+        self.mat = None
