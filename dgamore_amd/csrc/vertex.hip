@@ -194,9 +194,11 @@ __global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedStat
 // with off = niv - np, zero where |w| >= nw; gamma_sing = 0.5 F_pp,d - 1.5 F_pp,m and
 // gamma_trip = 0.5 F_pp,d + 0.5 F_pp,m; without fm, gs = F_pp of fd alone. One thread per
 // output element, j fastest.
+// f_dens / f_magn hold the bosonic frequencies [w0, w0 + nw); entries whose |w| lies outside that
+// range are zeroed (fill) or left as they are (a later call of another range writes them).
 __global__ void __launch_bounds__(256) vtx_pp_kernel(const double2* __restrict__ fd, const double2* __restrict__ fm,
                                                      double2* __restrict__ gs, double2* __restrict__ gt, int nq, int nb,
-                                                     int nw, int niv, int np) {
+                                                     int nw, int niv, int np, int w0, int fill) {
   const int T = nb * nb, TT = T * T, P2 = 2 * np, nv = 2 * niv, n = T * nv, off = niv - np;
   const size_t total = static_cast<size_t>(nq) * TT * P2 * P2;
   for (size_t idx = static_cast<size_t>(blockIdx.x) * blockDim.x + threadIdx.x; idx < total;
@@ -208,13 +210,15 @@ __global__ void __launch_bounds__(256) vtx_pp_kernel(const double2* __restrict__
     const int o4 = o % nb, o3 = (o / nb) % nb, o2 = (o / T) % nb, o1 = o / (T * nb);
     const int ri = o1 * nb + o2, cj = o4 * nb + o3;
     const int w = i - j;
+    const int lw = (w >= 0 ? w : -w) - w0;  // position of |w| in the given range
+    if (!fill && (lw < 0 || lw >= nw)) continue;
     double2 vd = make_double2(0.0, 0.0), vm = vd;
-    if (w >= 0 && w < nw) {
-      const size_t e = ((static_cast<size_t>(w) * nq + q) * n + ri * nv + i + off) * n + cj * nv + (P2 - 1 - j) + off;
+    if (w >= 0 && lw >= 0 && lw < nw) {
+      const size_t e = ((static_cast<size_t>(lw) * nq + q) * n + ri * nv + i + off) * n + cj * nv + (P2 - 1 - j) + off;
       vd = fd[e];
       if (fm) vm = fm[e];
-    } else if (w < 0 && -w < nw) {
-      const size_t e = ((static_cast<size_t>(-w) * nq + q) * n + ri * nv + (P2 - 1 - i) + off) * n + cj * nv + j + off;
+    } else if (w < 0 && lw >= 0 && lw < nw) {
+      const size_t e = ((static_cast<size_t>(lw) * nq + q) * n + ri * nv + (P2 - 1 - i) + off) * n + cj * nv + j + off;
       vd = cconj(fd[e]);
       if (fm) vm = cconj(fm[e]);
     }
@@ -319,6 +323,21 @@ extern "C" int dgami_ladder_invert(int nq, int nb, int nw, int niv, double beta,
   return vertex_solve(a, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
 }
 
+extern "C" int dgami_invert_dense(int nmat, int n, const dgami_c128* a, dgami_c128* a_inv, int32_t* info,
+                                  void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(a && a_inv, "invert_dense: null pointer");
+  DGAMI_ARG_CHECK(nmat >= 1 && n >= 1 && n <= BLK_MAX_N, "invert_dense: invalid sizes (1 <= n <= 1536)");
+  LadderArgs la = vertex_ladder_args(nmat, 1, 1, 1, 1.0, nullptr, a, nullptr, nullptr, info);
+  la.dense = 1;  // item q reads its matrix from a + q n^2
+  la.T = 1;
+  la.nv = n;
+  la.n = n;
+  la.inv_beta2 = 1.0;
+  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(a_inv), 0.0};
+  return vertex_solve(la, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
+}
+
 extern "C" int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
                                  const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r,
                                  const dgami_c128* vrg, const dgami_c128* chi_phys, dgami_c128* out, int32_t* info,
@@ -333,20 +352,35 @@ extern "C" int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, c
   return vertex_solve(a, VTX_OUT_FULL, o, static_cast<hipStream_t>(stream));
 }
 
+namespace dgami {
+static int pairing_vertex_pp(int nq, int nb, int nw, int w0, int fill, int niv, int np, const dgami_c128* f_dens,
+                             const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip,
+                             hipStream_t stream) {
+  DGAMI_ARG_CHECK(f_dens && gamma_sing, "pairing_vertex_pp: null pointer");
+  DGAMI_ARG_CHECK(f_magn || !gamma_trip, "pairing_vertex_pp: the triplet vertex needs f_magn");
+  DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && w0 >= 0 && niv >= 1 && np >= 1 && np <= niv,
+                  "pairing_vertex_pp: invalid sizes (need 1 <= np <= niv, w0 >= 0)");
+  const size_t total = static_cast<size_t>(nq) * nb * nb * nb * nb * 4 * np * np;
+  const size_t blocks = ceil_div(total, 256);
+  hipLaunchKernelGGL(vtx_pp_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0, stream,
+                     reinterpret_cast<const double2*>(f_dens), reinterpret_cast<const double2*>(f_magn),
+                     reinterpret_cast<double2*>(gamma_sing), reinterpret_cast<double2*>(gamma_trip), nq, nb, nw, niv,
+                     np, w0, fill);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+}  // namespace dgami
+
 extern "C" int dgami_pairing_vertex_pp(int nq, int nb, int nw, int niv, int np, const dgami_c128* f_dens,
                                        const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip,
                                        void* stream) {
-  using namespace dgami;
-  DGAMI_ARG_CHECK(f_dens && gamma_sing, "pairing_vertex_pp: null pointer");
-  DGAMI_ARG_CHECK(f_magn || !gamma_trip, "pairing_vertex_pp: the triplet vertex needs f_magn");
-  DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && np >= 1 && np <= niv,
-                  "pairing_vertex_pp: invalid sizes (need 1 <= np <= niv)");
-  const size_t total = static_cast<size_t>(nq) * nb * nb * nb * nb * 4 * np * np;
-  const size_t blocks = ceil_div(total, 256);
-  hipLaunchKernelGGL(vtx_pp_kernel, dim3(static_cast<unsigned>(blocks < 65536 ? blocks : 65536)), dim3(256), 0,
-                     static_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(f_dens),
-                     reinterpret_cast<const double2*>(f_magn), reinterpret_cast<double2*>(gamma_sing),
-                     reinterpret_cast<double2*>(gamma_trip), nq, nb, nw, niv, np);
-  DGAMI_HIP_CHECK(hipGetLastError());
-  return 0;
+  return dgami::pairing_vertex_pp(nq, nb, nw, 0, 1, niv, np, f_dens, f_magn, gamma_sing, gamma_trip,
+                                  static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_pairing_vertex_pp_range(int nq, int nb, int nw, int w0, int niv, int np, const dgami_c128* f_dens,
+                                             const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip,
+                                             void* stream) {
+  return dgami::pairing_vertex_pp(nq, nb, nw, w0, 0, niv, np, f_dens, f_magn, gamma_sing, gamma_trip,
+                                  static_cast<hipStream_t>(stream));
 }

@@ -78,25 +78,56 @@ def create_pairing_vertices_pp(f_dens, f_magn, niv_pp):
             FourPoint(gt.cpu().numpy(), SpinChannel.TRIP, nq, 0, 2, False, True, True, FrequencyNotation.PP))
 
 
+def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta, nb, niv_pp, w_chunk=16):
+    """Singlet and triplet pairing vertices [nq, nb, nb, nb, nb, 2np, 2np] on the device from the
+    ladder of both channels (the chain of solve, eliashberg_solver.py:584-629, with
+    create_full_vertex_q_r, :78-147, and the w = 0 transform, :44-75).
+
+    chi0_inv: QF [nw][nb^4][2niv][nq] (niv >= niv_pp; the reference's construct_fq_cheap cuts its
+    operands to niv_pp first); gammas: [2][nw][n][n] (density, magnetic) compound local vertices;
+    u_rs: [2][nb^4]; v_rs: [2][nq][nb^4] or None; vrgs: [2] x QF; chis: [2] x QW. The particle-hole
+    full vertices are formed for w_chunk bosonic frequencies at a time and transformed before the
+    next range, so F of every w is never held at once; pp entries with |v - v'| >= nw stay zero,
+    as in the reference."""
+    import torch
+
+    nw, nb4, nv, nq = chi0_inv.shape
+    shape = (nq, nb, nb, nb, nb, 2 * niv_pp, 2 * niv_pp)
+    gs = torch.zeros(shape, dtype=K.C128, device=chi0_inv.device)
+    gt = torch.zeros(shape, dtype=K.C128, device=chi0_inv.device)
+    for w0 in range(0, nw, w_chunk):
+        w1 = min(nw, w0 + w_chunk)
+        fs = [K.full_vertex(chi0_inv[w0:w1].contiguous(), gammas[c, w0:w1].contiguous(), u_rs[c],
+                            None if v_rs is None else v_rs[c], vrgs[c][w0:w1].contiguous(), chis[c][w0:w1].contiguous(),
+                            beta)[0] for c in range(2)]
+        K.pairing_vertex_pp_range(fs[0], fs[1], nb, niv_pp, w0, gs, gt)
+        del fs
+    return gs, gt
+
+
 class PairingOperator:
     """Device-resident linearised gap operator of one channel."""
 
     def __init__(self, gamma_pp_irr, chi0pp, q_grid, beta, channel, matrix_precision="fp64", orbital_transforms=None):
-        """matrix_precision "fp32" stores the transformed vertex in complex64 (half the bytes per
-        application); the vectors and the sums stay complex128. orbital_transforms: per-k (us,
-        conjs) of the irreducible-zone map (the reference's auto symmetry mode) or None."""
+        """gamma_pp_irr, chi0pp: arrays or device tensors. matrix_precision "fp32" stores the
+        transformed vertex in complex64 (half the bytes per application); the vectors and the
+        sums stay complex128. orbital_transforms: per-k (us, conjs) of the irreducible-zone map
+        (the reference's auto symmetry mode) or None."""
         import torch
+
+        def dev(x):
+            return x.contiguous() if isinstance(x, torch.Tensor) else K.to_dev(np.asarray(x))
 
         self.nk = tuple(q_grid.nk)
         self.channel = channel
         self.beta = beta
-        gam = np.asarray(gamma_pp_irr)
+        gam = dev(gamma_pp_irr)
         self.nb, self.np2 = gam.shape[1], gam.shape[-1]
         storage = {"fp64": torch.complex128, "fp32": torch.complex64}[matrix_precision]
-        self.g_hat, self.g_flip = K.pairing_setup(K.to_dev(gam), K.to_dev(np.asarray(q_grid.irrk_inv), torch.int32),
+        self.g_hat, self.g_flip = K.pairing_setup(gam, K.to_dev(np.asarray(q_grid.irrk_inv), torch.int32),
                                                   self.nk, channel, storage, orbital_transforms)
         n_k = int(np.prod(self.nk))
-        self.chi0pp = K.to_dev(np.asarray(chi0pp).reshape(n_k, self.nb**4, self.np2))
+        self.chi0pp = dev(chi0pp).reshape(n_k, self.nb**4, self.np2).contiguous()
         self.shape = self.nk + (self.nb, self.nb, self.np2)
         self.dim = int(np.prod(self.shape))
 
@@ -125,3 +156,18 @@ def solve_eliashberg_lanczos(gamma_r_pp, gchi0_q0_pp, ranks=(0, 0), v0=None):
     order = lambdas.argsort()[::-1]
     lambdas, gaps = lambdas[order], gaps[:, order]
     return lambdas, [gaps[:, i].reshape(op.shape) for i in range(n_eig)]
+
+
+def solve_eliashberg_arnoldi(op, n_eig=4, tol=1e-12, v0=None, which="LR"):
+    """Leading eigenpairs of the pairing operator by ARPACK's non-Hermitian Arnoldi (eigs) around the
+    device matvec, with their residuals |A x - lambda x| / |lambda|. No counterpart in the reference,
+    which runs the Hermitian eigsh (eliashberg_solver.py:430-443) on this operator although it is not
+    Hermitian; eigs solves the same eigenproblem without that assumption. Returns (lambdas sorted by
+    real part, descending; gaps [n_eig][nx,ny,nz,nb,nb,2np]; residuals)."""
+    lin = spla.LinearOperator(shape=(op.dim, op.dim), matvec=op.matvec, dtype=np.complex128)
+    lambdas, vecs = spla.eigs(lin, k=n_eig, which=which, tol=tol, v0=v0, maxiter=20000)
+    order = np.argsort(-lambdas.real)
+    lambdas, vecs = lambdas[order], vecs[:, order]
+    res = np.array([np.linalg.norm(op.matvec(vecs[:, i]) - lambdas[i] * vecs[:, i]) /
+                    (abs(lambdas[i]) * np.linalg.norm(vecs[:, i])) for i in range(n_eig)])
+    return lambdas, [vecs[:, i].reshape(op.shape) for i in range(n_eig)], res

@@ -241,6 +241,18 @@ def ladder_invert(chi0_inv, gamma_cmp, u_r_m, v_r_m, beta, check_info=True):
     return out, info
 
 
+def invert_dense(a, check_info=True):
+    """Inverses of dense complex128 matrices a [..., n, n] (blocked LU, dgami_invert_dense)."""
+    n = a.shape[-1]
+    flat = a.reshape(-1, n, n).contiguous()
+    out = torch.empty_like(flat)
+    info = torch.zeros(flat.shape[0], dtype=torch.int32, device=a.device)
+    _lib.call("dgami_invert_dense", flat.shape[0], n, _p(flat), _p(out), _p(info, torch.int32), _stream())
+    if check_info:
+        _raise_singular(info, "matrix")
+    return out.reshape(a.shape)
+
+
 def full_vertex(chi0_inv, gamma_cmp, u_r_m, v_r_m, vrg, chi_phys, beta, check_info=True):
     """Full vertex F_r [nw][nq][n][n] of one channel (vrg QF, chi_phys QW; both None: first term only)."""
     nw, nb4, nv, nq = chi0_inv.shape
@@ -277,6 +289,18 @@ def pairing_vertex_pp(f_dens, f_magn, nb, niv_pp, want_trip=True):
     gt = torch.empty(shape, dtype=C128, device=f_dens.device) if want_trip else None
     _lib.call("dgami_pairing_vertex_pp", nq, nb, nw, niv, int(niv_pp), _p(f_dens), _p(f_magn), _p(gs), _p(gt),
               _stream())
+    return gs, gt
+
+
+def pairing_vertex_pp_range(f_dens, f_magn, nb, niv_pp, w0, gs, gt=None):
+    """Write into gs (and gt) the pairing-vertex entries whose |w| lies in [w0, w0 + nw) from the full
+    vertices F_dens, F_magn [nw][nq][n][n] of those bosonic frequencies; other entries are untouched."""
+    nw, nq, n, _ = f_dens.shape
+    if f_magn is not None and tuple(f_magn.shape) != tuple(f_dens.shape):
+        raise ValueError("f_dens and f_magn must have the same shape")
+    niv = n // (2 * nb * nb)
+    _lib.call("dgami_pairing_vertex_pp_range", nq, nb, nw, int(w0), niv, int(niv_pp), _p(f_dens), _p(f_magn), _p(gs),
+              _p(gt), _stream())
     return gs, gt
 
 

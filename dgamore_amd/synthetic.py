@@ -193,16 +193,18 @@ def _uncompound(m, nb, nv):
 
 
 def _inverse_products(c0, chi, beta, want_f, device):
-    """beta^2 (chi^-1 - c0^-1) or beta^2 (c0^-1 - c0^-1 chi c0^-1), on a torch device when given
-    (the per-w inverses of large boxes take minutes in NumPy)."""
+    """beta^2 (chi^-1 - c0^-1) or beta^2 (c0^-1 - c0^-1 chi c0^-1); with a device, the per-w inverses
+    run in the library's blocked LU (those of large boxes take minutes in NumPy)."""
     if device is None:
         c0inv = np.linalg.inv(c0)
         return beta**2 * (c0inv - c0inv @ chi @ c0inv) if want_f else beta**2 * (np.linalg.inv(chi) - c0inv)
     import torch
 
+    from dgamore_amd import kernels as K
+
     c0d, chid = torch.as_tensor(c0, device=device), torch.as_tensor(chi, device=device)
-    c0inv = torch.linalg.inv(c0d)
-    out = c0inv - c0inv @ chid @ c0inv if want_f else torch.linalg.inv(chid) - c0inv
+    c0inv = K.invert_dense(c0d)  # the library's blocked LU
+    out = c0inv - c0inv @ chid @ c0inv if want_f else K.invert_dense(chid) - c0inv
     return beta**2 * out.cpu().numpy()
 
 

@@ -213,6 +213,24 @@ int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, const dgami_
 int dgami_pairing_vertex_pp(int nq, int nb, int nw, int niv, int np, const dgami_c128* f_dens,
                             const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip, void* stream);
 
+/* The same transform for one range of bosonic frequencies: f_dens / f_magn hold w in [w0, w0 + nw)
+ * ([nw][nq][n][n]); the entries of gamma_sing / gamma_trip whose |w| = |i - j| lies in that range
+ * are written, all others are left untouched. Called over consecutive ranges on zeroed outputs it
+ * gives dgami_pairing_vertex_pp of all w, while the full vertex of only one range is held at a time
+ * (at 48x48, 2 orbitals and 2np = 128 the full vertex of all w would take 176 GB per channel).
+ * Replaces the same reference code as dgami_pairing_vertex_pp, which transforms each rank's
+ * complete F (eliashberg_solver.py:44-58, 69-75, 620-629). */
+int dgami_pairing_vertex_pp_range(int nq, int nb, int nw, int w0, int niv, int np, const dgami_c128* f_dens,
+                                  const dgami_c128* f_magn, dgami_c128* gamma_sing, dgami_c128* gamma_trip,
+                                  void* stream);
+
+/* Batched inverse of nmat dense complex128 matrices a [nmat][n][n] (row major) into a_inv (same
+ * layout; may not alias a), n <= 1536: the blocked LU with partial pivoting and n right-hand
+ * sides of dgami_ladder_invert on given matrices. info: [nmat] (k + 1 for an exactly singular
+ * pivot at step k) or NULL. Replaces the dense FourPoint/LocalFourPoint.invert of the local
+ * vertices (local_sde.py:32-51, 62-66; four_point.py:590-600), which calls np.linalg.inv. */
+int dgami_invert_dense(int nmat, int n, const dgami_c128* a, dgami_c128* a_inv, int32_t* info, void* stream);
+
 /* Batched inverse of nmat nb^2 x nb^2 blocks stored element-major: in/out [nb^4 (M order)][nmat].
  * info: [nmat] (1 where a pivot was exactly zero) or NULL.
  * Replaces FourPoint.invert for objects without fermionic axes (four_point.py:207-209, 615-618)

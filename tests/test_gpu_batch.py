@@ -135,3 +135,41 @@ def test_model_batch_graph_chain_matches_eager_iterations(K, A):
             assert rel(g, e.cpu().numpy()) < 1e-12, (it, m.name)
     # the iteration moved Sigma
     assert max(float((g - s).abs().max()) for g, s in zip(mb.graph_sigmas, start)) > 1e-8
+
+
+def test_config3_batch_at_full_size(K):
+    """The 16-model batch of the benchmark (seed 180: 1-3 orbitals, beta 5-40, niv = niw =
+    round(1.5 beta), k-grids 16^2-32^2, n = 16..1080), orbital-symmetric inputs as benchmarked:
+    10 chained replays of the captured step, each equal to the eager step from the same Sigma,
+    and every model's ladder sums on one irreducible q-point (both channels, every w) equal to
+    the oracle's LU solves, the n = 1080 model included."""
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import ModelBatch, u_as_channel
+
+    cfgs = S.batch_configs(seed=180)
+    sizes = [c["nb"] ** 2 * 2 * c["niv"] for c in cfgs]
+    say(f"config-3 batch seed 180: 16 models, n = {sorted(sizes)}")
+    models = [S.make_model(c, device=torch.device("cuda"), symmetric=True) for c in cfgs]
+    mb = ModelBatch(models, mixing=0.3)
+    mb.capture(chain=True)
+    sig = [s.clone() for s in mb.graph_sigmas]
+    for it in range(10):
+        mb.replay()
+        torch.cuda.synchronize()
+        sig = mb.step(sig, keep=(it == 0))
+        torch.cuda.synchronize()
+        for g, e, m in zip(mb.graph_sigmas, sig, models):
+            err = float((g - e).abs().max() / e.abs().max())
+            assert err < 1e-12, (it, m.name, err)
+        if it == 0:
+            for p in mb.pipes:
+                m, last = p.m, p.last
+                q_sel = [p.nq - 1]
+                inv = K.qf_to_ref4(last["chi0_inv"], m.nb).cpu().numpy()[q_sel]
+                zero_v = np.zeros((1,) + (m.nb,) * 4)
+                for c, (ch, _) in enumerate(p.CHANNELS):
+                    ref = O.ladder_sum_solve(inv, m.gamma[ch], u_as_channel(m.u, ch), zero_v, m.beta)
+                    got = K.qf_to_ref4(last["aux"][c], m.nb).cpu().numpy()[q_sel]
+                    assert rel(got, ref) < RTOL, (m.name, ch)
+            say("ladder sums of every model match the oracle on one q-point; replays continue")
+    assert all(bool(torch.isfinite(s).all()) for s in sig)

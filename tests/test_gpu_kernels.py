@@ -461,7 +461,8 @@ def test_solve_sum_dense(K, nb, niv):
 
 
 # ----------------------------------------------------------------------------- whole SDE iteration
-@pytest.mark.parametrize("cfg,nk,niw,niv", [(0, (8, 8, 1), 4, 6), (2, (6, 6, 1), 2, 4)])
+@pytest.mark.parametrize("cfg,nk,niw,niv", [(0, (8, 8, 1), 4, 6), (2, (6, 6, 1), 2, 4),
+                                             (0, None, None, None)])  # the last: config 0 at its full size
 def test_pipeline_step_matches_oracle(K, cfg, nk, niw, niv):
     from dgamore_amd import synthetic as S
     from dgamore_amd.pipeline import SDEPipeline, u_as_channel

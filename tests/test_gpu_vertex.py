@@ -170,3 +170,20 @@ def test_pairing_vertex_fixture_combination(K):
     torch.cuda.synchronize()
     assert rel(gs, golden("e2e_gamma_irrq_sing_pp.npy")) < 1e-6
     assert rel(gt, golden("e2e_gamma_irrq_trip_pp.npy")) < 1e-6
+
+
+@pytest.mark.parametrize("nmat,n", [(3, 40), (2, 150), (2, 513)])
+def test_invert_dense(nmat, n):
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    from dgamore_amd import kernels as K
+
+    print(f"[gpu-case] invert_dense {nmat} matrices n={n}", flush=True)
+    rng = np.random.default_rng(n)
+    a = rng.standard_normal((nmat, n, n)) + 1j * rng.standard_normal((nmat, n, n)) + 3 * np.eye(n)
+    got = K.invert_dense(K.to_dev(a)).cpu().numpy()
+    ref = np.linalg.inv(a)
+    assert np.abs(got - ref).max() / np.abs(ref).max() < 1e-10
+    a[1, :, 7] = 0.0  # a singular matrix raises, as np.linalg.inv does
+    with pytest.raises(np.linalg.LinAlgError):
+        K.invert_dense(K.to_dev(a))
