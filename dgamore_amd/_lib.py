@@ -69,6 +69,7 @@ SIGNATURES["dgami_ladder_tune_candidates"] = ([_i, _i, _i, _c, _i], _i)
 SIGNATURES["dgami_ladder_tune_set"] = ([_i, _i, _i, _i], _i)
 SIGNATURES["dgami_ladder_tune_get"] = ([_i, _i, _i], _i)
 SIGNATURES["dgami_ladder_tune_clear"] = ([], _i)
+SIGNATURES["dgami_bubble_irrq_noalloc"] = SIGNATURES["dgami_bubble_irrq"]
 SIGNATURES["dgami_invert_dense"] = ([_i, _i, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_pairing_vertex_pp_range"] = ([_i, _i, _i, _i, _i, _i, _c, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_workspace_hold"] = ([], _i)

@@ -5,6 +5,9 @@
 // backward 3D FFT there and writes only the irreducible q-points, scaled by
 // -beta/N^2 (ifft 1/N times -beta/N) and filtered at 1e-12. Output is q
 // fastest (QF layout), so every workgroup writes one contiguous run.
+#include <mutex>
+#include <string>
+
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
@@ -54,24 +57,34 @@ __global__ void __launch_bounds__(256) bubble_kernel(const double2* __restrict__
 
 }  // namespace dgami
 
-extern "C" int dgami_bubble_irrq(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
-                                 const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0,
-                                 void* stream_) {
-  using namespace dgami;
-  hipStream_t stream = static_cast<hipStream_t>(stream_);
+namespace dgami {
+namespace {
+
+// Both entry points: validate, then G(k) -> [nb][nb][nvg][nk] (gk), its momentum FFT (gr) and the
+// bubble kernel. gk and gr hold nb^2 * 2 (niv + niw) * nk complex128 values each.
+struct BubbleShape {
+  FftPlan3D plan;
+  size_t gbytes;
+};
+
+int bubble_check(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, const dgami_c128* g_k,
+                 const int32_t* irrk_ind, int nq_irr, const dgami_c128* chi0, BubbleShape* sh) {
   DGAMI_ARG_CHECK(g_k != nullptr && irrk_ind != nullptr && chi0 != nullptr, "bubble: null pointer");
   DGAMI_ARG_CHECK(nb >= 1 && niw >= 0 && niv >= 1 && nq_irr >= 1, "bubble: invalid sizes");
   DGAMI_ARG_CHECK(niv_g >= niv + niw, "bubble: G must hold at least niv + niw fermionic frequencies");
-  FftPlan3D plan;
-  DGAMI_ARG_CHECK(nx > 0 && ny > 0 && nz > 0 && fft_plan_3d(nx, ny, nz, &plan),
+  DGAMI_ARG_CHECK(nx > 0 && ny > 0 && nz > 0 && fft_plan_3d(nx, ny, nz, &sh->plan),
                   "bubble: grid lengths must factor into 2, 3 and 5");
-  DGAMI_ARG_CHECK(nq_irr <= plan.ntot, "bubble: more irreducible points than grid points");
+  DGAMI_ARG_CHECK(nq_irr <= sh->plan.ntot, "bubble: more irreducible points than grid points");
+  sh->gbytes = static_cast<size_t>(nb) * nb * 2 * (niv + niw) * sh->plan.ntot * sizeof(double2);
+  return 0;
+}
+
+int bubble_run(const BubbleShape& sh, int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
+               const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0, double2* gk, double2* gr,
+               hipStream_t stream) {
+  const FftPlan3D& plan = sh.plan;
   const int nk = plan.ntot;
   const int nvg = 2 * (niv + niw);
-  const size_t gbytes = static_cast<size_t>(nb) * nb * nvg * nk * sizeof(double2);
-  double2* gk = static_cast<double2*>(workspace(WS_GK, gbytes));
-  double2* gr = static_cast<double2*>(workspace(WS_GR, gbytes));
-  if (gk == nullptr || gr == nullptr) return -2;
   int rc = launch_ref2_to_gk(reinterpret_cast<const double2*>(g_k), gk, nk, nb, 2 * niv_g, niv_g - niv - niw, nvg,
                              stream);
   if (rc) return rc;
@@ -87,4 +100,66 @@ extern "C" int dgami_bubble_irrq(int nx, int ny, int nz, int nb, int niv_g, int 
                      reinterpret_cast<double2*>(chi0), plan, nb, niw, niv, nvg, nq_irr, scale);
   DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+// Scratch of dgami_bubble_irrq_noalloc: a file-scope device array, one copy per GPU, that holds
+// gk and gr. Host code never uses the array's name as an address: the address on each device is
+// obtained once with hipGetSymbolAddress (with that device current) and kept in scratch_ptr.
+constexpr size_t BUBBLE_SCRATCH_BYTES = DGAMI_BUBBLE_NOALLOC_MAX_BYTES;
+__device__ __align__(256) double2 scratch[BUBBLE_SCRATCH_BYTES / sizeof(double2)];
+
+constexpr int MAX_DEVICES = 64;
+std::mutex g_scratch_mutex;
+void* g_scratch_ptr[MAX_DEVICES] = {};
+
+// Address of this device's scratch (looked up on the first call on the device).
+int scratch_address(void** out) {
+  int dev = 0;
+  DGAMI_HIP_CHECK(hipGetDevice(&dev));
+  DGAMI_ARG_CHECK(dev >= 0 && dev < MAX_DEVICES, "bubble_noalloc: device index out of range");
+  std::lock_guard<std::mutex> lock(g_scratch_mutex);
+  if (g_scratch_ptr[dev] == nullptr) {
+    void* p = nullptr;
+    DGAMI_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
+}  // namespace
+}  // namespace dgami
+
+extern "C" int dgami_bubble_irrq(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
+                                 const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0,
+                                 void* stream_) {
+  using namespace dgami;
+  BubbleShape sh;
+  int rc = bubble_check(nx, ny, nz, nb, niv_g, niw, niv, g_k, irrk_ind, nq_irr, chi0, &sh);
+  if (rc) return rc;
+  double2* gk = static_cast<double2*>(workspace(WS_GK, sh.gbytes));
+  double2* gr = static_cast<double2*>(workspace(WS_GR, sh.gbytes));
+  if (gk == nullptr || gr == nullptr) return -2;
+  return bubble_run(sh, nx, ny, nz, nb, niv_g, niw, niv, beta, g_k, irrk_ind, nq_irr, chi0, gk, gr,
+                    static_cast<hipStream_t>(stream_));
+}
+
+extern "C" int dgami_bubble_irrq_noalloc(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
+                                         const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0,
+                                         void* stream_) {
+  using namespace dgami;
+  BubbleShape sh;
+  int rc = bubble_check(nx, ny, nz, nb, niv_g, niw, niv, g_k, irrk_ind, nq_irr, chi0, &sh);
+  if (rc) return rc;
+  const size_t half = (sh.gbytes + 255) & ~size_t(255);
+  DGAMI_ARG_CHECK(2 * half <= BUBBLE_SCRATCH_BYTES,
+                  "bubble_noalloc: G needs " + std::to_string(2 * half) + " bytes of scratch, more than the " +
+                      std::to_string(BUBBLE_SCRATCH_BYTES) + " bytes the no-allocation entry point holds");
+  void* base = nullptr;
+  rc = scratch_address(&base);
+  if (rc) return rc;
+  double2* gk = static_cast<double2*>(base);
+  double2* gr = reinterpret_cast<double2*>(static_cast<char*>(base) + half);
+  return bubble_run(sh, nx, ny, nz, nb, niv_g, niw, niv, beta, g_k, irrk_ind, nq_irr, chi0, gk, gr,
+                    static_cast<hipStream_t>(stream_));
 }

@@ -110,6 +110,19 @@ int dgami_fft_grid(const dgami_c128* in, dgami_c128* out, int64_t batch, int nx,
 int dgami_bubble_irrq(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
                       const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0, void* stream);
 
+/* dgami_bubble_irrq without any device allocation. No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc). The G(k) re-layout and its momentum FFT live in a file-scope
+ * device array of DGAMI_BUBBLE_NOALLOC_MAX_BYTES per GPU (held until the process ends; its
+ * address is looked up once per device on the first call there). Same arguments and results as
+ * dgami_bubble_irrq; sizes needing more than the maximum (2 * nb^2 * 2(niv + niw) * nx*ny*nz * 16
+ * bytes) are refused with -1. Every call on a GPU shares that GPU's array, so calls that overlap
+ * on one GPU (on different streams) race; nothing in it needs to start at zero. */
+#define DGAMI_BUBBLE_NOALLOC_MAX_BYTES ((size_t)64 << 20)
+int dgami_bubble_irrq_noalloc(int nx, int ny, int nz, int nb, int niv_g, int niw, int niv, double beta,
+                              const dgami_c128* g_k, const int32_t* irrk_ind, int nq_irr, dgami_c128* chi0,
+                              void* stream);
+
 /* chi0 post-processing for one SDE iteration:
  *   chi0_full_sum = (1/beta^2) sum_v chi0           (all 2*niv_full frequencies)
  *   chi0_core_sum = (1/beta^2) sum_{|v|<niv_core} chi0
