@@ -34,10 +34,15 @@ namespace dgami {
 constexpr int MAP_STRIDE = BLK_MAP_STRIDE;  // interchange map per problem: count, dst[NB], src[NB], dest[NB]
 
 // ---------------------------------------------------------------------------
-// grid (ceil(n / ASM_ROWS), batch), 256 threads: ASM_ROWS rows of one problem per workgroup.
-// Each thread owns column pairs (j, j + 1); their orbital/frequency split is computed once and
-// reused for every row, and each pair is written as one 16- or 32-byte store.
+// grid (ceil(n / ASM_ROWS), ceil(batch / ASM_QG)), 256 threads: ASM_ROWS rows of ASM_QG
+// consecutive problems per workgroup. Consecutive problems are consecutive q-points of one
+// (channel, w) and share Gamma(channel, w): each Gamma element is loaded once for the group and
+// reused for every problem that shares it (the load is repeated only where the group crosses
+// to another (channel, w), and for every problem of the dense form). Each thread owns column
+// pairs (j, j + 1); their orbital/frequency split is computed once and reused for every row, and
+// each pair is written as one 16- or 32-byte store.
 constexpr int ASM_ROWS = 8;
+constexpr int ASM_QG = 8;
 
 template <typename S>
 __device__ __forceinline__ void store_pair(S* dst, double2 x, double2 y) {
@@ -51,17 +56,29 @@ __device__ __forceinline__ void store_pair(S* dst, double2 x, double2 y) {
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0) {
-  const int b = blockIdx.y;
-  const int item = item0 + b;
-  const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
-  const double2* G = ladder_matrix_ptr(a, chan, w, q);
-  const double2* U = a.dense ? nullptr : a.u_r + chan * a.T * a.T;
-  const double2* V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
-  S* Wb = st.W + static_cast<size_t>(b) * st.n * st.ld;
+__global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0, int nbatch) {
+  const int b0 = blockIdx.y * ASM_QG;
+  const int ng = min(ASM_QG, nbatch - b0);
   const int n = st.n, ld = st.ld, ncol = st.ncol, nv = a.nv, T = a.T;
   const int i0 = blockIdx.x * ASM_ROWS, r1 = min(n, i0 + ASM_ROWS);
   const double2 zero = make_double2(0.0, 0.0);
+  // the group's problems: matrix, U_r + V_r(q) and chi0^-1 bases, output matrix
+  const double2* Gk[ASM_QG];
+  const double2* Uk[ASM_QG];
+  const double2* Vk[ASM_QG];
+  const double2* Ck[ASM_QG];
+  int wk[ASM_QG], qk[ASM_QG];
+#pragma unroll
+  for (int k = 0; k < ASM_QG; ++k) {
+    const int item = item0 + b0 + min(k, ng - 1);
+    const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
+    Gk[k] = ladder_matrix_ptr(a, chan, w, q);
+    Uk[k] = a.dense ? nullptr : a.u_r + chan * T * T;
+    Vk[k] = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * T * T : nullptr;
+    Ck[k] = a.dense ? nullptr : a.chi0_inv + static_cast<size_t>(w) * T * T * nv * a.nq + q;
+    wk[k] = w;
+    qk[k] = q;
+  }
   for (int jb = 2 * threadIdx.x; jb < ld; jb += 512) {  // ld is even
     int cj[2], vj[2];
 #pragma unroll
@@ -72,29 +89,37 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
     }
     for (int i = i0; i < r1; ++i) {
       const int ri = i / nv, vi = i - ri * nv;
-      double2 val[2];
+      double2 g[2] = {zero, zero};
 #pragma unroll
-      for (int h = 0; h < 2; ++h) {
-        const int j = jb + h;
-        val[h] = zero;
-        if (j < n) {
-          if (a.dense) {
-            val[h] = G[static_cast<size_t>(i) * n + j];
-          } else {
-            double2 uv = U[ri * T + cj[h]];
-            if (V) uv = cadd(uv, V[ri * T + cj[h]]);
-            val[h] = cscale(csub(G[static_cast<size_t>(i) * n + j], uv), a.inv_beta2);
-            if (vi == vj[h])
-              val[h] = cadd(val[h], a.chi0_inv[((static_cast<size_t>(w) * T * T + ri * T + cj[h]) * nv + vi) * a.nq + q]);
-          }
-        } else if (j < ncol) {
-          val[h] = ladder_rhs_at(a, w, q, i, j - n);
+      for (int k = 0; k < ASM_QG; ++k) {
+        if (k >= ng) break;
+        if (k == 0 || Gk[k] != Gk[k - 1]) {
+#pragma unroll
+          for (int h = 0; h < 2; ++h) g[h] = jb + h < n ? Gk[k][static_cast<size_t>(i) * n + jb + h] : zero;
         }
+        double2 val[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const int j = jb + h;
+          val[h] = zero;
+          if (j < n) {
+            if (a.dense) {
+              val[h] = g[h];
+            } else {
+              double2 uv = Uk[k][ri * T + cj[h]];
+              if (Vk[k]) uv = cadd(uv, Vk[k][ri * T + cj[h]]);
+              val[h] = cscale(csub(g[h], uv), a.inv_beta2);
+              if (vi == vj[h]) val[h] = cadd(val[h], Ck[k][(static_cast<size_t>(ri * T + cj[h]) * nv + vi) * a.nq]);
+            }
+          } else if (j < ncol) {
+            val[h] = ladder_rhs_at(a, wk[k], qk[k], i, j - n);
+          }
+        }
+        store_pair(st.W + static_cast<size_t>(b0 + k) * n * ld + static_cast<size_t>(i) * ld + jb, val[0], val[1]);
       }
-      store_pair(Wb + static_cast<size_t>(i) * ld + jb, val[0], val[1]);
     }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) st.info[b] = 0;
+  if (blockIdx.x == 0 && threadIdx.x < ng) st.info[b0 + threadIdx.x] = 0;
 }
 
 // ---------------------------------------------------------------------------
@@ -832,7 +857,8 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
   static_assert(NPAN >= 1 && NPAN <= BLK_PANELS, "panels per outer block");
   const int n = st.n, ncol = st.ncol;
   DGAMI_ARG_CHECK(nb <= 65535, "ladder: batch too large");
-  hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(ceil_div(n, ASM_ROWS), nb), dim3(256), 0, stream, a, st, item0);
+  hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(ceil_div(n, ASM_ROWS), ceil_div(nb, ASM_QG)), dim3(256), 0, stream, a,
+                     st, item0, nb);
   DGAMI_HIP_CHECK(hipGetLastError());
   BlockedSteps<S, NB, RPT, MAXTH> k{a, st, nb, stream};
   const int BOTH = ROWS_SWAP | ROWS_TRSM;
