@@ -52,7 +52,7 @@ def parse():
                          "cores + FP64 refinement; bf16x3: the same with split-BF16 MFMA trailing updates)")
     ap.add_argument("--no-compare", action="store_true", help="skip the plain FP64 comparison run")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="target CPU-baseline sample time")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="target CPU-baseline sample time")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a 1 GiB device-to-device copy (3 times), the reference kernel for scaling "
                          "rocprofv3 FETCH_SIZE / WRITE_SIZE counts to bytes")
@@ -223,9 +223,17 @@ def copy_calibration():
 
 
 def cpu_baseline(model, target_s):
-    """Time the reference's per-q solve path (NumPy/LAPACK, complex128) on a bounded sample,
-    with all host threads and with one thread (the reference runs one MPI rank per core with
-    OMP_NUM_THREADS=1)."""
+    """Time the reference's per-q path restated in NumPy/LAPACK (the oracle) on bounded samples of
+    the workload, on this host's cores:
+    * ladder solves (numpy.linalg.solve with the nb^2 indicator right-hand sides, the reference's
+      v2 path) in complex128 with BLAS threads = the cores given to this process, in complex128 with
+      one thread (the reference runs one MPI rank per core with OMP_NUM_THREADS=1, README.md:125,
+      so its ideal q-parallel rate is the one-thread rate times the cores), and in complex64 (the
+      precision the reference stores) with all threads;
+    * the whole per-q chain of one SDE iteration for one irreducible q-point (direct bubble,
+      double-counting kernel, both channels' ladders, three-leg vertex, susceptibility, kernel)
+      and the Sigma sum over the full zone, extrapolated to the step (nq times the per-q chain
+      plus one Sigma sum)."""
     import numpy as np
     from threadpoolctl import threadpool_limits
 
@@ -235,33 +243,62 @@ def cpu_baseline(model, target_s):
     nb, niv, niw, beta = model.nb, model.niv, model.niw, model.beta
     g = model.giwk()
     grid = KGrid(model.nk)
-    # bubble of one irreducible q-point (direct k-sum form), then its ladder solves, repeated
     q_list = grid.get_irrq_list()[1:2]
     chi0 = O.bubble_direct(g, niw, niv, q_list, beta)
     inv = O.block_invert_vn1(chi0)
     zero_v = np.zeros((1,) + (nb,) * 4)
 
-    def timed(seconds):
+    rhs = O.ladder_rhs(nb, 2 * niv)
+
+    def timed(seconds, dtype):
+        """(q, w, channel) solves of one q-point: the ladder matrices of both channels assembled and
+        solved with the indicator right-hand sides in `dtype`, repeated for `seconds`."""
         solves = 0
+        b = np.broadcast_to(rhs.astype(dtype), (niw + 1,) + rhs.shape)
         t0 = time.perf_counter()
         while True:
             for ch in ("dens", "magn"):
-                ur = O.u_as_channel(model.u, ch)
-                O.ladder_sum_solve(inv, model.gamma[ch], ur, zero_v, beta)
+                a = O.ladder_matrix(inv[0], model.gamma[ch], O.u_as_channel(model.u, ch), zero_v[0], beta)
+                np.linalg.solve(a.astype(dtype), b)
                 solves += niw + 1
             el = time.perf_counter() - t0
             if el > seconds:
                 return solves, el
 
     threads = len(os.sched_getaffinity(0))
-    solves, el = timed(target_s)
+    with threadpool_limits(limits=threads):
+        solves, el = timed(target_s, np.complex128)
+        solves32, el32 = timed(target_s / 3, np.complex64)
     with threadpool_limits(limits=1):
-        solves1, el1 = timed(target_s / 3)
+        solves1, el1 = timed(target_s / 3, np.complex128)
+    # one q-point's chain of the SDE iteration and the Sigma sum (all threads)
+    with threadpool_limits(limits=threads):
+        t0 = time.perf_counter()
+        c0 = O.bubble_direct(g, niw, niv, q_list, beta)
+        iv = O.block_invert_vn1(c0)
+        kern = -O.dc_kernel(model.f_magn, c0, model.u, beta, niv)
+        for ch, wgt in (("dens", 1.0), ("magn", 3.0)):
+            ur = O.u_as_channel(model.u, ch)
+            aux = O.ladder_sum_solve(iv, model.gamma[ch], ur, zero_v, beta)
+            vrg = O.vrg(iv, aux, beta)
+            chi = O.chi_phys_shell(O.sum_over_vn(aux, beta), c0.sum(-1) / beta**2, c0.sum(-1) / beta**2, ur, zero_v)
+            kern = kern + wgt * O.kernel_r(vrg, chi, ur, zero_v, ch)
+        t_q = time.perf_counter() - t0
+        kern_all = np.zeros((grid.nk_irr,) + kern.shape[1:], dtype=complex)
+        t0 = time.perf_counter()
+        O.sigma_fft(kern_all, g, grid.irrk_inv, beta, niv, niw)
+        t_sigma = time.perf_counter() - t0
+    n = nb * nb * 2 * niv
     return {"value": solves / el, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"{solves} (q,w,channel) ladder solves of n={nb * nb * 2 * niv} via numpy.linalg.solve "
-                      f"(complex128, the reference's solve-based path) in {el:.1f} s with BLAS threads <= {threads}; "
-                      f"one thread: {solves1} solves in {el1:.1f} s",
-            "value_1_thread": solves1 / el1}
+            "sample": f"{solves} (q,w,channel) ladder solves of n={n} via numpy.linalg.solve (complex128, the "
+                      f"reference's solve-based path) in {el:.1f} s with BLAS threads = {threads}; one thread: "
+                      f"{solves1} solves in {el1:.1f} s; complex64: {solves32} solves in {el32:.1f} s; the per-q chain "
+                      f"of one SDE iteration (1 of {grid.nk_irr} irreducible q) in {t_q:.1f} s and the Sigma sum in "
+                      f"{t_sigma:.1f} s",
+            "value_1_thread": solves1 / el1, "value_1_thread_x_cores": solves1 / el1 * threads,
+            "value_c64": solves32 / el32,
+            "sde_step_s_extrapolated": t_q * grid.nk_irr + t_sigma,
+            "per_q_chain_s": t_q, "sigma_sum_s": t_sigma}
 
 
 def time_steps(pipe, steps, warmup, stream):
@@ -317,7 +354,11 @@ def dominant_kernel(pipe, stream, ms_ladder, precision_name):
 
     Compound sizes above 128 run the blocked LU, whose trailing-matrix update dominates: the
     library brackets each of its launches with events (dgami_profile_*) and returns the
-    algorithmic bytes (C read + written, A and B panels read once) and flops (8 m n k).
+    algorithmic flops (8 m n k per problem) and the bytes of the C tile read and written plus the
+    A and B panels read once. The ladder is matrix-core work (8/3 n^3 + 8 nb^2 n^2 flops per
+    solve against kilobytes of operands), so the bound is the matrix-core peak of the precision
+    (split-BF16: the BF16 peak / 3 products). `traffic` is the HBM bytes per launch measured with
+    rocprofv3 FETCH_SIZE / WRITE_SIZE (profiles/trailing_update_traffic.txt).
     Smaller sizes run one register-resident kernel per call, timed as the whole ladder call.
     """
     import torch
@@ -343,24 +384,29 @@ def dominant_kernel(pipe, stream, ms_ladder, precision_name):
     avg_ms = prof["ms"] / prof["launches"]
     tf = prof["flops"] / (prof["ms"] * 1e-3) / 1e12
     gbs = prof["bytes"] / (prof["ms"] * 1e-3) / 1e9
-    # the bound is the resource whose ideal time for the algorithmic work is longer
-    mfma_bound = prof["flops"] / (peak_tf * 1e12) >= prof["bytes"] / (PEAK_HBM_GBS * 1e9)
     traffic = None
     if os.path.exists(TRAFFIC_FILE):
         for line in open(TRAFFIC_FILE):
             f = line.split("#", 1)[0].split()
             if len(f) == 3 and f[0] == precision_name and f[1] == m.name:
                 traffic = float(f[2]) * 1e9
-    out = {"kernel": "trailing-matrix update of the blocked LU (blk_gemm_*)", "launches": prof["launches"],
-           "avg_launch_ms": avg_ms, "traffic": traffic,
-           "algorithmic_bytes_per_launch": prof["bytes"] / prof["launches"],
-           "algorithmic_flops_per_launch": prof["flops"] / prof["launches"],
-           "achieved_TFLOPs": tf, "achieved_GBps": gbs}
-    if mfma_bound:
-        out.update(bound="mfma", achieved=tf, peak=peak_tf, unit="TFLOP/s", frac=tf / peak_tf)
-    else:
-        out.update(bound="hbm", achieved=gbs, peak=PEAK_HBM_GBS, unit="GB/s", frac=gbs / PEAK_HBM_GBS)
-    return out
+    return {"kernel": "trailing-matrix update of the blocked LU (blk_gemm_*)", "launches": prof["launches"],
+            "avg_launch_ms": avg_ms, "bound": "mfma", "achieved": tf, "peak": peak_tf, "unit": "TFLOP/s",
+            "frac": tf / peak_tf, "traffic": traffic,
+            "algorithmic_flops_per_launch": prof["flops"] / prof["launches"],
+            "c_tile_and_panel_bytes_per_launch": prof["bytes"] / prof["launches"],
+            "hbm_GBps_of_measured_traffic": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9,
+            "hbm_frac_of_measured_traffic": None if traffic is None else traffic / (avg_ms * 1e-3) / 1e9 / PEAK_HBM_GBS}
+
+
+def ladder_algorithmic_bytes(pipe):
+    """HBM bytes one ladder call must move at least (complex128): Gamma_r(w) of both channels read,
+    chi0^-1 read, chi*_sum of both channels written."""
+    m = pipe.m
+    n = m.nb**2 * 2 * m.niv
+    nw = m.niw + 1
+    nch = len(pipe.CHANNELS)
+    return 16.0 * (nch * nw * n * n + nw * m.nb**4 * 2 * m.niv * pipe.nq * (1 + nch))
 
 
 def rel_err(a, b):
@@ -468,6 +514,10 @@ def main():
         "ladder_solves_per_s": solves / (ms_ladder * 1e-3),
         "ladder_roofline": {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                             "frac": achieved / peak, "flops_per_solve": pipe.ladder_flops_per_solve()},
+        "ladder_traffic": None if roof.get("traffic") is None else {
+            "trailing_update_GB_per_call": roof["traffic"] * roof["launches"] / 1e9,
+            "algorithmic_GB_per_call": ladder_algorithmic_bytes(pipe) / 1e9,
+            "ratio": roof["traffic"] * roof["launches"] / ladder_algorithmic_bytes(pipe)},
         "singular_solves": info_bad,
         "compare_fp64": compare,
         "roofline": roof,
