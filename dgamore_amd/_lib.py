@@ -72,6 +72,9 @@ SIGNATURES["dgami_ladder_tune_clear"] = ([], _i)
 SIGNATURES["dgami_bubble_irrq_noalloc"] = SIGNATURES["dgami_bubble_irrq"]
 SIGNATURES["dgami_invert_dense"] = ([_i, _i, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_pairing_vertex_pp_range"] = ([_i, _i, _i, _i, _i, _i, _c, _c, _c, _c, _c], _i)
+SIGNATURES["dgami_generalized_chi"] = ([_i, _i, _i, _i, _d, _i, _c, _c, _c, _c], _i)
+SIGNATURES["dgami_local_gamma"] = ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c], _i)
+SIGNATURES["dgami_local_sigma"] = ([_i, _i, _i, _i, _d] + [_c] * 11, _i)
 SIGNATURES["dgami_workspace_hold"] = ([], _i)
 SIGNATURES["dgami_workspace_release"] = ([], _i)
 

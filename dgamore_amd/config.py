@@ -8,7 +8,7 @@ subset those functions read, with the same attribute names.
 
 from types import SimpleNamespace
 
-sys = SimpleNamespace(beta=1.0, mu=0.0, n=0.0, n_bands=1, occ=None, occ_k=None)
+sys = SimpleNamespace(beta=1.0, mu=0.0, n=0.0, n_bands=1, occ=None, occ_k=None, occ_dmft=None)
 box = SimpleNamespace(niw_core=0, niv_core=0, niv_shell=0, niv_full=0, niv_dmft=0)
 lattice = SimpleNamespace(nk=(1, 1, 1), nq=(1, 1, 1), k_grid=None, q_grid=None)
 self_consistency = SimpleNamespace(mixing=1.0, max_iter=1, epsilon=1e-4, mixing_strategy="linear",

@@ -304,6 +304,50 @@ def pairing_vertex_pp_range(f_dens, f_magn, nb, niv_pp, w0, gs, gt=None):
     return gs, gt
 
 
+def generalized_chi(g2_cmp, g, nb, beta, dens):
+    """Local chi_r CMP [nw][n][n] from g2 CMP (w >= 0); dens subtracts the disconnected part with
+    G [nb][nb][2 niv_g]."""
+    nw, n, _ = g2_cmp.shape
+    niv = n // (2 * nb * nb)
+    out = torch.empty_like(g2_cmp)
+    _lib.call("dgami_generalized_chi", nb, nw, niv, 0 if g is None else g.shape[-1] // 2, float(beta),
+              1 if dens else 0, _p(g2_cmp), _p(g), _p(out), _stream())
+    return out
+
+
+def local_gamma(chi_inv, chit_inv, u_r_m, nb, beta, niv_full=None):
+    """Gamma = beta^2 (chi_inv - chit_inv) + U_r on the core box (CMP [nw][nc][nc]); with niv_full
+    also the vertex padded with U_r to the full box. chit_inv None: chi_inv is Gamma itself and
+    only the padded vertex is formed. Returns (gamma or None, gamma_pad or None)."""
+    nw, nc, _ = chi_inv.shape
+    if chit_inv is not None and tuple(chit_inv.shape) != tuple(chi_inv.shape):
+        raise ValueError("chi_inv and chit_inv must have the same shape")
+    if chit_inv is None and niv_full is None:
+        raise ValueError("padding needs niv_full")
+    niv_core = nc // (2 * nb * nb)
+    gamma = torch.empty_like(chi_inv) if chit_inv is not None else None
+    pad = None
+    if niv_full is not None:
+        nf = nb * nb * 2 * niv_full
+        pad = torch.empty((nw, nf, nf), dtype=C128, device=chi_inv.device)
+    _lib.call("dgami_local_gamma", nb, nw, niv_core, int(niv_full or niv_core), float(beta), _p(chi_inv),
+              _p(chit_inv), _p(u_r_m), _p(gamma), _p(pad), _stream())
+    return gamma, pad
+
+
+def local_sigma(vrg_d, vrg_m, chi_d, chi_m, ud_m, um_m, u_ref, g, beta, occ=None):
+    """Local Schwinger-Dyson self-energy [nb][nb][2 niv] (vrg QF and chi QW with nq = 1; the
+    Hartree-Fock constant is added when occ [nb][nb] is given)."""
+    nw, nb4, nv, nq = vrg_d.shape
+    nb = int(round(nb4 ** 0.25))
+    if nq != 1 or tuple(chi_d.shape) != (nw, nb4, 1):
+        raise ValueError("local_sigma takes local (nq = 1) vertices")
+    sigma = torch.empty((nb, nb, nv), dtype=C128, device=vrg_d.device)
+    _lib.call("dgami_local_sigma", nb, nw - 1, nv // 2, g.shape[-1] // 2, float(beta), _p(vrg_d), _p(vrg_m),
+              _p(chi_d), _p(chi_m), _p(ud_m), _p(um_m), _p(u_ref), _p(g), _p(occ), _p(sigma), _stream())
+    return sigma
+
+
 def vrg_chi_phys(chi0_inv, aux, full, core, u_r_m, v_r_m, nb, beta, want_vrg=True):
     """-> (vrg QF or None, chi_phys QW)."""
     nw, _, nv, nq = aux.shape

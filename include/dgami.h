@@ -244,6 +244,37 @@ int dgami_pairing_vertex_pp_range(int nq, int nb, int nw, int w0, int niv, int n
  * vertices (local_sde.py:32-51, 62-66; four_point.py:590-600), which calls np.linalg.inv. */
 int dgami_invert_dense(int nmat, int n, const dgami_c128* a, dgami_c128* a_inv, int32_t* info, void* stream);
 
+/* Local generalized susceptibility of one channel from the DMFT two-particle Green's function:
+ *   chi = beta g2,  density (dens != 0): chi(w = 0) -= 2 beta G_ab(v) G_cd(v')
+ * g2, chi: CMP [nw][n][n] (w >= 0, n = nb^2 2 niv); g: [nb][nb][2 niv_g] (centred; NULL when
+ * dens = 0). Replaces create_generalized_chi (local_sde.py:17-29). */
+int dgami_generalized_chi(int nb, int nw, int niv, int niv_g, double beta, int dens, const dgami_c128* g2,
+                          const dgami_c128* g, dgami_c128* chi, void* stream);
+
+/* Local irreducible vertex from the inverted susceptibilities:
+ *   gamma = beta^2 (chi_inv - chit_inv) + u_r                  CMP [nw][nc][nc], nc = nb^2 2 niv_core
+ * chit_inv is the inverse of the shell-corrected chi~ cut to the core (or of chi0 without a shell);
+ * chit_inv NULL: chi_inv holds gamma itself (only the padding is done). u_r: [nb^4] M order.
+ * gamma_pad (or NULL): the same vertex on the full box CMP [nw][nf][nf], nf = nb^2 2 niv_full,
+ * equal to u_r outside the core; gamma may be NULL when gamma_pad is given. Replaces the combination step of
+ * create_gamma_r / create_gamma_r_with_shell_correction (local_sde.py:32-51) and pad_with_u
+ * (local_four_point.py:652-668). */
+int dgami_local_gamma(int nb, int nw, int niv_core, int niv_full, double beta, const dgami_c128* chi_inv,
+                      const dgami_c128* chit_inv, const dgami_c128* u_r, dgami_c128* gamma, dgami_c128* gamma_pad,
+                      void* stream);
+
+/* Local self-energy from the Schwinger-Dyson equation with the local three-leg vertices:
+ *   inner_r = vrg_r - vrg_r . U_r . chi_r,  inner = (inner_dens - inner_magn) / 2 (w < 0 by symmetry)
+ *   Sigma_ij(v) = -(1/beta) sum_{w = -niw..niw} U_kjop inner_ilpo(w, v) G_lk(v - w)
+ *               + sum_cd U_dens,ijcd n_dc                                  (when occ != NULL)
+ * vrg_*: QF with nq = 1 ([niw+1][nb^4][2 niv]); chi_*: QW with nq = 1; u_dens / u_magn: [nb^4] M
+ * order; u: [nb^4] reference order; g: [nb][nb][2 niv_g], niv_g >= niv + niw; occ: [nb][nb] or
+ * NULL; sigma: [nb][nb][2 niv]. nb <= 4. Replaces get_loc_self_energy_vrg (local_sde.py:150-172). */
+int dgami_local_sigma(int nb, int niw, int niv, int niv_g, double beta, const dgami_c128* vrg_dens,
+                      const dgami_c128* vrg_magn, const dgami_c128* chi_dens, const dgami_c128* chi_magn,
+                      const dgami_c128* u_dens, const dgami_c128* u_magn, const dgami_c128* u, const dgami_c128* g,
+                      const dgami_c128* occ, dgami_c128* sigma, void* stream);
+
 /* Batched inverse of nmat nb^2 x nb^2 blocks stored element-major: in/out [nb^4 (M order)][nmat].
  * info: [nmat] (1 where a pivot was exactly zero) or NULL.
  * Replaces FourPoint.invert for objects without fermionic axes (four_point.py:207-209, 615-618)

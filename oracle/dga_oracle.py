@@ -94,6 +94,15 @@ PARITY = {
                     "analytic vertex (test_oracle_vertex.py)",
     "pairing_vertices": "reference-held gamma_irrq_{sing,trip}_pp.npy = combination of f_irrq_{dens,magn}_pp.npy "
                         "(test_oracle_vertex.py)",
+    "local_bubble": "reference-held gchi0_loc.npy from g_dmft.npy (test_oracle_local_sde.py)",
+    "generalized_chi": "closed form beta g2 - 2 beta delta_{w0} G G (dens) on an analytic g2 (test_oracle_local_sde.py)",
+    "gamma_shell": "no shell: beta^2 (chi^-1 - chi0^-1); with a shell: an RPA chi built from a constant U_r on the "
+                   "full box returns Gamma = U_r exactly, which the core-only chi0 fails (test_oracle_local_sde.py)",
+    "full_vertex_from_gamma": "one band, Gamma = U: F = U / (1 + U sum_v chi0(v) / beta^2) in every (v, v'); the "
+                              "BSE form beta^2 (chi0^-1 - chi0^-1 chi chi0^-1) of the same Gamma (test_oracle_local_sde.py)",
+    "local_sigma": "reference-held siw_dga_local.npy from vrg_{dens,magn}_loc.npy, chi_{dens,magn}_loc.npy and "
+                   "g_dmft.npy, up to its Hartree-Fock constant (test_oracle_local_sde.py)",
+    "local_hartree": "closed form sum_cd (2 U_abcd - U_adcb) n_dc; one band: U n (test_oracle_local_sde.py)",
 }
 
 
@@ -757,3 +766,93 @@ def pairing_matvec(g_hat, g_hat_flip, chi0pp, gap, sign, beta):
 def dense_solve_reference(a, b):
     """LAPACK reference solve, used by tests as an independent check."""
     return spla.solve(a, b)
+
+
+# -----------------------------------------------------------------------------
+# Local Schwinger-Dyson equation  (dgamore/local_sde.py:17-217; bubble_gen.py:20-30)
+# Local objects: [o1,o2,o3,o4,w(>=0),v(,v')], compound rows (o1,o2,v), cols (o4,o3,v').
+# -----------------------------------------------------------------------------
+def local_bubble(g_loc, niw, niv, beta, filter_values=True):
+    """chi0_abcd(w, v) = -beta G_ad(v) G_cb(v - w), w in [0, niw], v in [-niv, niv) (bubble_gen.py:20-30,
+    which forms every w and keeps w >= 0 in its objects). g_loc: [nb, nb, 2 niv_g]."""
+    nivg = g_loc.shape[-1] // 2
+    v = np.arange(-niv, niv)
+    w = np.arange(niw + 1)
+    gl = g_loc[:, None, None, :, None, nivg + v]                               # G_ad(v)
+    gr = g_loc.swapaxes(0, 1)[None, :, :, None][..., nivg + v[None, :] - w[:, None]]  # G_cb(v - w)
+    out = -beta * gl * gr
+    return filter_small(out) if filter_values else out
+
+
+def generalized_chi(g2, g_loc, beta, channel, niv_core):
+    """chi_r = beta g2 (w >= 0); density: chi[w = 0] -= 2 beta G_ab(v) G_cd(v') (local_sde.py:17-29)."""
+    chi = beta * np.asarray(g2, dtype=complex).copy()
+    if channel == "dens":
+        nivg = g_loc.shape[-1] // 2
+        gs = g_loc[..., nivg - niv_core:nivg + niv_core]
+        chi[:, :, :, :, 0] -= 2.0 * beta * gs[:, :, None, None, :, None] * gs[None, None, :, :, None, :]
+    return chi
+
+
+def local_uncompound(m, nb):
+    """[w, n, n] -> [o1,o2,o3,o4,w,v,v'] (inverse of compound_local_vertex)."""
+    nw, n = m.shape[0], m.shape[1]
+    nv = n // (nb * nb)
+    return m.reshape(nw, nb, nb, nv, nb, nb, nv).transpose(1, 2, 5, 4, 0, 3, 6)
+
+
+def _u_full(u_r, nw, nv):
+    """A local interaction on every (w, v, v') of a [o1..o4, w, v, v'] object."""
+    return np.broadcast_to(np.asarray(u_r, dtype=complex)[..., None, None, None], u_r.shape + (nw, nv, nv))
+
+
+def gamma_shell(chi_r, chi0_full, u_r, beta, niv_core):
+    """Gamma_r = beta^2 (chi_r^-1 - chi_tilde_core^-1) + U_r with chi_tilde = (chi0^-1 + U_r / beta^2)^-1 on the
+    full box (U_r on every (v, v')), cut to the core (local_sde.py:40-51).
+
+    chi_r: [o1..o4, w, 2nc, 2nc]; chi0_full: [o1..o4, w, 2nf] (diagonal in v); u_r: [nb]*4."""
+    nb, nw, nvf = chi0_full.shape[0], chi0_full.shape[4], chi0_full.shape[5]
+    nf, nc = nvf // 2, niv_core
+    inv0 = block_invert_vn1(chi0_full[None])[0]
+    diag = np.einsum("abcdwv,vp->abcdwvp", inv0, np.eye(nvf))
+    chi_t = np.linalg.inv(compound_local_vertex(diag + _u_full(u_r, nw, nvf) / beta**2))
+    core = local_uncompound(chi_t, nb)[..., nf - nc:nf + nc, nf - nc:nf + nc]
+    chi_t_core_inv = np.linalg.inv(compound_local_vertex(core))
+    g = beta**2 * (np.linalg.inv(compound_local_vertex(chi_r)) - chi_t_core_inv)
+    return local_uncompound(g, nb) + _u_full(u_r, nw, 2 * nc)
+
+
+def full_vertex_from_gamma(gamma, chi0_full, u_r, beta):
+    """F_r = Gamma_u [1 + chi0 Gamma_u / beta^2]^-1 on the full box, Gamma_u = Gamma in the core and U_r outside
+    (local_sde.py:69-76 with local_four_point.py:652-668)."""
+    nb, nw, nvf = chi0_full.shape[0], chi0_full.shape[4], chi0_full.shape[5]
+    nvc = gamma.shape[-1]
+    d = (nvf - nvc) // 2
+    gu = np.array(_u_full(u_r, nw, nvf))
+    gu[..., d:d + nvc, d:d + nvc] = gamma
+    gm = compound_local_vertex(gu)
+    c0 = compound_local_vertex(np.einsum("abcdwv,vp->abcdwvp", chi0_full, np.eye(nvf)))
+    n = gm.shape[-1]
+    return local_uncompound(gm @ np.linalg.inv(np.eye(n) + c0 @ gm / beta**2), nb)
+
+
+def local_sigma(vrg_d, vrg_m, chi_d, chi_m, g_loc, u, beta, niw, niv):
+    """The frequency-dependent part of the local Schwinger-Dyson self-energy (local_sde.py:150-168):
+    inner_r = vrg_r - vrg_r . U_r . chi_r, inner = (inner_d - inner_m) / 2 on the full bosonic range,
+    Sigma_ij(v) = -(1/beta) sum_w U_kjop inner_ilpo(w, v) G_lk(v - w). Returns [nb, nb, 2 niv]."""
+    def inner(vr, ur, cr):
+        vu = np.einsum("abcdwv,dcef->abefwv", vr, ur)
+        return vr - np.einsum("abefwv,feghw->abghwv", vu, cr)
+
+    full = to_full_niw(0.5 * (inner(vrg_d, u_as_channel(u, "dens"), chi_d) -
+                              inner(vrg_m, u_as_channel(u, "magn"), chi_m)), 1)
+    nivg = g_loc.shape[-1] // 2
+    w = np.arange(-niw, niw + 1)
+    v = np.arange(-niv, niv)
+    g_wv = g_loc[..., nivg + v[None, :] - w[:, None]]  # GreensFunction.get_g_wv: G_ab(v - w)
+    return -1.0 / beta * np.einsum("kjop,ilpowv,lkwv->ijv", u, full, g_wv)
+
+
+def local_hartree(u, occ):
+    """Hartree-Fock constant of the local self-energy: sum_cd U_dens,abcd n_dc (local_sde.py:169)."""
+    return np.einsum("abcd,dc->ab", u_as_channel(u, "dens"), occ)

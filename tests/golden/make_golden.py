@@ -32,6 +32,13 @@ FILES = {
     "end_2_end/siw_dga_local.npy": "e2e_siw_dga_local.npy",
     "end_2_end/f_irrq_dens_pp.npy": "e2e_f_irrq_dens_pp.npy",
     "end_2_end/f_irrq_magn_pp.npy": "e2e_f_irrq_magn_pp.npy",
+    # local Schwinger-Dyson inputs/outputs of the same run (w >= 0 half range)
+    "end_2_end/g_dmft.npy": "e2e_g_dmft.npy",
+    "end_2_end/gchi0_loc.npy": "e2e_gchi0_loc.npy",
+    "end_2_end/vrg_dens_loc.npy": "e2e_vrg_dens_loc.npy",
+    "end_2_end/vrg_magn_loc.npy": "e2e_vrg_magn_loc.npy",
+    "end_2_end/chi_dens_loc.npy": "e2e_chi_dens_loc.npy",
+    "end_2_end/chi_magn_loc.npy": "e2e_chi_magn_loc.npy",
     # lambda correction: 1 band, 4x4x1, beta=12.5
     "lambda_correction/sp/chi_dens_loc.npy": "lc_sp_chi_dens_loc.npy",
     "lambda_correction/sp/chi_magn_loc.npy": "lc_sp_chi_magn_loc.npy",
@@ -50,12 +57,32 @@ FILES = {
 }
 
 
+# Bosonic-frequency subsets of larger files (the quantities are computed per w independently):
+# (source, indices along the w axis, which is axis 4 of a local vertex [o1,o2,o3,o4,w,v,v']).
+W_SUBSETS = {
+    # SrVO3: local full vertices from gamma_{dens,magn}_loc.npy, beta = 1, no shell
+    "srvo3_f_dens_loc_w0_1_10.npy": ("srvo3_end2end/f_dens_loc.npy", [0, 1, 10]),
+    "srvo3_f_magn_loc_w0_1_10.npy": ("srvo3_end2end/f_magn_loc.npy", [0, 1, 10]),
+}
+
+FILES.update({
+    "end_2_end/g_loc.npy": "e2e_g_loc.npy",
+    "srvo3_end2end/g_loc.npy": "srvo3_g_loc.npy",
+})
+
+
 def main(ref_root):
+    import numpy as np
+
     src_root = os.path.join(ref_root, "tests", "test_data")
     dst_root = os.path.dirname(os.path.abspath(__file__))
     for src, dst in FILES.items():
         shutil.copyfile(os.path.join(src_root, src), os.path.join(dst_root, dst))
         print(f"{src} -> {dst}")
+    for dst, (src, ws) in W_SUBSETS.items():
+        data = np.load(os.path.join(src_root, src), allow_pickle=False)
+        np.save(os.path.join(dst_root, dst), np.ascontiguousarray(np.take(data, ws, axis=4)), allow_pickle=False)
+        print(f"{src}[w = {ws}] -> {dst}")
 
 
 if __name__ == "__main__":
