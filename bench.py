@@ -226,10 +226,10 @@ def cpu_baseline(model, target_s):
     """Time the reference's per-q path restated in NumPy/LAPACK (the oracle) on bounded samples of
     the workload, on this host's cores:
     * ladder solves (numpy.linalg.solve with the nb^2 indicator right-hand sides, the reference's
-      v2 path) in complex128 with BLAS threads = the cores given to this process, in complex128 with
-      one thread (the reference runs one MPI rank per core with OMP_NUM_THREADS=1, README.md:125,
-      so its ideal q-parallel rate is the one-thread rate times the cores), and in complex64 (the
-      precision the reference stores) with all threads;
+      v2 path): the headline value runs one single-threaded solver per core in parallel, as the
+      reference runs one MPI rank per core with OMP_NUM_THREADS=1 (README.md:125), in complex128 and
+      in complex64 (the precision the reference stores); beside it one solver with BLAS threads =
+      the cores, and one single-threaded solver;
     * the whole per-q chain of one SDE iteration for one irreducible q-point (direct bubble,
       double-counting kernel, both channels' ladders, three-leg vertex, susceptibility, kernel)
       and the Sigma sum over the full zone, extrapolated to the step (nq times the per-q chain
@@ -265,10 +265,22 @@ def cpu_baseline(model, target_s):
             if el > seconds:
                 return solves, el
 
-    threads = len(os.sched_getaffinity(0))
+    threads = min(len(os.sched_getaffinity(0)), 16)
+
+    def q_parallel(seconds, dtype):
+        """The reference's q-parallel layout on this host: `threads` independent single-threaded
+        solvers (its MPI ranks with OMP_NUM_THREADS=1), here as host threads, each assembling and
+        solving its own ladder matrices (LAPACK releases the interpreter lock)."""
+        from concurrent.futures import ThreadPoolExecutor
+
+        with threadpool_limits(limits=1), ThreadPoolExecutor(threads) as ex:
+            res = list(ex.map(lambda _: timed(seconds, dtype), range(threads)))
+        return sum(r[0] for r in res), max(r[1] for r in res)
+
+    solves, el = q_parallel(target_s, np.complex128)
+    solves32, el32 = q_parallel(target_s / 3, np.complex64)
     with threadpool_limits(limits=threads):
-        solves, el = timed(target_s, np.complex128)
-        solves32, el32 = timed(target_s / 3, np.complex64)
+        solves_blas, el_blas = timed(target_s / 3, np.complex128)
     with threadpool_limits(limits=1):
         solves1, el1 = timed(target_s / 3, np.complex128)
     # one q-point's chain of the SDE iteration and the Sigma sum (all threads)
@@ -291,11 +303,13 @@ def cpu_baseline(model, target_s):
     n = nb * nb * 2 * niv
     return {"value": solves / el, "unit": "solves/s", "cores": threads, "kind": "port",
             "sample": f"{solves} (q,w,channel) ladder solves of n={n} via numpy.linalg.solve (complex128, the "
-                      f"reference's solve-based path) in {el:.1f} s with BLAS threads = {threads}; one thread: "
-                      f"{solves1} solves in {el1:.1f} s; complex64: {solves32} solves in {el32:.1f} s; the per-q chain "
-                      f"of one SDE iteration (1 of {grid.nk_irr} irreducible q) in {t_q:.1f} s and the Sigma sum in "
-                      f"{t_sigma:.1f} s",
+                      f"reference's solve-based path) in {el:.1f} s by {threads} single-threaded solvers in parallel "
+                      f"(the reference's q-parallel ranks); complex64 the same way: {solves32} solves in {el32:.1f} s; "
+                      f"one solver with BLAS threads = {threads}: {solves_blas} solves in {el_blas:.1f} s; one "
+                      f"thread: {solves1} solves in {el1:.1f} s; the per-q chain of one SDE iteration (1 of "
+                      f"{grid.nk_irr} irreducible q) in {t_q:.1f} s and the Sigma sum in {t_sigma:.1f} s",
             "value_1_thread": solves1 / el1, "value_1_thread_x_cores": solves1 / el1 * threads,
+            "value_blas_threads": solves_blas / el_blas,
             "value_c64": solves32 / el32,
             "sde_step_s_extrapolated": t_q * grid.nk_irr + t_sigma,
             "per_q_chain_s": t_q, "sigma_sum_s": t_sigma}
