@@ -75,6 +75,10 @@ SIGNATURES["dgami_pairing_vertex_pp_range"] = ([_i, _i, _i, _i, _i, _i, _c, _c, 
 SIGNATURES["dgami_generalized_chi"] = ([_i, _i, _i, _i, _d, _i, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_local_gamma"] = ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_local_sigma"] = ([_i, _i, _i, _i, _d] + [_c] * 11, _i)
+SIGNATURES["dgami_g_full_dmu"] = ([_i, _i, _i, _d, _c, _c, _c, _i, _i, _c, _c], _i)
+SIGNATURES["dgami_fit_smom"] = ([_i, _i, _i, _d, _c, _c, _c, _c], _i)
+SIGNATURES["dgami_update_mu"] = ([_i, _i, _i, _d, _d, _d, _c, _c, _i, _c, _c, _c, _c, _c], _i)
+SIGNATURES["dgami_occupations"] = ([_i, _i, _i, _d, _c, _c, _i, _c, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_workspace_hold"] = ([], _i)
 SIGNATURES["dgami_workspace_release"] = ([], _i)
 

@@ -309,7 +309,9 @@ namespace dgami {
 template <int T, int TS>
 __global__ void __launch_bounds__(256) g_full_kernel(const double2* __restrict__ ek, const double2* __restrict__ sigma,
                                                     double2* __restrict__ g, int nk, int nv, int sigma_nk, int s_nv,
-                                                    int s_v0, double mu, double beta) {
+                                                    int s_v0, double mu_v, const double* __restrict__ mu_p,
+                                                    double beta) {
+  const double mu = mu_p != nullptr ? *mu_p : mu_v;
   const int lane = threadIdx.x % TS;
   const size_t team = (blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x) / TS;
   const size_t nteam = static_cast<size_t>(nk) * nv;
@@ -357,8 +359,9 @@ __global__ void __launch_bounds__(256) block_invert_kernel(const double2* __rest
 }
 }  // namespace dgami
 
-extern "C" int dgami_g_full(int nk, int nb, int niv, double beta, double mu, const dgami_c128* ek_,
-                            const dgami_c128* sigma_, int sigma_nk, int sigma_niv, dgami_c128* g_, void* stream_) {
+static int g_full_launch(int nk, int nb, int niv, double beta, double mu, const double* mu_p, const dgami_c128* ek_,
+                         const dgami_c128* sigma_, int sigma_nk, int sigma_niv, dgami_c128* g_, void* stream_) {
+  using namespace dgami;
   hipStream_t stream = static_cast<hipStream_t>(stream_);
   DGAMI_ARG_CHECK(ek_ && sigma_ && g_, "g_full: null pointer");
   DGAMI_ARG_CHECK(nk >= 1 && nb >= 1 && nb <= 4 && niv >= 1 && beta > 0, "g_full: invalid sizes");
@@ -371,7 +374,7 @@ extern "C" int dgami_g_full(int nk, int nb, int niv, double beta, double mu, con
   const size_t nteam = static_cast<size_t>(nk) * nv;
 #define LAUNCH(T, TS)                                                                                             \
   hipLaunchKernelGGL((g_full_kernel<T, TS>), dim3(team_blocks<T, TS>(nteam)), dim3(256), 0, stream, ek, sg, g, nk, \
-                     nv, sigma_nk, 2 * sigma_niv, sigma_niv - niv, mu, beta)
+                     nv, sigma_nk, 2 * sigma_niv, sigma_niv - niv, mu, mu_p, beta)
   switch (nb) {
     case 1: LAUNCH(1, 1); break;
     case 2: LAUNCH(2, 2); break;
@@ -381,6 +384,17 @@ extern "C" int dgami_g_full(int nk, int nb, int niv, double beta, double mu, con
 #undef LAUNCH
   DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
+}
+
+extern "C" int dgami_g_full(int nk, int nb, int niv, double beta, double mu, const dgami_c128* ek,
+                            const dgami_c128* sigma, int sigma_nk, int sigma_niv, dgami_c128* g, void* stream) {
+  return g_full_launch(nk, nb, niv, beta, mu, nullptr, ek, sigma, sigma_nk, sigma_niv, g, stream);
+}
+
+extern "C" int dgami_g_full_dmu(int nk, int nb, int niv, double beta, const double* mu, const dgami_c128* ek,
+                                const dgami_c128* sigma, int sigma_nk, int sigma_niv, dgami_c128* g, void* stream) {
+  DGAMI_ARG_CHECK(mu != nullptr, "g_full_dmu: null mu");
+  return g_full_launch(nk, nb, niv, beta, 0.0, mu, ek, sigma, sigma_nk, sigma_niv, g, stream);
 }
 
 extern "C" int dgami_block_invert(int nmat, int nb, const dgami_c128* in_, dgami_c128* out_, int32_t* info,

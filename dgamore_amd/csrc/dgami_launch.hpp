@@ -22,6 +22,7 @@ enum WorkspaceSlot {
   WS_TRI = 11,       // inverse diagonal blocks of the triangular factors
   WS_TRI_VEC = 12,   // complex64 right-hand sides of the triangular sweeps
   WS_SELFCONS = 13,  // Hartree-Fock transforms
+  WS_MU = 14,        // chemical-potential secant: per-k occupations and the secant state
 };
 
 int launch_fft_grid(const double2* in, double2* out, int64_t batch, int nx, int ny, int nz, int sign, double scale,

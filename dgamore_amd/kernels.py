@@ -103,12 +103,45 @@ def fft_grid(x, nk, sign, scale=1.0, out=None):
 
 
 def g_full(ek, sigma, mu, beta, niv):
-    """G(k, v) [N,nb,nb,2 niv] from e_k [N,nb,nb] and Sigma [1 or N][nb][nb][2 niv_sigma] (centred cut)."""
+    """G(k, v) [N,nb,nb,2 niv] from e_k [N,nb,nb] and Sigma [1 or N][nb][nb][2 niv_sigma] (centred cut).
+    mu: a number, or a float64 device tensor of one element (read by the kernel)."""
     N, nb = ek.shape[0], ek.shape[-1]
     g = torch.empty((N, nb, nb, 2 * niv), dtype=C128, device=ek.device)
-    _lib.call("dgami_g_full", N, nb, int(niv), float(beta), float(mu), _p(ek), _p(sigma), sigma.shape[0],
-              sigma.shape[-1] // 2, _p(g), _stream())
+    if isinstance(mu, torch.Tensor):
+        _lib.call("dgami_g_full_dmu", N, nb, int(niv), float(beta), _p(mu, torch.float64, "mu"), _p(ek), _p(sigma),
+                  sigma.shape[0], sigma.shape[-1] // 2, _p(g), _stream())
+    else:
+        _lib.call("dgami_g_full", N, nb, int(niv), float(beta), float(mu), _p(ek), _p(sigma), sigma.shape[0],
+                  sigma.shape[-1] // 2, _p(g), _stream())
     return g
+
+
+def fit_smom(sigma, beta):
+    """Local moments of sigma [N,nb,nb,2 niv] on the device: (mom0, mom1) [nb,nb] complex tensors."""
+    N, nb, _, nv = sigma.shape
+    m0 = torch.empty((nb, nb), dtype=C128, device=sigma.device)
+    m1 = torch.empty_like(m0)
+    _lib.call("dgami_fit_smom", N, nb, nv // 2, float(beta), _p(sigma), _p(m0), _p(m1), _stream())
+    return m0, m1
+
+
+def update_mu(mu, target, ek, sigma, hloc, smom0, beta, mixing=1.0, info=None):
+    """Secant update of the device chemical potential mu (float64 tensor of one element, in place)
+    to the target filling, then mu = mixing mu_new + (1 - mixing) mu_old; info (float64 [3] or
+    None): mu_new (nan on failure), status, smom0 asymmetry."""
+    N, nb = ek.shape[0], ek.shape[-1]
+    _lib.call("dgami_update_mu", N, nb, sigma.shape[-1] // 2, float(beta), float(target), float(mixing), _p(ek),
+              _p(sigma), sigma.shape[0], _p(hloc), _p(smom0), _p(mu, torch.float64, "mu"),
+              _p(info, torch.float64, "info"), _stream())
+    return mu
+
+
+def occupations(ek, sigma, smom0, mu, beta, occ_k, occ):
+    """occ_k [N,nb,nb] and occ [nb,nb] (complex, written in place) of the k-dependent model at the device mu."""
+    N, nb = ek.shape[0], ek.shape[-1]
+    _lib.call("dgami_occupations", N, nb, sigma.shape[-1] // 2, float(beta), _p(ek), _p(sigma), sigma.shape[0],
+              _p(smom0), _p(mu, torch.float64, "mu"), _p(occ_k), _p(occ), _stream())
+    return occ_k, occ
 
 
 def bubble_irrq(giwk, nk, niw, niv, beta, irrk_ind):

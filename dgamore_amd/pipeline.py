@@ -18,7 +18,10 @@ every intermediate kept in HBM in the library's q-fastest layouts:
 
 iterate() adds the chemical-potential update and the new occupations of the
 self-consistency loop (nonlocal_sde.py:895-914): the frequency sums run in
-dgami_fill_sums, the secant iteration on mu stays on the host.
+dgami_fill_sums, the secant iteration on mu on the host. iterate_device() and
+capture_iteration() run the same loop with mu, its secant (dgami_update_mu), the
+local moments (dgami_fit_smom) and the occupations (dgami_occupations) on the
+device, so that one graph replay is one whole iteration.
 
 The MPI distribution of the reference (irreducible q over ranks) becomes the
 (channel, w, q) work list of a single ladder launch.
@@ -252,6 +255,69 @@ class SDEPipeline:
             history.append(mu)
             self.update_occupation(sigma, mu)
         return sigma, history
+
+    # -- the self-consistency loop on the device ---------------------------------------------------
+    def device_loop_state(self, sigma=None, mu=None, n_target=None):
+        """Start the device-resident loop: mu as a float64 device scalar, the occupations at
+        (sigma, mu) and the target filling (that of the start, as iterate() takes it)."""
+        sigma = self.sigma_dmft if sigma is None else sigma
+        mu = self.m.mu if mu is None else mu
+        n = self.update_occupation(sigma, mu)
+        self.n_target = n if n_target is None else n_target
+        self.mu_dev = torch.tensor([float(mu)], dtype=torch.float64, device=self.ek.device)
+        self.mu_info = torch.zeros(3, dtype=torch.float64, device=self.ek.device)
+        self.hloc = K.to_dev(np.asarray(self.m.ek).reshape(self.N, self.m.nb, self.m.nb).mean(axis=0))
+        return self.mu_dev
+
+    def device_iteration(self, sigma):
+        """One iteration of the self-consistency loop with no host work (nonlocal_sde.py:777-914):
+        the SDE step at the device mu, the local moments of the new Sigma (dgami_fit_smom), the secant
+        update of mu to the target filling with mixing (dgami_update_mu) and the occupations of the
+        next Hartree-Fock terms (dgami_occupations). Returns the new Sigma; mu_dev, occ and occ_k are
+        updated in place, mu_info holds the secant's result."""
+        beta = self.m.beta
+        new = self.step(sigma, self.mu_dev)
+        smom0, _ = K.fit_smom(new, beta)
+        K.update_mu(self.mu_dev, self.n_target, self.ek, new, self.hloc, smom0, beta, self.mixing, info=self.mu_info)
+        K.occupations(self.ek, new, smom0, self.mu_dev, beta, self.occ_k, self.occ)
+        return new
+
+    def iterate_device(self, n_iter, sigma=None, mu=None, n_target=None):
+        """iterate() with the chemical potential and the occupations kept on the device. Returns
+        (sigma, mu history)."""
+        sigma = self.sigma_dmft if sigma is None else sigma
+        self.device_loop_state(sigma, mu, n_target)
+        history = [float(self.mu_dev.item())]
+        for _ in range(n_iter):
+            sigma = self.device_iteration(sigma)
+            history.append(float(self.mu_dev.item()))
+        return sigma, history
+
+    def capture_iteration(self, mu=None, n_target=None, chain=True):
+        """Capture one whole self-consistency iteration (device_iteration) as a HIP graph: Sigma from
+        graph_sigma to graph_out, mu in mu_dev and the occupations all updated by the graph, so that
+        chained replays are consecutive iterations of the reference's loop without host work."""
+        if self.lambda_correction is not None:
+            raise ValueError("the lambda correction finds lambda on the host and cannot be captured as a graph")
+        nb = self.m.nb
+        self.graph_sigma = self.sigma_dmft.expand(self.N, nb, nb, 2 * self.niv_s).contiguous().clone()
+        self.device_loop_state(self.graph_sigma, mu, n_target)
+        mu0 = self.mu_dev.clone()
+        occ0, occ_k0 = self.occ.clone(), self.occ_k.clone()
+        self.device_iteration(self.graph_sigma)  # warm-up: workspaces are allocated outside the capture
+        torch.cuda.synchronize()
+        self.mu_dev.copy_(mu0)
+        self.occ.copy_(occ0)
+        self.occ_k.copy_(occ_k0)
+        torch.cuda.synchronize()
+        self.graph = torch.cuda.CUDAGraph()
+        _lib.hold_workspaces(self.graph)
+        with torch.cuda.graph(self.graph):
+            self.graph_out = self.device_iteration(self.graph_sigma)
+            if chain:
+                self.graph_sigma.copy_(self.graph_out)
+        torch.cuda.synchronize()
+        return self.graph
 
     # -- accounting ---------------------------------------------------------------------------
     def solves_per_step(self):

@@ -275,6 +275,11 @@ int dgami_local_sigma(int nb, int niw, int niv, int niv_g, double beta, const dg
                       const dgami_c128* u_dens, const dgami_c128* u_magn, const dgami_c128* u, const dgami_c128* g,
                       const dgami_c128* occ, dgami_c128* sigma, void* stream);
 
+/* dgami_g_full with mu read from device memory (a double), so that a captured iteration can
+ * carry the chemical potential that dgami_update_mu changes on the device. */
+int dgami_g_full_dmu(int nk, int nb, int niv, double beta, const double* mu, const dgami_c128* ek,
+                     const dgami_c128* sigma, int sigma_nk, int sigma_niv, dgami_c128* g, void* stream);
+
 /* Batched inverse of nmat nb^2 x nb^2 blocks stored element-major: in/out [nb^4 (M order)][nmat].
  * info: [nmat] (1 where a pivot was exactly zero) or NULL.
  * Replaces FourPoint.invert for objects without fermionic axes (four_point.py:207-209, 615-618)
@@ -352,6 +357,39 @@ int dgami_sigma_from_kernel(int nx, int ny, int nz, int nb, int niw, int niv_cor
 int dgami_hartree_fock(int nx, int ny, int nz, int nb, const dgami_c128* u, const dgami_c128* v,
                        const dgami_c128* occ, const dgami_c128* occ_k, dgami_c128* hartree, dgami_c128* fock,
                        void* stream);
+
+/* Local moments of a self-energy: mom0 = mean Re Sigma_loc and mom1 = mean Im Sigma_loc * v over the
+ * top n_fit = max(int(0.2 niv), 4) positive frequencies of the k-mean Sigma_loc.
+ * sigma: [nk][nb][nb][2 niv]; mom0, mom1: [nb][nb] (imaginary parts zero).
+ * Replaces SelfEnergy.fit_smom (self_energy.py:51-75). */
+int dgami_fit_smom(int nk, int nb, int niv, double beta, const dgami_c128* sigma, dgami_c128* mom0, dgami_c128* mom1,
+                   void* stream);
+
+/* Chemical-potential update on the device: the secant iteration of update_mu on
+ *   f(mu) = 2 Tr[ Fermi(beta (Re hloc + smom0 - mu)) + (1/nk) sum_k occ_k(mu) ] - target
+ * (occ_k: the Matsubara sums of dgami_fill_sums with the model hloc) from mu_old = *mu, with the
+ * derivative-free steps, tolerance (|p - p1| <= 1e-6) and 50-iteration limit of
+ * scipy.optimize.newton, then *mu = mixing mu_new + (1 - mixing) mu_old, or *mu unchanged when the
+ * iteration fails (the reference's update_mu returns nan then and its caller keeps mu).
+ * mu: one double in device memory (in/out); info: three doubles or NULL: mu_new (nan on failure),
+ * status (1 converged, 2 failed), the largest |smom0 - smom0^T| met (the model's density matrix
+ * is formed from the symmetric part of smom0, exact for a symmetric Sigma). The number of kernel
+ * launches is fixed (evaluations after convergence return at once), so the call can be captured.
+ * hloc, smom0: [nb][nb] (real parts used); ek, sigma as in dgami_fill_sums.
+ * Replaces update_mu / root_fun / get_total_fill (greens_function.py:20-80) and the mu mixing of
+ * the self-consistency loop (nonlocal_sde.py:895-911). */
+int dgami_update_mu(int nk, int nb, int niv, double beta, double target, double mixing, const dgami_c128* ek,
+                    const dgami_c128* sigma, int sigma_nk, const dgami_c128* hloc, const dgami_c128* smom0, double* mu,
+                    double* info, void* stream);
+
+/* Occupations of the k-dependent model at the device mu:
+ *   occ_k = Fermi(beta (Re e_k + smom0 - mu)) + Matsubara part of dgami_fill_sums (e_k model),
+ *   occ = k-mean of occ_k, entries below 1e-12 set to zero (complex outputs, imaginary parts zero).
+ * occ_k: [nk][nb][nb]; occ: [nb][nb]. Replaces GreensFunction.get_fill_nonlocal
+ * (greens_function.py:212-240) as called by the self-consistency loop (nonlocal_sde.py:911-914). */
+int dgami_occupations(int nk, int nb, int niv, double beta, const dgami_c128* ek, const dgami_c128* sigma,
+                      int sigma_nk, const dgami_c128* smom0, const double* mu, dgami_c128* occ_k, dgami_c128* occ,
+                      void* stream);
 
 /* New self-energy of one iteration and its linear mixing:
  *   val[k][a][b][v] = sigma_nl + H_ab + F_ab(k) + delta_sigma_ab(v)   for |v| < niv_core (centred)
