@@ -134,7 +134,8 @@ def eliashberg_matvec(steps, warmup):
 def batch_graph(steps, warmup, seed, symmetric=True):
     """16 seeded models of uneven sizes in one ModelBatch: autotune every ladder shape, time one eager
     step, capture the step of all models as one HIP graph, check one replay against the eager step,
-    then time `steps` chained replays (self-consistency iterations with mixing 0.3 at fixed mu)."""
+    then time `steps` chained replays (whole self-consistency iterations with mixing 0.3: every
+    model's mu secant and occupations run inside the graph)."""
     import torch
 
     from dgamore_amd import synthetic as S
@@ -153,25 +154,40 @@ def batch_graph(steps, warmup, seed, symmetric=True):
     t_tune = time.perf_counter() - t1
     print(f"[bench] autotuned {len(tuned)} ladder shapes in {t_tune:.1f}s", file=sys.stderr, flush=True)
     stream = torch.cuda.current_stream()
+    # every step is a whole self-consistency iteration: each model's mu secant and occupations run on
+    # the device after its new Sigma
+    mb.start_scf()
+    state0 = [(p.mu_dev.clone(), p.occ.clone(), p.occ_k.clone()) for p in mb.pipes]
+
+    def restore():
+        for p, (mu, occ, occ_k) in zip(mb.pipes, state0):
+            p.mu_dev.copy_(mu)
+            p.occ.copy_(occ)
+            p.occ_k.copy_(occ_k)
+
     for _ in range(max(1, warmup)):
-        eager = mb.step()
+        restore()
+        eager = mb.step(scf=True)
+    restore()
     torch.cuda.synchronize()
     e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     e0.record(stream)
-    eager = mb.step()
+    eager = mb.step(scf=True)
     e1.record(stream)
     torch.cuda.synchronize()
     ms_eager = e0.elapsed_time(e1)
+    eager_mu = [float(p.mu_dev.item()) for p in mb.pipes]
     print(f"[bench] eager step of all models {ms_eager:.2f} ms", file=sys.stderr, flush=True)
-    mb.capture(chain=True)
+    mb.capture(chain=True, scf=True)
     start = [s.clone() for s in mb.graph_sigmas]
-    for s, s0 in zip(mb.graph_sigmas, start):
-        s.copy_(s0)
+    state0 = [(p.mu_dev.clone(), p.occ.clone(), p.occ_k.clone()) for p in mb.pipes]
     outs = mb.replay()
     torch.cuda.synchronize()
     err = max(float(torch.linalg.vector_norm(o - e) / torch.linalg.vector_norm(e)) for o, e in zip(outs, eager))
+    mu_err = max(abs(float(p.mu_dev.item()) - m) for p, m in zip(mb.pipes, eager_mu))
     for s, s0 in zip(mb.graph_sigmas, start):
         s.copy_(s0)
+    restore()
     torch.cuda.synchronize()
     e0.record(stream)
     for _ in range(steps):
@@ -191,11 +207,13 @@ def batch_graph(steps, warmup, seed, symmetric=True):
         "data": "synthetic (16 seeded DMFT-like models, random-perturbation local vertices)",
         "config": {"workload": f"batch of {len(models)} models (seed {seed}): 1-3 orbitals, beta 5-40, niv=niw="
                                f"round(1.5 beta), k-grids 16^2-32^2, compound sizes {min(sizes)}..{max(sizes)}; "
-                               "one HIP graph per SCF iteration of all models, linear mixing 0.3",
+                               "one HIP graph per SCF iteration of all models (mu secant and occupations "
+                               "on the device), linear mixing 0.3",
                    "models": [p.m.name for p in mb.pipes], "n_compound": sizes, "solves_per_step": solves},
         "eager_ms_per_step": ms_eager, "graph_speedup": ms_eager / ms,
         "ladder_tflops_per_s": mb.ladder_flops_per_step() / (ms * 1e-3) / 1e12,
-        "replay_vs_eager_rel_err": err, "finite": finite,
+        "replay_vs_eager_rel_err": err, "replay_vs_eager_mu_err": mu_err, "finite": finite,
+        "mu_after_replays": [float(p.mu_dev.item()) for p in mb.pipes],
         "sigma_change_over_replays": delta,
         "autotune": {f"n={t['n']},nrhs={t['nrhs']}": {"choice": t["name"], "source": t["source"],
                                                       "times_ms": t.get("times_ms", {})} for t in tuned},

@@ -275,8 +275,12 @@ class SDEPipeline:
         update of mu to the target filling with mixing (dgami_update_mu) and the occupations of the
         next Hartree-Fock terms (dgami_occupations). Returns the new Sigma; mu_dev, occ and occ_k are
         updated in place, mu_info holds the secant's result."""
+        return self.scf_update(self.step(sigma, self.mu_dev))
+
+    def scf_update(self, new):
+        """The part of the self-consistency loop after the SDE step, on the device: local moments of
+        the new Sigma, the secant update of mu_dev with mixing, and the occupations at the new mu."""
         beta = self.m.beta
-        new = self.step(sigma, self.mu_dev)
         smom0, _ = K.fit_smom(new, beta)
         K.update_mu(self.mu_dev, self.n_target, self.ek, new, self.hloc, smom0, beta, self.mixing, info=self.mu_info)
         K.occupations(self.ek, new, smom0, self.mu_dev, beta, self.occ_k, self.occ)
@@ -336,8 +340,9 @@ class ModelBatch:
     dgami_ladder_solve_sum_multi call (the small compound sizes share register-kernel launches, the
     large ones take the blocked LU, every shape in the kernel the autotuner picked), then each
     model's back part to its new, mixed Sigma. capture() records the whole step of all models as
-    one HIP graph; with chain, consecutive replays are consecutive self-consistency iterations at
-    fixed mu and occupations. The reference runs one model per invocation
+    one HIP graph; with chain, consecutive replays are consecutive self-consistency iterations, at
+    fixed mu and occupations, or (scf) with every model's mu secant and occupations on the device
+    in the graph. The reference runs one model per invocation
     (nonlocal_sde.py:777-914); this replaces a loop of such runs."""
 
     def __init__(self, models, mixing=0.3, symmetries="two_dimensional_square"):
@@ -355,25 +360,49 @@ class ModelBatch:
                 out.append(autotune.tune_ladder(*key, nchan=len(p.CHANNELS), **kw))
         return out
 
-    def step(self, sigmas=None, mus=None, keep=False):
+    def step(self, sigmas=None, mus=None, keep=False, scf=False):
+        """One step of every model. scf: each model's mu is its device mu (start_scf) and the step
+        ends with its secant update of mu and its new occupations, a whole self-consistency
+        iteration of every model."""
         sigmas = [p.sigma_dmft for p in self.pipes] if sigmas is None else sigmas
+        if scf:
+            mus = [p.mu_dev for p in self.pipes]
         mus = [None] * len(self.pipes) if mus is None else mus
         fronts = [p.front(s, mu) for p, s, mu in zip(self.pipes, sigmas, mus)]
         outs = K.ladder_solve_sum_multi([p.ladder_model(f) for p, f in zip(self.pipes, fronts)],
                                         check_info=not K.capturing())
-        return [p.back(f, aux, info, s, None, keep) for p, f, (aux, info), s in zip(self.pipes, fronts, outs, sigmas)]
+        news = [p.back(f, aux, info, s, None, keep) for p, f, (aux, info), s in zip(self.pipes, fronts, outs, sigmas)]
+        if scf:
+            news = [p.scf_update(n) for p, n in zip(self.pipes, news)]
+        return news
 
-    def capture(self, chain=True):
+    def start_scf(self, sigmas=None):
+        """Device mu, occupations and target filling of every model at its start (SDEPipeline.device_loop_state)."""
+        sigmas = [None] * len(self.pipes) if sigmas is None else sigmas
+        for p, s in zip(self.pipes, sigmas):
+            p.device_loop_state(s)
+
+    def capture(self, chain=True, scf=False):
         """Record one step of every model as one HIP graph reading graph_sigmas and writing
-        graph_outs (and, with chain, copying graph_outs back into graph_sigmas)."""
+        graph_outs (and, with chain, copying graph_outs back into graph_sigmas). scf: the step is a
+        whole self-consistency iteration with each model's mu and occupations on the device."""
         self.graph_sigmas = [p.sigma_dmft.expand(p.N, p.m.nb, p.m.nb, 2 * p.niv_s).contiguous().clone()
                              for p in self.pipes]
-        self.step(self.graph_sigmas)  # warm-up: workspaces and occupations
+        if scf:
+            self.start_scf(self.graph_sigmas)
+            saved = [(p.mu_dev.clone(), p.occ.clone(), p.occ_k.clone()) for p in self.pipes]
+        self.step(self.graph_sigmas, scf=scf)  # warm-up: workspaces and occupations
         torch.cuda.synchronize()
+        if scf:  # the warm-up moved mu and the occupations; the graph starts from the initial state
+            for p, (mu, occ, occ_k) in zip(self.pipes, saved):
+                p.mu_dev.copy_(mu)
+                p.occ.copy_(occ)
+                p.occ_k.copy_(occ_k)
+            torch.cuda.synchronize()
         self.graph = torch.cuda.CUDAGraph()
         _lib.hold_workspaces(self.graph)
         with torch.cuda.graph(self.graph):
-            self.graph_outs = self.step(self.graph_sigmas)
+            self.graph_outs = self.step(self.graph_sigmas, scf=scf)
             if chain:
                 for s, o in zip(self.graph_sigmas, self.graph_outs):
                     s.copy_(o)

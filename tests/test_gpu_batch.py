@@ -173,3 +173,27 @@ def test_config3_batch_at_full_size(K):
                     assert rel(got, ref) < RTOL, (m.name, ch)
             say("ladder sums of every model match the oracle on one q-point; replays continue")
     assert all(bool(torch.isfinite(s).all()) for s in sig)
+
+
+def test_model_batch_scf_graph_matches_eager_iterations(K, A):
+    """The captured step of every model as a whole self-consistency iteration (mu secant and
+    occupations on the device): chained replays equal eager scf steps, Sigma and every model's mu."""
+    from dgamore_amd.pipeline import ModelBatch
+
+    models = _small_batch()
+    say("model batch HIP graph with the device mu secant, chained replays vs eager iterations (3)")
+    mb = ModelBatch(models, mixing=0.3)
+    mb.capture(chain=True, scf=True)
+    eager = ModelBatch(models, mixing=0.3)
+    sig = [s.clone() for s in mb.graph_sigmas]
+    eager.start_scf(sig)
+    mu0 = [float(p.mu_dev.item()) for p in mb.pipes]
+    for it in range(3):
+        mb.replay()
+        torch.cuda.synchronize()
+        sig = eager.step(sig, scf=True)
+        torch.cuda.synchronize()
+        for g, e, pg, pe, m in zip(mb.graph_sigmas, sig, mb.pipes, eager.pipes, models):
+            assert rel(g, e.cpu().numpy()) < 1e-12, (it, m.name)
+            assert abs(float(pg.mu_dev.item()) - float(pe.mu_dev.item())) < 1e-12, (it, m.name)
+    assert max(abs(float(p.mu_dev.item()) - m) for p, m in zip(mb.pipes, mu0)) > 1e-8
