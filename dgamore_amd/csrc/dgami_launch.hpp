@@ -23,6 +23,7 @@ enum WorkspaceSlot {
   WS_TRI_VEC = 12,   // complex64 right-hand sides of the triangular sweeps
   WS_SELFCONS = 13,  // Hartree-Fock transforms
   WS_MU = 14,        // chemical-potential secant: per-k occupations and the secant state
+  WS_LADDER_G32 = 15, // complex64 scaled vertex of the fused first block of the blocked LU
 };
 
 int launch_fft_grid(const double2* in, double2* out, int64_t batch, int nx, int ny, int nz, int sign, double scale,

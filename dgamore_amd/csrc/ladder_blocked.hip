@@ -44,23 +44,48 @@ constexpr int MAP_STRIDE = BLK_MAP_STRIDE;  // interchange map per problem: coun
 constexpr int ASM_ROWS = 8;
 constexpr int ASM_QG = 8;
 
-template <typename S>
+template <typename S, bool NT>
 __device__ __forceinline__ void store_pair(S* dst, double2 x, double2 y) {
+  typedef float F4 __attribute__((ext_vector_type(4)));
+  typedef double D2 __attribute__((ext_vector_type(2)));
   if constexpr (std::is_same<S, float2>::value) {
-    *reinterpret_cast<float4*>(dst) = make_float4(static_cast<float>(x.x), static_cast<float>(x.y),
-                                                  static_cast<float>(y.x), static_cast<float>(y.y));
+    const F4 v = F4{static_cast<float>(x.x), static_cast<float>(x.y), static_cast<float>(y.x), static_cast<float>(y.y)};
+    if constexpr (NT)
+      __builtin_nontemporal_store(v, reinterpret_cast<F4*>(dst));
+    else
+      *reinterpret_cast<F4*>(dst) = v;
   } else {
-    dst[0] = x;
-    dst[1] = y;
+    if constexpr (NT) {
+      __builtin_nontemporal_store(D2{x.x, x.y}, reinterpret_cast<D2*>(dst));
+      __builtin_nontemporal_store(D2{y.x, y.y}, reinterpret_cast<D2*>(dst + 1));
+    } else {
+      dst[0] = x;
+      dst[1] = y;
+    }
   }
 }
 
 template <typename S>
+__device__ __forceinline__ S to_storage(double2 x) {
+  if constexpr (std::is_same<S, float2>::value)
+    return make_float2(static_cast<float>(x.x), static_cast<float>(x.y));
+  else
+    return x;
+}
+
+// With a fused first block (st.fuse_e = E > 0) only columns [0, E) of every row and the rows
+// [0, E) are written, and the rows' flags are cleared (E is a multiple of ASM_ROWS).
+template <typename S, bool NT>
 __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, BlockedState<S> st, int item0, int nbatch) {
   const int b0 = blockIdx.y * ASM_QG;
   const int ng = min(ASM_QG, nbatch - b0);
-  const int n = st.n, ld = st.ld, ncol = st.ncol, nv = a.nv, T = a.T;
+  const int n = st.n, ncol = st.ncol, nv = a.nv, T = a.T;
   const int i0 = blockIdx.x * ASM_ROWS, r1 = min(n, i0 + ASM_ROWS);
+  const int E = st.fuse_e;
+  const int ld = (E > 0 && i0 >= E) ? E : st.ld;  // columns written in these rows (ld and E are even)
+  if (E > 0)
+    for (int e = threadIdx.x; e < ng * (r1 - i0); e += blockDim.x)
+      st.rowflag[static_cast<size_t>(b0 + e / (r1 - i0)) * n + i0 + e % (r1 - i0)] = 0;
   const double2 zero = make_double2(0.0, 0.0);
   // the group's problems: matrix, U_r + V_r(q) and chi0^-1 bases, output matrix
   const double2* Gk[ASM_QG];
@@ -79,7 +104,7 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
     wk[k] = w;
     qk[k] = q;
   }
-  for (int jb = 2 * threadIdx.x; jb < ld; jb += 512) {  // ld is even
+  for (int jb = 2 * threadIdx.x; jb < ld; jb += 2 * blockDim.x) {  // ld is even
     int cj[2], vj[2];
 #pragma unroll
     for (int h = 0; h < 2; ++h) {
@@ -115,8 +140,22 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
             val[h] = ladder_rhs_at(a, wk[k], qk[k], i, j - n);
           }
         }
-        store_pair(st.W + static_cast<size_t>(b0 + k) * n * ld + static_cast<size_t>(i) * ld + jb, val[0], val[1]);
+        store_pair<S, NT>(st.W + static_cast<size_t>(b0 + k) * n * st.ld + static_cast<size_t>(i) * st.ld + jb, val[0],
+                          val[1]);
       }
+    }
+  }
+  if (E > 0 && i0 >= E && st.g32 != nullptr) {
+    // complex64 fused block: the trailing elements of these rows that the complex64 vertex does
+    // not hold, the v = v' diagonal of every orbital pair and the right-hand sides, are written here
+    const int nr = ncol - n, per = T + nr, rows = r1 - i0;
+    for (int e = threadIdx.x; e < rows * ng * per; e += blockDim.x) {
+      const int s2 = e % per, k = (e / per) % ng, i = i0 + e / (per * ng);
+      const int c = s2 < T ? s2 * nv + (i % nv) : n + (s2 - T);
+      if (c < E) continue;
+      const LadderSrc src = ladder_src(a, item0 + b0 + k);
+      st.W[static_cast<size_t>(b0 + k) * n * st.ld + static_cast<size_t>(i) * st.ld + c] =
+          to_storage<S>(ladder_aug_at(a, src, i, c));
     }
   }
   if (blockIdx.x == 0 && threadIdx.x < ng) st.info[b0 + threadIdx.x] = 0;
@@ -341,6 +380,19 @@ __global__ void __launch_bounds__(MAXTH) blk_panel_kernel(BlockedState<S> st, in
 // ntile_a tiles of 64 columns cover [c_lo, c_hi), the rest [0, l_hi).
 constexpr int ROWS_SWAP = 1, ROWS_TRSM = 2;
 
+// Row r of column col of problem b as the trailing update of a fused first block sees it: W
+// once the row has been written (flag set by an earlier slot than `slot`), else the source
+// matrix. Rows above fuse_e are always in W.
+template <typename S>
+__device__ __forceinline__ S fused_read(const BlockedState<S>& st, const S* W, int b, int r, int col, int slot,
+                                        const LadderSrc& src) {
+  if (r >= st.fuse_e) {
+    const int f = st.rowflag[static_cast<size_t>(b) * st.n + r];
+    if (f == 0 || f > slot) return to_storage<S>(ladder_aug_at(st.src, src, r, col));
+  }
+  return W[static_cast<size_t>(r) * st.ld + col];
+}
+
 template <typename S, int NB>
 __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0, int slot, int c_lo, int c_hi,
                                                       int l_hi, int flags, int ntile_a, int ntile, int nprob) {
@@ -374,11 +426,29 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
 #pragma unroll
     for (int c = 0; c < NB; ++c)
       pv[c] = (c < nbw && s_dest[c] >= 0) ? W[static_cast<size_t>(p0 + c) * ld + col] : czero<S>();
-    for (int e = 0; e < outer; ++e)
-      W[static_cast<size_t>(s_dst[e]) * ld + col] = W[static_cast<size_t>(s_src[e]) * ld + col];
+    if (st.fuse_e > 0 && p0 < st.fuse_e && col >= st.fuse_e) {
+      // trailing columns of a fused first block: rows below it that were not written yet are the
+      // source rows, and a row written here is flagged with this panel's slot
+      const LadderSrc src = ladder_src(st.src, st.item0 + b);
+      for (int e = 0; e < outer; ++e)
+        W[static_cast<size_t>(s_dst[e]) * ld + col] = fused_read<S>(st, W, b, s_src[e], col, slot, src);
 #pragma unroll
-    for (int c = 0; c < NB; ++c)
-      if (c < nbw && s_dest[c] >= 0) W[static_cast<size_t>(s_dest[c]) * ld + col] = pv[c];
+      for (int c = 0; c < NB; ++c)
+        if (c < nbw && s_dest[c] >= 0) {
+          const int d = s_dest[c];
+          W[static_cast<size_t>(d) * ld + col] = pv[c];
+          if (d >= st.fuse_e) {
+            int* f = st.rowflag + static_cast<size_t>(b) * n + d;
+            if (*f == 0) *f = slot + 1;
+          }
+        }
+    } else {
+      for (int e = 0; e < outer; ++e)
+        W[static_cast<size_t>(s_dst[e]) * ld + col] = W[static_cast<size_t>(s_src[e]) * ld + col];
+#pragma unroll
+      for (int c = 0; c < NB; ++c)
+        if (c < nbw && s_dest[c] >= 0) W[static_cast<size_t>(s_dest[c]) * ld + col] = pv[c];
+    }
   }
   if (!(flags & ROWS_TRSM) || col < p0) return;
   S x[NB];
@@ -401,10 +471,56 @@ __global__ void __launch_bounds__(64) blk_rows_kernel(BlockedState<S> st, int p0
 struct GemmRange {
   int r0, r1, c0, c1, k0, kw;
   int tiles_r, tiles_c;
+  int fused;  // the trailing update of a fused first block: C rows not yet written come from the source
 };
 
-inline GemmRange gemm_range(int r0, int r1, int c0, int c1, int k0, int kw, int tm = 64) {
-  return GemmRange{r0, r1, c0, c1, k0, kw, static_cast<int>(ceil_div(r1 - r0, tm)), static_cast<int>(ceil_div(c1 - c0, 64))};
+inline GemmRange gemm_range(int r0, int r1, int c0, int c1, int k0, int kw, int tm = 64, int fused = 0) {
+  return GemmRange{r0, r1, c0, c1, k0, kw, static_cast<int>(ceil_div(r1 - r0, tm)), static_cast<int>(ceil_div(c1 - c0, 64)),
+                   fused};
+}
+
+// The complex64 vertex of problem b's (chan, w) group for a fused update, or nullptr.
+template <typename S>
+__device__ __forceinline__ const float2* fused_vertex32(const BlockedState<S>& st, const GemmRange& g, int b) {
+  if (st.g32 == nullptr) return nullptr;
+  const int grp = (st.item0 + b) / st.src.nq;
+  return st.g32 + static_cast<size_t>(grp - st.g32_first) * st.n * st.n;
+}
+
+// C element (r, c) of an update: W, or for a fused update (all its rows lie below the fused
+// block) the source matrix where the block's interchanges did not write the row.
+template <typename S>
+__device__ __forceinline__ S gemm_c_at(const BlockedState<S>& st, const S* W, const GemmRange& g, int b, int r, int c,
+                                       const LadderSrc& src, const float2* g32) {
+  if (st.rowflag[static_cast<size_t>(b) * st.n + r] == 0) {
+    if constexpr (std::is_same<S, float2>::value) {
+      // off the v = v' diagonal the element is the stored complex64 (gamma - u_r) / beta^2; the
+      // assembly wrote the diagonal and the right-hand sides of these rows
+      if (g32 != nullptr)
+        return (c < st.n && r % st.src.nv != c % st.src.nv) ? g32[static_cast<size_t>(r) * st.n + c]
+                                                            : W[static_cast<size_t>(r) * st.ld + c];
+    }
+    return to_storage<S>(ladder_aug_at(st.src, src, r, c));
+  }
+  return W[static_cast<size_t>(r) * st.ld + c];
+}
+
+// The complex64 vertex of a fused first block: (gamma - u_r) / beta^2 of the (chan, w) groups
+// [g0, g0 + ng) in complex64, each element as the assembly forms it.
+__global__ void __launch_bounds__(256) blk_vertex32_kernel(LadderArgs a, float2* __restrict__ g32, int g0, int ng) {
+  const size_t nn = static_cast<size_t>(a.n) * a.n;
+  const size_t total = nn * ng;
+  for (size_t e = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x; e < total;
+       e += static_cast<size_t>(gridDim.x) * blockDim.x) {
+    const int gl = static_cast<int>(e / nn);
+    const size_t ij = e - static_cast<size_t>(gl) * nn;
+    const int grp = g0 + gl, chan = grp / a.nw;
+    const int i = static_cast<int>(ij / a.n), j = static_cast<int>(ij - static_cast<size_t>(i) * a.n);
+    const int ri = i / a.nv, cj = j / a.nv;
+    const double2 x = cscale(csub(a.gamma[static_cast<size_t>(grp) * nn + ij], a.u_r[chan * a.T * a.T + ri * a.T + cj]),
+                             a.inv_beta2);
+    g32[e] = make_float2(static_cast<float>(x.x), static_cast<float>(x.y));
+  }
 }
 
 template <typename S, int KC, int TM = 64>
@@ -423,7 +539,7 @@ __device__ __forceinline__ void gemm_stage(const S* W, int ld, const GemmRange& 
 // Thread (ty, tx) of the 16x16 layout owns rows ty + 16 i (i < RI) and columns tx + 16 j
 // (j < 4) of a (16 RI) x 64 tile; RI = 8 gives 12 LDS reads per 32 complex FMAs, enough
 // to keep the FP64 pipes busy.
-template <typename S, int KC, int RI>
+template <typename S, int KC, int RI, bool FUSED = false>
 __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, GemmRange g, int nprob) {
   constexpr int TM = 16 * RI;
   int b, t;
@@ -438,13 +554,25 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, GemmR
   const int ty = tid / 16, tx = tid % 16;
   // the C tile is loaded first so that its latency overlaps the operand staging
   S acc[RI][4];
+  if constexpr (FUSED) {
+    const LadderSrc src = ladder_src(st.src, st.item0 + b);
+    const float2* g32 = fused_vertex32(st, g, b);
 #pragma unroll
-  for (int i = 0; i < RI; ++i)
+    for (int i = 0; i < RI; ++i)
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int r = rt + ty + 16 * i, c = ct + tx + 16 * j;
-      acc[i][j] = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
-    }
+      for (int j = 0; j < 4; ++j) {
+        const int r = rt + ty + 16 * i, c = ct + tx + 16 * j;
+        acc[i][j] = (r < g.r1 && c < g.c1) ? gemm_c_at<S>(st, W, g, b, r, c, src, g32) : czero<S>();
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < RI; ++i)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int r = rt + ty + 16 * i, c = ct + tx + 16 * j;
+        acc[i][j] = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : czero<S>();
+      }
+  }
   for (int kc = 0; kc < g.kw; kc += KC) {
     const int kn = min(KC, g.kw - kc);
     if (kc > 0) __syncthreads();
@@ -481,7 +609,7 @@ __global__ void __launch_bounds__(256) blk_gemm_kernel(BlockedState<S> st, GemmR
 // C_im -= [a_im, a_re] . [b_re; b_im]. Operand layout of the 16x16x4 MFMA (lane l):
 // a = A[l%16][l/16], b = B[l/16][l%16], d[e] = D[4*(l/16)+e][l%16]. The accumulators start
 // from the C tile and the A operand enters negated.
-template <int KC>
+template <int KC, bool FUSED = false>
 __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<float2> st, GemmRange g, int nprob) {
   typedef float V4 __attribute__((ext_vector_type(4)));
   int b, t;
@@ -497,6 +625,8 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<float2>
   const int wr = (wave >> 1) * 32, wc = (wave & 1) * 32;
   const int l16 = lane & 15, kq = lane >> 4, kcl = kq >> 1;
   const bool odd = (kq & 1) != 0;
+  const LadderSrc src = FUSED ? ladder_src(st.src, st.item0 + b) : LadderSrc{};
+  const float2* g32 = FUSED ? fused_vertex32(st, g, b) : nullptr;
   V4 accr[2][2], acci[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -506,7 +636,13 @@ __global__ void __launch_bounds__(256) blk_gemm_mfma_kernel(BlockedState<float2>
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = rt + wr + 16 * i + 4 * kq + e;
-        const float2 v = (r < g.r1 && c < g.c1) ? W[static_cast<size_t>(r) * ld + c] : make_float2(0.f, 0.f);
+        float2 v = make_float2(0.f, 0.f);
+        if (r < g.r1 && c < g.c1) {
+          if constexpr (FUSED)
+            v = gemm_c_at<float2>(st, W, g, b, r, c, src, g32);
+          else
+            v = W[static_cast<size_t>(r) * ld + c];
+        }
         accr[i][j][e] = v.x;
         acci[i][j][e] = v.y;
       }
@@ -580,7 +716,7 @@ __device__ __forceinline__ __bf16 __attribute__((ext_vector_type(8))) swap_negat
   return __builtin_bit_cast(__bf16 __attribute__((ext_vector_type(8))), u);
 }
 
-template <int KC>
+template <int KC, bool FUSED = false>
 __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float2> st, GemmRange g, int nprob) {
   typedef float V4 __attribute__((ext_vector_type(4)));
   typedef __bf16 B8 __attribute__((ext_vector_type(8)));
@@ -606,17 +742,56 @@ __global__ void __launch_bounds__(256) blk_gemm_bf16x3_kernel(BlockedState<float
   // (reading it after the products instead frees 32 registers for a third wave per SIMD, but
   // measured 7% slower)
   float2 cin[2][2][4];
+  if constexpr (FUSED) {
+    // rows not yet written come from the complex64 vertex (off the v = v' diagonal) or the source;
+    // each row's flag and frequency, and each column's, are looked up once
+    const LadderSrc src = ladder_src(st.src, st.item0 + b);
+    const float2* g32 = fused_vertex32(st, g, b);
+    const int n = st.n, nv = st.src.nv;
+    int rflag[2][4], rv[2][4], cv[2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i)
-#pragma unroll
-    for (int j = 0; j < 2; ++j) {
-      const int c = ct + wc + 16 * j + l16;
+    for (int i = 0; i < 2; ++i)
 #pragma unroll
       for (int e = 0; e < 4; ++e) {
         const int r = rt + wr + 16 * i + 4 * kg + e;
-        cin[i][j][e] = (full || (r < g.r1 && c < g.c1)) ? W[static_cast<unsigned>(r * ld + c)] : make_float2(0.f, 0.f);
+        rflag[i][e] = r < g.r1 ? st.rowflag[static_cast<size_t>(b) * n + r] : 1;
+        rv[i][e] = r % nv;
       }
-    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j) cv[j] = (ct + wc + 16 * j + l16) % nv;
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int c = ct + wc + 16 * j + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = rt + wr + 16 * i + 4 * kg + e;
+          float2 x = make_float2(0.f, 0.f);
+          if (r < g.r1 && c < g.c1) {
+            if (g32 == nullptr)
+              x = rflag[i][e] != 0 ? W[static_cast<unsigned>(r * ld + c)]
+                                   : to_storage<float2>(ladder_aug_at(st.src, src, r, c));
+            else  // the assembly wrote the diagonal and the right-hand sides of these rows
+              x = (rflag[i][e] != 0 || c >= n || rv[i][e] == cv[j]) ? W[static_cast<unsigned>(r * ld + c)]
+                                                                     : g32[static_cast<unsigned>(r * n + c)];
+          }
+          cin[i][j][e] = x;
+        }
+      }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const int c = ct + wc + 16 * j + l16;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+          const int r = rt + wr + 16 * i + 4 * kg + e;
+          cin[i][j][e] = (full || (r < g.r1 && c < g.c1)) ? W[static_cast<unsigned>(r * ld + c)] : make_float2(0.f, 0.f);
+        }
+      }
+  }
   V4 accr[2][2], acci[2][2];
 #pragma unroll
   for (int i = 0; i < 2; ++i)
@@ -764,6 +939,12 @@ int blocked_copy_info(const int* st_info, int32_t* info, int item0, int nitems, 
 // vector-FMA kernel: the FP64 MFMA (v_mfma_f64_16x16x4_f64) returned wrong products on
 // gfx950 when the compiler rewrote its source registers right after issue, which source
 // code cannot prevent.
+// The first outer block takes over the assembly of its trailing rows unless DGAMI_LADDER_FUSE=0.
+static bool ladder_fuse_enabled() {
+  const char* v = getenv("DGAMI_LADDER_FUSE");
+  return !(v && std::string(v) == "0");
+}
+
 static bool use_vector_gemm() {
   const char* v = getenv("DGAMI_BLOCKED_GEMM");
   return v && std::string(v) == "vector";
@@ -795,7 +976,7 @@ struct BlockedSteps {
     DGAMI_HIP_CHECK(hipGetLastError());
     return 0;
   }
-  int gemm(int r0, int r1, int c0, int c1, int k0, int kw) {
+  int gemm(int r0, int r1, int c0, int c1, int k0, int kw, int fused = 0) {
     if (r1 <= r0 || c1 <= c0 || kw <= 0) return 0;
     int mode = BLK_GEMM_VECTOR;
     if constexpr (std::is_same<S, float2>::value) mode = use_vector_gemm() ? BLK_GEMM_VECTOR : st.gemm_mode;
@@ -805,31 +986,44 @@ struct BlockedSteps {
     // a 128-row tile over the short updates is mostly empty.
     constexpr bool c128 = std::is_same<S, double2>::value;
     const bool tall = c128 && mode == BLK_GEMM_VECTOR && r1 - r0 > 96 && st.tall_tiles;
-    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw, tall ? 128 : 64);
+    const GemmRange g = gemm_range(r0, r1, c0, c1, k0, kw, tall ? 128 : 64, fused);
     const size_t grid = xcd_grid(nb, static_cast<size_t>(g.tiles_r) * g.tiles_c);
     DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: trailing update grid too large");
     hipEvent_t start = nullptr;
     const bool prof = profile_on();
     if (prof && profile_begin(stream, &start)) return -2;
+    const dim3 gd(static_cast<unsigned>(grid));
     if (mode == BLK_GEMM_VECTOR) {
       if constexpr (c128) {
-        if (tall)
-          hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st,
-                             g, nb);
-        else
-          hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 4>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st,
-                             g, nb);
+        if (tall) {
+          if (fused)
+            hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8, true>), gd, dim3(256), 0, stream, st, g, nb);
+          else
+            hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 8>), gd, dim3(256), 0, stream, st, g, nb);
+        } else {
+          if (fused)
+            hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 4, true>), gd, dim3(256), 0, stream, st, g, nb);
+          else
+            hipLaunchKernelGGL((blk_gemm_kernel<S, 16, 4>), gd, dim3(256), 0, stream, st, g, nb);
+        }
       } else {
-        hipLaunchKernelGGL((blk_gemm_kernel<S, 32, 4>), dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
-                           nb);
+        if (fused)
+          hipLaunchKernelGGL((blk_gemm_kernel<S, 32, 4, true>), gd, dim3(256), 0, stream, st, g, nb);
+        else
+          hipLaunchKernelGGL((blk_gemm_kernel<S, 32, 4>), gd, dim3(256), 0, stream, st, g, nb);
       }
     } else if constexpr (std::is_same<S, float2>::value) {
-      if (mode == BLK_GEMM_BF16X3)
-        hipLaunchKernelGGL(blk_gemm_bf16x3_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
-                           nb);
-      else
-        hipLaunchKernelGGL(blk_gemm_mfma_kernel<32>, dim3(static_cast<unsigned>(grid)), dim3(256), 0, stream, st, g,
-                           nb);
+      if (mode == BLK_GEMM_BF16X3) {
+        if (fused)
+          hipLaunchKernelGGL((blk_gemm_bf16x3_kernel<32, true>), gd, dim3(256), 0, stream, st, g, nb);
+        else
+          hipLaunchKernelGGL((blk_gemm_bf16x3_kernel<32>), gd, dim3(256), 0, stream, st, g, nb);
+      } else {
+        if (fused)
+          hipLaunchKernelGGL((blk_gemm_mfma_kernel<32, true>), gd, dim3(256), 0, stream, st, g, nb);
+        else
+          hipLaunchKernelGGL((blk_gemm_mfma_kernel<32>), gd, dim3(256), 0, stream, st, g, nb);
+      }
     }
     DGAMI_HIP_CHECK(hipGetLastError());
     if (prof) {
@@ -857,8 +1051,28 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
   static_assert(NPAN >= 1 && NPAN <= BLK_PANELS, "panels per outer block");
   const int n = st.n, ncol = st.ncol;
   DGAMI_ARG_CHECK(nb <= 65535, "ladder: batch too large");
-  hipLaunchKernelGGL(blk_assemble_kernel<S>, dim3(ceil_div(n, ASM_ROWS), ceil_div(nb, ASM_QG)), dim3(256), 0, stream, a,
-                     st, item0, nb);
+  // the first outer block takes the assembly of its trailing rows (when there are trailing columns)
+  // complex64 fuses only with a shared vertex and no V_r, whose elements it reads from a complex64
+  // copy (the complex128 vertex would double the bytes of the C tile it replaces)
+  constexpr bool c64 = std::is_same<S, float2>::value;
+  const bool fuse = ladder_fuse_enabled() && n > NPAN * NB && (!c64 || (!a.dense && a.v_r == nullptr));
+  st.fuse_e = fuse ? NPAN * NB : 0;
+  st.item0 = item0;
+  st.src = a;
+  st.g32 = nullptr;
+  if (fuse && c64) {
+    const int g0 = item0 / a.nq, g1 = (item0 + nb - 1) / a.nq;
+    float2* g32 = static_cast<float2*>(
+        workspace(WS_LADDER_G32, static_cast<size_t>(g1 - g0 + 1) * n * n * sizeof(float2)));
+    if (!g32) return -2;
+    hipLaunchKernelGGL(blk_vertex32_kernel, dim3(8192), dim3(256), 0, stream, a, g32, g0, g1 - g0 + 1);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    st.g32 = g32;
+    st.g32_first = g0;
+  }
+  // the matrices are written once and read later from HBM: streaming (non-temporal) stores
+  hipLaunchKernelGGL((blk_assemble_kernel<S, true>), dim3(ceil_div(n, ASM_ROWS), ceil_div(nb, ASM_QG)), dim3(256), 0,
+                     stream, a, st, item0, nb);
   DGAMI_HIP_CHECK(hipGetLastError());
   BlockedSteps<S, NB, RPT, MAXTH> k{a, st, nb, stream};
   const int BOTH = ROWS_SWAP | ROWS_TRSM;
@@ -896,7 +1110,7 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
         DGAMI_STEP(k.rows(Pj, j, E, ncol, 0, ROWS_TRSM));
       }
     }
-    DGAMI_STEP(k.gemm(E, n, E, ncol, P, E - P));
+    DGAMI_STEP(k.gemm(E, n, E, ncol, P, E - P, P == 0 && st.fuse_e > 0));
   }
 #undef DGAMI_STEP
   return 0;

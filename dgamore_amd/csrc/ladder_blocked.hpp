@@ -26,6 +26,18 @@ struct BlockedState {
   int gemm_mode;   // BlockedGemm (complex64 only)
   int variant;     // blocked-LU variant (panel width and rows per thread), -1: chosen by n
   int tall_tiles;  // complex128: 128-row update tiles for updates of more than 96 rows (0: 64-row tiles)
+  // First outer block fused with the assembly (fuse_e > 0): only the block column [0, fuse_e) and
+  // the block row [0, fuse_e) are assembled; every other element is formed from the source
+  // matrix where it is first read (the interchanges of the block's panels and its trailing update).
+  int fuse_e;       // width of the fused first block (0: the whole matrix is assembled)
+  int* rowflag;     // [batch][n]: 0 while row r >= fuse_e of the trailing columns is the source row r
+                    // (not in W), else 1 + the panel slot whose interchanges first wrote it
+  int item0;        // first item of the batch
+  LadderArgs src;   // the items' matrices
+  // complex64 with a shared vertex and no V_r: (gamma - u_r) / beta^2 of the batch's (chan, w)
+  // groups in complex64 ([group - g32_first][n][n]), the fused update's source off the v = v' diagonal
+  const float2* g32;
+  int g32_first;
 };
 
 // Allocate the workspace of `batch` problems in slots ws and ws+1; returns false on failure.
@@ -33,13 +45,19 @@ template <typename S>
 inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t batch, int ws) {
   st.W = static_cast<S*>(workspace(ws, batch * n * ld * sizeof(S)));
   const size_t aux = batch * n * sizeof(double2) + batch * n * sizeof(int) + batch * BLK_PANELS * BLK_MAP_STRIDE * sizeof(int) +
-                     batch * sizeof(int) + 1024;
+                     batch * sizeof(int) + batch * n * sizeof(int) + 1024;
   char* p = static_cast<char*>(workspace(ws + 1, aux));
   if (!st.W || !p) return false;
   st.dinv = reinterpret_cast<double2*>(p);
   st.piv = reinterpret_cast<int*>(st.dinv + batch * n);
   st.rowmap = st.piv + batch * n;
   st.info = st.rowmap + batch * BLK_PANELS * BLK_MAP_STRIDE;
+  st.rowflag = st.info + batch;
+  st.fuse_e = 0;
+  st.item0 = 0;
+  st.src = LadderArgs{};
+  st.g32 = nullptr;
+  st.g32_first = 0;
   st.n = n;
   st.ld = ld;
   st.ncol = ncol;

@@ -74,6 +74,31 @@ __device__ __forceinline__ const double2* ladder_matrix_ptr(const LadderArgs& a,
   return a.gamma + (static_cast<size_t>(chan) * a.nw + w) * nn;
 }
 
+// Operands of one item for its augmented matrix [A | B]: the matrix (shared vertex or dense),
+// U_r and V_r(q) (non-dense only), and its (w, q).
+struct LadderSrc {
+  const double2* G;
+  const double2* U;
+  const double2* V;
+  int w, q;
+};
+
+__device__ __forceinline__ LadderSrc ladder_src(const LadderArgs& a, int item) {
+  const int q = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
+  LadderSrc s;
+  s.G = ladder_matrix_ptr(a, chan, w, q);
+  s.U = a.dense ? nullptr : a.u_r + chan * a.T * a.T;
+  s.V = (a.v_r && !a.dense) ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * a.T * a.T : nullptr;
+  s.w = w;
+  s.q = q;
+  return s;
+}
+
+// Element (i, j) of the augmented matrix [A | B] of an item, j < n + (right-hand sides).
+__device__ __forceinline__ double2 ladder_aug_at(const LadderArgs& a, const LadderSrc& s, int i, int j) {
+  return j < a.n ? ladder_elem(a, s.G, s.U, s.V, s.w, s.q, i, j) : ladder_rhs_at(a, s.w, s.q, i, j - a.n);
+}
+
 // Up to LADDER_MULTI_MAX models solved by one launch of the register-resident kernel.
 constexpr int LADDER_MULTI_MAX = 16;
 struct LadderMulti {
