@@ -39,10 +39,26 @@ __global__ void __launch_bounds__(256) bubble_kernel(const double2* __restrict__
   const double2* gl = g_r + (static_cast<size_t>(a * nb + d) * nvg + (niw + v)) * N;
   const double2* gr = g_r + (static_cast<size_t>(c * nb + b) * nvg + (niw + v - w)) * N;
   const int nx = plan.n[0], ny = plan.n[1], nz = plan.n[2];
-  for (int r = threadIdx.x; r < N; r += blockDim.x) {
-    const int z = r % nz, y = (r / nz) % ny, x = r / (nz * ny);
-    const int rm = (((nx - x) % nx) * ny + (ny - y) % ny) * nz + (nz - z) % nz;  // index of -R
-    buf[r] = cmul(gl[r], gr[rm]);
+  // (x, y, z) of r = threadIdx.x + m blockDim.x, advanced by blockDim.x with carries instead of
+  // divided out for every element
+  const int st = blockDim.x;
+  const int dz = st % nz, dy = (st / nz) % ny, dx = st / (nz * ny);
+  int z = threadIdx.x % nz, y = (threadIdx.x / nz) % ny, x = threadIdx.x / (nz * ny);
+  for (int r = threadIdx.x; r < N; r += st) {
+    const int xm = x == 0 ? 0 : nx - x, ym = y == 0 ? 0 : ny - y, zm = z == 0 ? 0 : nz - z;
+    buf[r] = cmul(gl[r], gr[(xm * ny + ym) * nz + zm]);  // G_cb at -R
+    z += dz;
+    int cy = dy, cx = dx;
+    if (z >= nz) {
+      z -= nz;
+      ++cy;
+    }
+    y += cy;
+    if (y >= ny) {
+      y -= ny;
+      ++cx;
+    }
+    x += cx;
   }
   __syncthreads();
   fft_3d_lds(buf, plan, tw, 1.0);

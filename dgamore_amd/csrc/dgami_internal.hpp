@@ -271,20 +271,24 @@ __device__ __forceinline__ void fft_stage(double2* buf, int L, int nfft, int nf2
   constexpr int MB = fft_max_bfly(R);
   const int nbf = L / R;  // butterflies per transform
   const int total = nbf * nfft;
+  const int step = L / (Ns * R);
   double2 v[MB][R];
+  int boff[MB], dst[MB];  // each butterfly's transform base and output index, kept for the write
 #pragma unroll
   for (int c = 0; c < MB; ++c) {
     const int b = threadIdx.x + c * blockDim.x;
     if (b < total) {
       const int f = b / nbf, j = b - f * nbf;
-      const int k = j % Ns;
-      const double2* base = buf + (f / nf2) * s1 + (f % nf2) * s2;
+      const int jq = j / Ns, k = j - jq * Ns;
+      const int fq = f / nf2;
+      boff[c] = fq * s1 + (f - fq * nf2) * s2;
+      dst[c] = jq * Ns * R + k;
+      const double2* base = buf + boff[c];
 #pragma unroll
       for (int r = 0; r < R; ++r) v[c][r] = base[(j + r * nbf) * xstride];
       if (Ns > 1) {
-        const int step = L / (Ns * R);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[c][r] = cmul(v[c][r], tw[(k * r * step) % L]);
+        for (int r = 1; r < R; ++r) v[c][r] = cmul(v[c][r], tw[k * r * step]);  // k r step < L
       }
       dft_small<R>(v[c], sign);
     }
@@ -294,12 +298,9 @@ __device__ __forceinline__ void fft_stage(double2* buf, int L, int nfft, int nf2
   for (int c = 0; c < MB; ++c) {
     const int b = threadIdx.x + c * blockDim.x;
     if (b < total) {
-      const int f = b / nbf, j = b - f * nbf;
-      const int k = j % Ns;
-      const int d = (j / Ns) * Ns * R + k;
-      double2* base = buf + (f / nf2) * s1 + (f % nf2) * s2;
+      double2* base = buf + boff[c];
 #pragma unroll
-      for (int r = 0; r < R; ++r) base[(d + r * Ns) * xstride] = v[c][r];
+      for (int r = 0; r < R; ++r) base[(dst[c] + r * Ns) * xstride] = v[c][r];
     }
   }
   __syncthreads();
