@@ -333,6 +333,45 @@ def cpu_baseline(model, target_s):
             "per_q_chain_s": t_q, "sigma_sum_s": t_sigma}
 
 
+def lambda_error(prec, name):
+    """The lambda correction (spch) of the single-band workload (64x64, niw = niv = 60, n = 120),
+    where it is defined, in the plain form and in the mixed form `prec`: both channels' lambda and
+    the corrected Sigma of one step, and their relative differences. The local susceptibility
+    sums are set below the uncorrected lattice sums (0.9 and 0.8 of them), so that both sum rules
+    have a root."""
+    import numpy as np
+    import torch
+
+    from dgamore_amd import _lib
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import SDEPipeline
+
+    m = S.make_model(1)
+    p0 = SDEPipeline(m)
+    p0.step(keep=True)
+    torch.cuda.synchronize()
+    counts, N = p0.grid.irrk_count, p0.N
+
+    def lattice_sum(x):
+        half = x[:, 0, :].T.cpu().numpy()
+        full = np.concatenate([np.conj(half[:, :0:-1]), half], axis=1)
+        return (counts[:, None] * full).sum() / N
+
+    sums = {"dens": 0.9 * lattice_sum(p0.last["chi_phys"][0]), "magn": 0.8 * lattice_sum(p0.last["chi_phys"][1])}
+    out = {}
+    for tag, pr in (("plain", _lib.PLAIN_FP64), (name, prec)):
+        p = SDEPipeline(m, pr, lambda_correction="spch", chi_loc_sums=sums)
+        p.step(keep=True)
+        torch.cuda.synchronize()
+        out[tag] = (dict(p.lambdas), p.last["sigma_nl"].clone())
+    (lp, sp), (lm, sm) = out["plain"], out[name]
+    return {"workload": f"{m.name}: 1 orbital, k-grid {m.nk[0]}x{m.nk[1]}, niw={m.niw}, niv={m.niv}, n={2 * m.niv}",
+            "lambda_plain": {k: float(np.real(v)) for k, v in lp.items()},
+            f"lambda_{name}": {k: float(np.real(v)) for k, v in lm.items()},
+            "lambda_rel_err": max(abs(lm[k] - lp[k]) / abs(lp[k]) for k in lp),
+            "sigma_rel_err": rel_err(sm, sp)}
+
+
 def time_steps(pipe, steps, warmup, stream):
     import torch
 
@@ -520,6 +559,11 @@ def main():
             },
         }
 
+    lam_check = None
+    if not args.no_compare and prec != _lib.PLAIN_FP64:
+        lam_check = lambda_error(prec, args.precision)
+        print(f"[bench] lambda check: {lam_check}", file=sys.stderr, flush=True)
+
     cpu = None if args.no_cpu_baseline else cpu_baseline(model, args.cpu_seconds)
     line = {
         "metric": METRIC,
@@ -552,6 +596,7 @@ def main():
             "ratio": roof["traffic"] * roof["launches"] / ladder_algorithmic_bytes(pipe)},
         "singular_solves": info_bad,
         "compare_fp64": compare,
+        "lambda_check": lam_check,
         "roofline": roof,
         "cpu_baseline": cpu,
     }
