@@ -137,7 +137,8 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
               if (vi == vj[h]) val[h] = cadd(val[h], Ck[k][(static_cast<size_t>(ri * T + cj[h]) * nv + vi) * a.nq]);
             }
           } else if (j < ncol) {
-            val[h] = ladder_rhs_at(a, wk[k], qk[k], i, j - n);
+            val[h] = a.rhs_mode == LADDER_RHS_VERTEX ? ladder_vertex_part(a, Gk[k], Uk[k], Vk[k], i, j - n)
+                                                     : ladder_rhs_at(a, wk[k], qk[k], i, j - n);
           }
         }
         store_pair<S, NT>(st.W + static_cast<size_t>(b0 + k) * n * st.ld + static_cast<size_t>(i) * st.ld + jb, val[0],
@@ -1145,6 +1146,13 @@ int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0,
                       hipStream_t stream) {
   const LadderArgs unused{};
   BlockedSteps<double2, 32, 1, 768> k{unused, st, nb, stream};
+  return k.gemm(r0, r1, c0, c1, k0, kw);
+}
+
+int blocked_gemm_c64(BlockedState<float2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,
+                     hipStream_t stream) {
+  const LadderArgs unused{};
+  BlockedSteps<float2, 32, 3, 384> k{unused, st, nb, stream};
   return k.gemm(r0, r1, c0, c1, k0, kw);
 }
 

@@ -76,6 +76,10 @@ int blocked_factor(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, 
 int blocked_gemm_fp64(BlockedState<double2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,
                       hipStream_t stream);
 
+// The same update on complex64 problems with the state's matrix-core kernel (st.gemm_mode).
+int blocked_gemm_c64(BlockedState<float2>& st, int nb, int r0, int r1, int c0, int c1, int k0, int kw,
+                     hipStream_t stream);
+
 // info[item0 + b] = first zero pivot of problem b of the batch (no-op for info == nullptr).
 int blocked_copy_info(const int* st_info, int32_t* info, int item0, int nitems, hipStream_t stream);
 

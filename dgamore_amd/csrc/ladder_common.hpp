@@ -23,8 +23,20 @@ struct LadderArgs {
 };
 
 // Right-hand sides of the blocked LU: the nb^2 indicator columns of the summed ladder, the
-// identity (full inverse) or the block-diagonal chi0^-1 (X chi0^-1 for the full vertex).
-enum LadderRhs { LADDER_RHS_INDICATOR = 0, LADDER_RHS_IDENTITY = 1, LADDER_RHS_CHI0INV = 2 };
+// identity (full inverse), the block-diagonal chi0^-1 (X chi0^-1 for the full vertex) or the
+// vertex part A - chi0^-1 = (gamma - u_r - v_r)/beta^2 (X (A - chi0^-1) for the full vertex
+// without the cancellation of chi0^-1 - chi0^-1 X chi0^-1).
+enum LadderRhs { LADDER_RHS_INDICATOR = 0, LADDER_RHS_IDENTITY = 1, LADDER_RHS_CHI0INV = 2, LADDER_RHS_VERTEX = 3 };
+
+// (gamma - u_r - v_r)[i][j] / beta^2 in compound indices (non-dense items).
+__device__ __forceinline__ double2 ladder_vertex_part(const LadderArgs& a, const double2* __restrict__ G,
+                                                      const double2* __restrict__ U, const double2* __restrict__ V,
+                                                      int i, int j) {
+  const int T = a.T, ri = i / a.nv, cj = j / a.nv;
+  double2 uv = U[ri * T + cj];
+  if (V) uv = cadd(uv, V[ri * T + cj]);
+  return cscale(csub(G[static_cast<size_t>(i) * a.n + j], uv), a.inv_beta2);
+}
 
 // Element (i, j) of the ladder matrix A(chan, w, q) in compound indices
 // (nonlocal_sde.py:93-95): chi0_inv delta_vv' + (gamma - u_r - v_r)/beta^2.
@@ -35,9 +47,7 @@ __device__ __forceinline__ double2 ladder_elem(const LadderArgs& a, const double
   const int nv = a.nv, T = a.T;
   const int ri = i / nv, vi = i - ri * nv;
   const int cj = j / nv, vj = j - cj * nv;
-  double2 uv = U[ri * T + cj];
-  if (V) uv = cadd(uv, V[ri * T + cj]);
-  double2 val = cscale(csub(G[static_cast<size_t>(i) * a.n + j], uv), a.inv_beta2);
+  double2 val = ladder_vertex_part(a, G, U, V, i, j);
   if (vi == vj) val = cadd(val, a.chi0_inv[((static_cast<size_t>(w) * T * T + ri * T + cj) * nv + vi) * a.nq + q]);
   return val;
 }
@@ -96,7 +106,9 @@ __device__ __forceinline__ LadderSrc ladder_src(const LadderArgs& a, int item) {
 
 // Element (i, j) of the augmented matrix [A | B] of an item, j < n + (right-hand sides).
 __device__ __forceinline__ double2 ladder_aug_at(const LadderArgs& a, const LadderSrc& s, int i, int j) {
-  return j < a.n ? ladder_elem(a, s.G, s.U, s.V, s.w, s.q, i, j) : ladder_rhs_at(a, s.w, s.q, i, j - a.n);
+  if (j < a.n) return ladder_elem(a, s.G, s.U, s.V, s.w, s.q, i, j);
+  if (a.rhs_mode == LADDER_RHS_VERTEX) return ladder_vertex_part(a, s.G, s.U, s.V, i, j - a.n);
+  return ladder_rhs_at(a, s.w, s.q, i, j - a.n);
 }
 
 // Up to LADDER_MULTI_MAX models solved by one launch of the register-resident kernel.

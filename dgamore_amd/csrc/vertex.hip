@@ -16,23 +16,26 @@
 #include "ladder_blocked.hpp"
 #include "tri_solve.hpp"
 
+#include <type_traits>
+
 namespace dgami {
 
 constexpr int VTX_STEP = 2 * TRI_B;  // rows per back-substitution step
 
 // s_y[h*TRI_B + r][lane] = (D_h s_y_h)[r][lane] for the 32 rows of half h; D_h = inverse of the
 // diagonal block of U (row-major TRI_B x TRI_B). Thread (grp, lane) forms rows grp*8 .. grp*8+7.
-__device__ __forceinline__ void vtx_apply_inverse(double2 (&s_y)[VTX_STEP][65], double2 (&s_m)[TRI_B][TRI_B + 1],
-                                                  const double2* __restrict__ D, int h, int grp, int lane) {
+template <typename S>
+__device__ __forceinline__ void vtx_apply_inverse(S (&s_y)[VTX_STEP][65], S (&s_m)[TRI_B][TRI_B + 1],
+                                                  const S* __restrict__ D, int h, int grp, int lane) {
   __syncthreads();
   for (int e = threadIdx.x; e < TRI_B * TRI_B; e += 256) s_m[e / TRI_B][e % TRI_B] = D[e];
   __syncthreads();
-  double2 acc[8];
+  S acc[8];
 #pragma unroll
-  for (int i = 0; i < 8; ++i) acc[i] = make_double2(0.0, 0.0);
+  for (int i = 0; i < 8; ++i) acc[i] = czero<S>();
 #pragma unroll 4
   for (int k = 0; k < TRI_B; ++k) {
-    const double2 y = s_y[h * TRI_B + k][lane];
+    const S y = s_y[h * TRI_B + k][lane];
 #pragma unroll
     for (int i = 0; i < 8; ++i) acc[i] = cfma(acc[i], s_m[grp * 8 + i][k], y);
   }
@@ -46,21 +49,22 @@ __device__ __forceinline__ void vtx_apply_inverse(double2 (&s_y)[VTX_STEP][65], 
 // from these rows, so with the halves 0 = [c0, c0+32) and 1 = [c0+32, c0+cb)
 //   X_1 = U_11^-1 Y_1,  Y_0 -= U_01 X_1,  X_0 = U_00^-1 Y_0.
 // grid: xcd_grid(nprob, ntile) workgroups of 256 threads, one 64-column tile each.
-__global__ void __launch_bounds__(256) vtx_backsolve_kernel(BlockedState<double2> st, const double2* __restrict__ dinvblk,
+template <typename S>
+__global__ void __launch_bounds__(256) vtx_backsolve_kernel(BlockedState<S> st, const S* __restrict__ dinvblk,
                                                             int nblk, int c0, int cb, int nr, int ntile, int nprob) {
   int b, tile;
   if (!xcd_tile(blockIdx.x, nprob, ntile, b, tile)) return;
   const int n = st.n, ld = st.ld;
-  double2* W = st.W + static_cast<size_t>(b) * n * ld;
+  S* W = st.W + static_cast<size_t>(b) * n * ld;
   const int cc = n + tile * 64;
-  __shared__ double2 s_y[VTX_STEP][65];
-  __shared__ double2 s_m[TRI_B][TRI_B + 1];
+  __shared__ S s_y[VTX_STEP][65];
+  __shared__ S s_m[TRI_B][TRI_B + 1];
   const int lane = threadIdx.x & 63, grp = threadIdx.x >> 6;
   const bool colok = tile * 64 + lane < nr;
   for (int r = grp; r < VTX_STEP; r += 4)
-    s_y[r][lane] = (r < cb && colok) ? W[static_cast<size_t>(c0 + r) * ld + cc + lane] : make_double2(0.0, 0.0);
+    s_y[r][lane] = (r < cb && colok) ? W[static_cast<size_t>(c0 + r) * ld + cc + lane] : czero<S>();
   // inverse diagonal blocks of U: [nblk][2][TRI_B][TRI_B], U at offset TRI_B*TRI_B
-  const double2* Dinv = dinvblk + static_cast<size_t>(b) * nblk * 2 * TRI_B * TRI_B + TRI_B * TRI_B;
+  const S* Dinv = dinvblk + static_cast<size_t>(b) * nblk * 2 * TRI_B * TRI_B + TRI_B * TRI_B;
   const int kb0 = c0 / TRI_B;
   if (cb > TRI_B) {
     vtx_apply_inverse(s_y, s_m, Dinv + static_cast<size_t>(kb0 + 1) * 2 * TRI_B * TRI_B, 1, grp, lane);
@@ -68,15 +72,15 @@ __global__ void __launch_bounds__(256) vtx_backsolve_kernel(BlockedState<double2
     const int kw = cb - TRI_B;
     for (int e = threadIdx.x; e < TRI_B * TRI_B; e += 256) {
       const int r = e / TRI_B, k = e % TRI_B;
-      s_m[r][k] = k < kw ? W[static_cast<size_t>(c0 + r) * ld + c0 + TRI_B + k] : make_double2(0.0, 0.0);
+      s_m[r][k] = k < kw ? W[static_cast<size_t>(c0 + r) * ld + c0 + TRI_B + k] : czero<S>();
     }
     __syncthreads();
-    double2 acc[8];
+    S acc[8];
 #pragma unroll
     for (int i = 0; i < 8; ++i) acc[i] = s_y[grp * 8 + i][lane];
 #pragma unroll 4
     for (int k = 0; k < TRI_B; ++k) {
-      const double2 x = s_y[TRI_B + k][lane];
+      const S x = s_y[TRI_B + k][lane];
 #pragma unroll
       for (int i = 0; i < 8; ++i) acc[i] = cfms(acc[i], s_m[grp * 8 + i][k], x);
     }
@@ -92,13 +96,15 @@ __global__ void __launch_bounds__(256) vtx_backsolve_kernel(BlockedState<double2
 
 // out[item][i][j] = X[i][j] (the right-hand-side columns after the back substitution).
 // grid (ceil(n/4), nprob), 256 threads.
-__global__ void __launch_bounds__(256) vtx_copy_kernel(BlockedState<double2> st, double2* __restrict__ out, int item0) {
+template <typename S>
+__global__ void __launch_bounds__(256) vtx_copy_kernel(BlockedState<S> st, double2* __restrict__ out, int item0) {
   const int b = blockIdx.y, n = st.n;
-  const double2* W = st.W + static_cast<size_t>(b) * n * st.ld + n;
+  const S* W = st.W + static_cast<size_t>(b) * n * st.ld + n;
   double2* O = out + static_cast<size_t>(item0 + b) * n * n;
   const int r1 = min(n, (blockIdx.x + 1) * 4);
   for (int i = blockIdx.x * 4; i < r1; ++i)
-    for (int j = threadIdx.x; j < n; j += 256) O[static_cast<size_t>(i) * n + j] = W[static_cast<size_t>(i) * st.ld + j];
+    for (int j = threadIdx.x; j < n; j += 256)
+      O[static_cast<size_t>(i) * n + j] = to_c128(W[static_cast<size_t>(i) * st.ld + j]);
 }
 
 struct VertexArgs {
@@ -106,15 +112,18 @@ struct VertexArgs {
   const double2* chi_phys;  // QW [nw][T*T][nq]
   double2* out;             // [nw][nq][n][n]
   double beta2;
+  int z_form;               // 1: the right-hand sides hold Z = X (A - chi0^-1), 0: Y = X chi0^-1
 };
 
 // Full vertex of one problem, rows (ri, v) for every ri < T at one v, every column j = (cj, v'):
 //   F = beta^2 (D(v) delta_vv' - sum_s D(v)[ri][s] Y[(s, v)][j]) + sum_s P(v)[ri][s] V(v')[s][cj]
 // with Y = X chi0^-1 (right-hand-side columns), D(v) = chi0^-1(q,w,v), V(v) = vrg(q,w,v) and
-// P(v) = (u - u chi_r u) V(v), all nb^2 x nb^2 in M order. grid (nv, nprob), 256 threads.
-template <int T>
-__global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedState<double2> st, VertexArgs o,
-                                                       int item0) {
+// P(v) = (u - u chi_r u) V(v), all nb^2 x nb^2 in M order. With z_form the columns hold
+// Z = X (A - chi0^-1) and the first term is beta^2 sum_s D(v)[ri][s] Z[(s, v)][j], the same
+// value (chi0^-1 - chi0^-1 X chi0^-1 = chi0^-1 X (A - chi0^-1)) without the cancellation of two
+// terms that grow as v^2. grid (nv, nprob), 256 threads.
+template <int T, typename S>
+__global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedState<S> st, VertexArgs o, int item0) {
   constexpr int TT = T * T;
   const int v = blockIdx.x, b = blockIdx.y;
   const int item = item0 + b;
@@ -160,13 +169,13 @@ __global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedStat
     }
     __syncthreads();
   }
-  const double2* Y = st.W + static_cast<size_t>(b) * n * st.ld + n;
+  const S* Y = st.W + static_cast<size_t>(b) * n * st.ld + n;
   double2* O = o.out + static_cast<size_t>(item) * n * n;
   for (int j = tid; j < n; j += 256) {
     const int cj = j / nv, vj = j - cj * nv;
     double2 y[T], vc[T];
 #pragma unroll
-    for (int s = 0; s < T; ++s) y[s] = Y[static_cast<size_t>(s * nv + v) * st.ld + j];
+    for (int s = 0; s < T; ++s) y[s] = to_c128(Y[static_cast<size_t>(s * nv + v) * st.ld + j]);
     if (f2) {
 #pragma unroll
       for (int s = 0; s < T; ++s) vc[s] = o.vrg[((static_cast<size_t>(w) * TT + s * T + cj) * nv + vj) * nq + q];
@@ -176,7 +185,7 @@ __global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedStat
       double2 dy = make_double2(0.0, 0.0);
 #pragma unroll
       for (int s = 0; s < T; ++s) dy = cfma(dy, s_d[ri][s], y[s]);
-      double2 f = csub(vj == v ? s_d[ri][cj] : make_double2(0.0, 0.0), dy);
+      double2 f = o.z_form ? dy : csub(vj == v ? s_d[ri][cj] : make_double2(0.0, 0.0), dy);
       f = cscale(f, o.beta2);
       if (f2) {
 #pragma unroll
@@ -234,44 +243,56 @@ __global__ void __launch_bounds__(256) vtx_pp_kernel(const double2* __restrict__
 // ---------------------------------------------------------------------------
 enum VertexOut { VTX_OUT_INVERSE = 0, VTX_OUT_FULL = 1 };
 
-static int vertex_solve(LadderArgs a, int what, const VertexArgs& o, hipStream_t stream) {
+template <typename S>
+static int vertex_solve_t(LadderArgs a, int what, const VertexArgs& o_in, int gemm_mode, hipStream_t stream) {
   const int n = a.n, nr = n, ncol = 2 * n;
   const int ld = (ncol + 3) & ~3;
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "vertex: compound size above 1536 is not supported");
-  a.rhs_mode = what == VTX_OUT_FULL ? LADDER_RHS_CHI0INV : LADDER_RHS_IDENTITY;
+  // complex64 factorisations carry the vertex part as right-hand sides (no cancellation in F)
+  constexpr bool c64 = std::is_same<S, float2>::value;
+  a.rhs_mode = what == VTX_OUT_FULL ? (c64 ? LADDER_RHS_VERTEX : LADDER_RHS_CHI0INV) : LADDER_RHS_IDENTITY;
+  VertexArgs o = o_in;
+  o.z_form = c64 && what == VTX_OUT_FULL;
   const size_t items = static_cast<size_t>(a.nw) * a.nq;
-  const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);
+  const size_t per = static_cast<size_t>(n) * ld * sizeof(S);
   // up to 32 GiB of problems (factors and n right-hand sides) per pass
   size_t batch = (size_t(32) << 30) / per;
   if (batch < 1) batch = 1;
   if (batch > items) batch = items;
   if (batch > 65535) batch = 65535;
-  BlockedState<double2> st;
+  BlockedState<S> st;
   if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
+  if constexpr (std::is_same<S, float2>::value) st.gemm_mode = gemm_mode;
   const int nblk = static_cast<int>(ceil_div(n, TRI_B));
-  double2* tri = static_cast<double2*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(double2)));
+  S* tri = static_cast<S*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(S)));
   if (!tri) return -2;
   const int ntile = static_cast<int>(ceil_div(nr, 64));
   for (size_t item0 = 0; item0 < items; item0 += batch) {
     const int nb = static_cast<int>(items - item0 < batch ? items - item0 : batch);
-    int rc = blocked_factor<double2>(a, st, static_cast<int>(item0), nb, stream);
+    int rc = blocked_factor<S>(a, st, static_cast<int>(item0), nb, stream);
     if (rc) return rc;
-    TriFactors<double2> f{st.W, tri, st.dinv, n, ld, nblk};
+    TriFactors<S> f{st.W, tri, st.dinv, n, ld, nblk};
     rc = tri_diag_inverse(f, nb, false, stream);
     if (rc) return rc;
     for (int c0 = (n - 1) / VTX_STEP * VTX_STEP; c0 >= 0; c0 -= VTX_STEP) {
       const int cb = min(VTX_STEP, n - c0);
-      hipLaunchKernelGGL(vtx_backsolve_kernel, dim3(static_cast<unsigned>(xcd_grid(nb, ntile))), dim3(256), 0, stream,
-                         st, tri, nblk, c0, cb, nr, ntile, nb);
+      hipLaunchKernelGGL(vtx_backsolve_kernel<S>, dim3(static_cast<unsigned>(xcd_grid(nb, ntile))), dim3(256), 0,
+                         stream, st, tri, nblk, c0, cb, nr, ntile, nb);
       DGAMI_HIP_CHECK(hipGetLastError());
-      if (c0 > 0 && (rc = blocked_gemm_fp64(st, nb, 0, c0, n, n + nr, c0, cb, stream)) != 0) return rc;
+      if (c0 > 0) {
+        if constexpr (std::is_same<S, float2>::value)
+          rc = blocked_gemm_c64(st, nb, 0, c0, n, n + nr, c0, cb, stream);
+        else
+          rc = blocked_gemm_fp64(st, nb, 0, c0, n, n + nr, c0, cb, stream);
+        if (rc) return rc;
+      }
     }
     if (what == VTX_OUT_INVERSE) {
-      hipLaunchKernelGGL(vtx_copy_kernel, dim3(static_cast<unsigned>(ceil_div(n, 4)), nb), dim3(256), 0, stream, st,
+      hipLaunchKernelGGL(vtx_copy_kernel<S>, dim3(static_cast<unsigned>(ceil_div(n, 4)), nb), dim3(256), 0, stream, st,
                          o.out, static_cast<int>(item0));
     } else {
 #define DGAMI_VTX_FULL(TV)                                                                                       \
-  hipLaunchKernelGGL(vtx_full_kernel<TV>, dim3(a.nv, nb), dim3(256), 0, stream, a, st, o, static_cast<int>(item0))
+  hipLaunchKernelGGL((vtx_full_kernel<TV, S>), dim3(a.nv, nb), dim3(256), 0, stream, a, st, o, static_cast<int>(item0))
       switch (a.T) {
         case 1: DGAMI_VTX_FULL(1); break;
         case 4: DGAMI_VTX_FULL(4); break;
@@ -286,6 +307,14 @@ static int vertex_solve(LadderArgs a, int what, const VertexArgs& o, hipStream_t
     if (rc) return rc;
   }
   return 0;
+}
+
+// precision: DGAMI_PLAIN_FP64 (complex128 LU), or a complex64 LU with FP32 (DGAMI_MIXED_FP32_IR) or
+// split-BF16 (DGAMI_MIXED_BF16X3_IR) matrix-core updates and no FP64 refinement.
+static int vertex_solve(LadderArgs a, int what, const VertexArgs& o, hipStream_t stream, int precision = 0) {
+  if (precision == 0) return vertex_solve_t<double2>(a, what, o, 0, stream);
+  DGAMI_ARG_CHECK(precision == 1 || precision == 2, "vertex: precision must be 0, 1 or 2");
+  return vertex_solve_t<float2>(a, what, o, precision == 2 ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA, stream);
 }
 
 static LadderArgs vertex_ladder_args(int nq, int nb, int nw, int niv, double beta, const dgami_c128* chi0_inv,
@@ -319,7 +348,7 @@ extern "C" int dgami_ladder_invert(int nq, int nb, int nw, int niv, double beta,
   DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && out, "ladder_invert: null pointer");
   DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "ladder_invert: invalid sizes");
   const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
-  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(out), 0.0};
+  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(out), 0.0, 0};
   return vertex_solve(a, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
 }
 
@@ -334,7 +363,7 @@ extern "C" int dgami_invert_dense(int nmat, int n, const dgami_c128* a, dgami_c1
   la.nv = n;
   la.n = n;
   la.inv_beta2 = 1.0;
-  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(a_inv), 0.0};
+  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(a_inv), 0.0, 0};
   return vertex_solve(la, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
 }
 
@@ -348,8 +377,23 @@ extern "C" int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, c
   DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "full_vertex: invalid sizes");
   const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
   VertexArgs o{reinterpret_cast<const double2*>(vrg), reinterpret_cast<const double2*>(chi_phys),
-               reinterpret_cast<double2*>(out), beta * beta};
+               reinterpret_cast<double2*>(out), beta * beta, 0};
   return vertex_solve(a, VTX_OUT_FULL, o, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_full_vertex_prec(int nq, int nb, int nw, int niv, double beta, int precision,
+                                      const dgami_c128* chi0_inv, const dgami_c128* gamma, const dgami_c128* u_r,
+                                      const dgami_c128* v_r, const dgami_c128* vrg, const dgami_c128* chi_phys,
+                                      dgami_c128* out, int32_t* info, void* stream) {
+  using namespace dgami;
+  DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && out, "full_vertex: null pointer");
+  DGAMI_ARG_CHECK((vrg == nullptr) == (chi_phys == nullptr), "full_vertex: vrg and chi_phys go together");
+  DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "full_vertex: invalid sizes");
+  DGAMI_ARG_CHECK(precision >= 0 && precision <= 2, "full_vertex: precision must be 0, 1 or 2");
+  const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
+  VertexArgs o{reinterpret_cast<const double2*>(vrg), reinterpret_cast<const double2*>(chi_phys),
+               reinterpret_cast<double2*>(out), beta * beta, 0};
+  return vertex_solve(a, VTX_OUT_FULL, o, static_cast<hipStream_t>(stream), precision);
 }
 
 namespace dgami {

@@ -78,7 +78,8 @@ def create_pairing_vertices_pp(f_dens, f_magn, niv_pp):
             FourPoint(gt.cpu().numpy(), SpinChannel.TRIP, nq, 0, 2, False, True, True, FrequencyNotation.PP))
 
 
-def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta, nb, niv_pp, w_chunk=16):
+def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta, nb, niv_pp, w_chunk=16,
+                                 precision=0):
     """Singlet and triplet pairing vertices [nq, nb, nb, nb, nb, 2np, 2np] on the device from the
     ladder of both channels (the chain of solve, eliashberg_solver.py:584-629, with
     create_full_vertex_q_r, :78-147, and the w = 0 transform, :44-75).
@@ -88,7 +89,8 @@ def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta,
     u_rs: [2][nb^4]; v_rs: [2][nq][nb^4] or None; vrgs: [2] x QF; chis: [2] x QW. The particle-hole
     full vertices are formed for w_chunk bosonic frequencies at a time and transformed before the
     next range, so F of every w is never held at once; pp entries with |v - v'| >= nw stay zero,
-    as in the reference."""
+    as in the reference. precision 1 / 2 factors the ladder matrices in complex64 with FP32 /
+    split-BF16 matrix-core updates (dgami_full_vertex_prec)."""
     import torch
 
     nw, nb4, nv, nq = chi0_inv.shape
@@ -99,7 +101,7 @@ def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta,
         w1 = min(nw, w0 + w_chunk)
         fs = [K.full_vertex(chi0_inv[w0:w1].contiguous(), gammas[c, w0:w1].contiguous(), u_rs[c],
                             None if v_rs is None else v_rs[c], vrgs[c][w0:w1].contiguous(), chis[c][w0:w1].contiguous(),
-                            beta)[0] for c in range(2)]
+                            beta, precision=precision)[0] for c in range(2)]
         K.pairing_vertex_pp_range(fs[0], fs[1], nb, niv_pp, w0, gs, gt)
         del fs
     return gs, gt

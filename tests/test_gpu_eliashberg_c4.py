@@ -8,6 +8,8 @@ w = 0 particle-particle transform give the singlet pairing vertex; the linearise
 * the four leading eigenpairs from ARPACK's non-Hermitian Arnoldi have residuals
   |A x - lambda x| / |lambda| <= 1e-8;
 * the same operator with its vertex stored in complex64 gives the same eigenvalues to 1e-5;
+* the pairing vertex built from complex64 ladder factorisations (dgami_full_vertex_prec) gives
+  the same eigenvalues to 1e-6 with FP32 matrix-core updates and to 1e-5 with split-BF16 ones;
 * on an 8x8 grid (dimension 2048) the Arnoldi eigenvalues equal those of the dense matrix of the
   same device operator.
 The model's fermionic box is the pairing box itself (niv = np = 64) and its bosonic box covers
@@ -30,7 +32,8 @@ def say(msg):
     sys.stdout.flush()
 
 
-def _pairing_operator(nk, niw, niv, precisions=("fp64",)):
+def _pairing_operator(nk, niw, niv, precisions=("fp64",), vertex_precisions=(0,)):
+    """{(vertex precision, storage): PairingOperator} and {vertex precision: build seconds}."""
     from dgamore_amd import eliashberg_solver as es
     from dgamore_amd import kernels as K
     from dgamore_amd import synthetic as S
@@ -48,15 +51,24 @@ def _pairing_operator(nk, niw, niv, precisions=("fp64",)):
                                   m.beta)
         vrgs.append(vrg)
         chis.append(chi)
-    gs, _ = es.pairing_vertices_from_ladder(front["chi0_inv"], pipe.gamma, pipe.u_r, pipe.v_r, vrgs, chis, m.beta,
-                                            m.nb, niv)
     chi0pp = K.chi0_pp(front["g"], niv)
-    del front, aux, vrgs, chis
-    ops = {p: es.PairingOperator(gs, chi0pp, KGrid(nk), m.beta, "sing", p) for p in precisions}
     torch.cuda.synchronize()
-    say(f"  pairing operator(s) {list(ops)} for {nk}, niw={niw}, 2np={2 * niv} built in "
-        f"{time.perf_counter() - t0:.1f} s")
-    return ops
+    say(f"  ladder of both channels for {nk}, niw={niw}, 2np={2 * niv} in {time.perf_counter() - t0:.1f} s")
+    ops, secs = {}, {}
+    for vp in vertex_precisions:
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        gs, _ = es.pairing_vertices_from_ladder(front["chi0_inv"], pipe.gamma, pipe.u_r, pipe.v_r, vrgs, chis,
+                                                m.beta, m.nb, niv, precision=vp)
+        torch.cuda.synchronize()
+        secs[vp] = time.perf_counter() - t1
+        say(f"  pairing vertices (full vertex precision {vp}) in {secs[vp]:.2f} s")
+        for p in precisions:
+            ops[(vp, p)] = es.PairingOperator(gs, chi0pp, KGrid(nk), m.beta, "sing", p)
+        del gs
+    torch.cuda.synchronize()
+    say(f"  pairing operator(s) {list(ops)} built in {time.perf_counter() - t0:.1f} s")
+    return ops, secs
 
 
 def test_eliashberg_config4_leading_eigenpairs():
@@ -65,8 +77,8 @@ def test_eliashberg_config4_leading_eigenpairs():
     from dgamore_amd import eliashberg_solver as es
 
     say("Eliashberg config 4: 2 orbitals, 48x48, beta=20, niw=127, 2np=128, singlet, from the ladder")
-    ops = _pairing_operator((48, 48, 1), 127, 64, ("fp64", "fp32"))
-    op = ops["fp64"]
+    ops, secs = _pairing_operator((48, 48, 1), 127, 64, ("fp64", "fp32"), (0, 1, 2))
+    op = ops[(0, "fp64")]
     assert op.dim == 48 * 48 * 4 * 128
     v0 = np.random.default_rng(7).standard_normal(op.dim) + 0j
     t0 = time.perf_counter()
@@ -76,10 +88,17 @@ def test_eliashberg_config4_leading_eigenpairs():
         f"{t_arn:.1f} s")
     assert np.all(res < 1e-8), res
     assert all(np.isfinite(g).all() for g in gaps)
-    lams32, _, res32 = es.solve_eliashberg_arnoldi(ops["fp32"], 4, tol=1e-12, v0=v0)
+    lams32, _, res32 = es.solve_eliashberg_arnoldi(ops[(0, "fp32")], 4, tol=1e-12, v0=v0)
     err = np.abs(lams32 - lams).max() / np.abs(lams).max()
     say(f"  complex64 vertex: lambda {np.round(lams32, 8).tolist()}, max rel. error {err:.2e}")
     assert err < 1e-5
+    for vp, name, tol in ((1, "FP32", 1e-6), (2, "split-BF16", 1e-5)):
+        lams_m, _, res_m = es.solve_eliashberg_arnoldi(ops[(vp, "fp64")], 4, tol=1e-12, v0=v0)
+        err = np.abs(lams_m - lams).max() / np.abs(lams).max()
+        say(f"  {name} complex64 LU: lambda {np.round(lams_m, 8).tolist()}, max rel. error {err:.2e}, "
+            f"pairing vertices in {secs[vp]:.2f} s (complex128 LU: {secs[0]:.2f} s)")
+        assert np.all(res_m < 1e-8), res_m
+        assert err < tol
 
 
 def test_eliashberg_arnoldi_matches_dense_eigenvalues_8x8():
@@ -88,7 +107,7 @@ def test_eliashberg_arnoldi_matches_dense_eigenvalues_8x8():
     from dgamore_amd import eliashberg_solver as es
 
     say("Eliashberg 8x8, niw=7, 2np=8 (dimension 2048): Arnoldi vs dense eig of the device operator")
-    op = _pairing_operator((8, 8, 1), 7, 4)["fp64"]
+    op = _pairing_operator((8, 8, 1), 7, 4)[0][(0, "fp64")]
     assert op.dim == 2048
     cols = []
     for j0 in range(0, op.dim, 8):  # the matvec takes up to 8 vectors per application

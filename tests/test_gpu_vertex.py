@@ -83,6 +83,31 @@ def test_full_vertex(K, nb, niv, niw, nq, second_term):
     assert rel(f.permute(1, 0, 2, 3), ref) < RTOL
 
 
+@pytest.mark.parametrize("nb,niv,niw,nq", CASES + [(3, 40, 0, 1)])
+@pytest.mark.parametrize("precision", [1, 2])
+def test_full_vertex_complex64_lu(K, nb, niv, niw, nq, precision):
+    """The complex64 factorisation (no refinement). FP32 updates: F within 1e-5 of its largest
+    element, or, for the worse-conditioned random matrices, within the complex128 LU's error
+    scaled by the ratio of the two unit roundoffs (2^-24 / 2^-53), capped at 1e-4. Split-BF16
+    updates keep 16 significant bits of every factor (hi + lo, the lo*lo product dropped), a unit
+    roundoff of 2^-17: bounds 1e-4, 2^36 times the complex128 error, capped at 1e-3."""
+    say(f"full_vertex_prec nb={nb} niv={niv} niw={niw} nq={nq} precision={precision}")
+    chi0_inv, gamma, u, v, beta = ladder_case(nb, niv, niw, nq, seed=nb * 10 + niv + 2)
+    vrg, chi = _vrg_chi(nb, chi0_inv.shape, seed=nb + 5)
+    ref = O.full_vertex_q(chi0_inv, gamma, u, v, vrg, chi, beta)
+    args = _dev_inputs(K, chi0_inv, gamma, u, v) + (K.ref4_to_qf(K.to_dev(vrg)), K.ref4w_to_qw(K.to_dev(chi)), beta)
+    f, info = K.full_vertex(*args, precision=precision)
+    f64, _ = K.full_vertex(*args)
+    torch.cuda.synchronize()
+    assert int(info.abs().sum()) == 0
+    err, err64 = rel(f.permute(1, 0, 2, 3), ref), rel(f64.permute(1, 0, 2, 3), ref)
+    say(f"  rel. error {err:.2e} (complex128 LU: {err64:.2e})")
+    if precision == 1:
+        assert err < min(1e-4, max(1e-5, 2.0**29 * err64))
+    else:
+        assert err < min(1e-3, max(1e-4, 2.0**36 * err64))
+
+
 def test_full_vertex_3orb_box(K):
     nb, niv = 3, 40
     say(f"full_vertex 3-orbital box nb={nb} niv={niv} n=720 one (q, w)")
