@@ -131,7 +131,7 @@ def eliashberg_matvec(steps, warmup):
                       "rel_err_c64_vs_c128": err}), flush=True)
 
 
-def batch_graph(steps, warmup, seed, symmetric=True):
+def batch_graph(steps, warmup, seed, symmetric=True, precision="plain"):
     """16 seeded models of uneven sizes in one ModelBatch: autotune every ladder shape, time one eager
     step, capture the step of all models as one HIP graph, check one replay against the eager step,
     then time `steps` chained replays (whole self-consistency iterations with mixing 0.3: every
@@ -148,7 +148,10 @@ def batch_graph(steps, warmup, seed, symmetric=True):
         models.append(S.make_model(c, device=torch.device("cuda"), symmetric=symmetric))
         print(f"[bench] built {c['name']} (n={c['nb'] ** 2 * 2 * c['niv']}) {time.perf_counter() - t0:.1f}s",
               file=sys.stderr, flush=True)
-    mb = ModelBatch(models, mixing=0.3)
+    from dgamore_amd import _lib
+
+    prec = {"plain": _lib.PLAIN_FP64, "mixed": _lib.MIXED_FP32_IR, "bf16x3": _lib.MIXED_BF16X3_IR}[precision]
+    mb = ModelBatch(models, mixing=0.3, precision=prec)
     t1 = time.perf_counter()
     tuned = mb.tune(verbose=True)
     t_tune = time.perf_counter() - t1
@@ -178,6 +181,26 @@ def batch_graph(steps, warmup, seed, symmetric=True):
     ms_eager = e0.elapsed_time(e1)
     eager_mu = [float(p.mu_dev.item()) for p in mb.pipes]
     print(f"[bench] eager step of all models {ms_eager:.2f} ms", file=sys.stderr, flush=True)
+    compare = None
+    if prec != _lib.PLAIN_FP64:
+        # the same iteration with every ladder solve in complex128
+        mb.precision = _lib.PLAIN_FP64
+        restore()
+        torch.cuda.synchronize()
+        e0.record(stream)
+        plain = mb.step(scf=True)
+        e1.record(stream)
+        torch.cuda.synchronize()
+        ms_plain = e0.elapsed_time(e1)
+        plain_mu = [float(p.mu_dev.item()) for p in mb.pipes]
+        mb.precision = prec
+        restore()
+        torch.cuda.synchronize()
+        compare = {"eager_ms_per_step_plain": ms_plain, "eager_speedup_vs_plain": ms_plain / ms_eager,
+                   "sigma_rel_err_vs_plain": max(float((e - q).abs().max() / q.abs().max())
+                                                 for e, q in zip(eager, plain)),
+                   "mu_err_vs_plain": max(abs(a - b) for a, b in zip(eager_mu, plain_mu))}
+        print(f"[bench] plain eager step {ms_plain:.2f} ms; {compare}", file=sys.stderr, flush=True)
     mb.capture(chain=True, scf=True)
     start = [s.clone() for s in mb.graph_sigmas]
     state0 = [(p.mu_dev.clone(), p.occ.clone(), p.occ_k.clone()) for p in mb.pipes]
@@ -203,7 +226,8 @@ def batch_graph(steps, warmup, seed, symmetric=True):
     print(json.dumps({
         "metric": METRIC,
         "value": solves / (ms * 1e-3), "unit": "solves/s", "n_gpus": 1, "steps": steps, "warmup": warmup,
-        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "c128",
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "c128" if prec == _lib.PLAIN_FP64 else f"{precision} LU + c128 refinement (n > 64), c128 (n <= 64)",
         "data": "synthetic (16 seeded DMFT-like models, random-perturbation local vertices)",
         "config": {"workload": f"batch of {len(models)} models (seed {seed}): 1-3 orbitals, beta 5-40, niv=niw="
                                f"round(1.5 beta), k-grids 16^2-32^2, compound sizes {min(sizes)}..{max(sizes)}; "
@@ -218,6 +242,7 @@ def batch_graph(steps, warmup, seed, symmetric=True):
         "autotune": {f"n={t['n']},nrhs={t['nrhs']}": {"choice": t["name"], "source": t["source"],
                                                       "times_ms": t.get("times_ms", {})} for t in tuned},
         "autotune_seconds": t_tune,
+        "precision": precision, "compare_plain": compare,
     }), flush=True)
 
 
@@ -499,7 +524,8 @@ def main():
         eliashberg_matvec(args.steps, args.warmup)
         return
     if args.batch:
-        batch_graph(args.steps if "--steps" in sys.argv else 10, args.warmup, args.batch_seed, not args.no_symmetric)
+        batch_graph(args.steps if "--steps" in sys.argv else 10, args.warmup, args.batch_seed, not args.no_symmetric,
+                    args.precision)
         return
     import torch
 

@@ -65,6 +65,7 @@ class LadderModel(ctypes.Structure):
 
 
 SIGNATURES["dgami_ladder_solve_sum_multi"] = ([_i, _c, _c], _i)
+SIGNATURES["dgami_ladder_solve_sum_multi_prec"] = ([_i, _c, _i, _c], _i)
 SIGNATURES["dgami_kernel_orbital_expand"] = ([_i, _i, _i, _i, _i, _c, _c, _c, _c, _c, _c], _i)
 SIGNATURES["dgami_ladder_tune_candidates"] = ([_i, _i, _i, _c, _i], _i)
 SIGNATURES["dgami_ladder_tune_set"] = ([_i, _i, _i, _i], _i)

@@ -59,10 +59,15 @@ int ladder_solve_fp64(const LadderArgs& a, hipStream_t stream) {
 
 }  // namespace dgami
 
-extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_model* models, void* stream) {
-  using namespace dgami;
+namespace dgami {
+
+// Models of a batch: the register-resident ones share launches in FP64 (at n <= 64 a
+// complex64 factorisation would save nothing); the others take the blocked LU in `precision`.
+static int ladder_multi(int nmodels, const dgami_ladder_model* models, int precision, hipStream_t s) {
   DGAMI_ARG_CHECK(nmodels >= 0 && (nmodels == 0 || models), "ladder_multi: null model table");
-  hipStream_t s = static_cast<hipStream_t>(stream);
+  DGAMI_ARG_CHECK(precision == DGAMI_PLAIN_FP64 || precision == DGAMI_MIXED_FP32_IR ||
+                      precision == DGAMI_MIXED_BF16X3_IR,
+                  "ladder_multi: unknown precision mode");
   std::vector<LadderArgs> small;
   int rc = 0;
   for (int m = 0; m < nmodels; ++m) {
@@ -91,7 +96,7 @@ extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_mode
     DGAMI_ARG_CHECK(items < (size_t(1) << 30), "ladder_multi: too many problems in a model");
     if (fp64_reg_choice(a.n, a.T) >= 0)
       small.push_back(a);
-    else if ((rc = ladder_solve_fp64(a, s)) != 0)
+    else if ((rc = precision == DGAMI_PLAIN_FP64 ? ladder_solve_fp64(a, s) : ladder_solve_mixed(a, precision, s)) != 0)
       return rc;
   }
   // the register-resident models, LADDER_MULTI_MAX per launch, in the largest variant of the group
@@ -112,6 +117,17 @@ extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_mode
     if ((rc = ladder_reg_launch_multi(variant, mm, s)) != 0) return rc;
   }
   return 0;
+}
+
+}  // namespace dgami
+
+extern "C" int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_model* models, void* stream) {
+  return dgami::ladder_multi(nmodels, models, DGAMI_PLAIN_FP64, static_cast<hipStream_t>(stream));
+}
+
+extern "C" int dgami_ladder_solve_sum_multi_prec(int nmodels, const dgami_ladder_model* models, int precision,
+                                                 void* stream) {
+  return dgami::ladder_multi(nmodels, models, precision, static_cast<hipStream_t>(stream));
 }
 
 extern "C" int dgami_ladder_solve_sum(int nchan, int nq, int nb, int nw, int niv, double beta,

@@ -220,11 +220,14 @@ def _resolve_fp64(out, info, chi0_inv, gamma_cmp, u_r_m, v_r_m, beta):
         info[c, w, qi] = inf[0, 0]
 
 
-def ladder_solve_sum_multi(models, check_info=True):
-    """Plain-FP64 ladder solves of several models in one call (dgami_ladder_solve_sum_multi).
+def ladder_solve_sum_multi(models, check_info=True, precision=_lib.PLAIN_FP64):
+    """Ladder solves of several models in one call (dgami_ladder_solve_sum_multi; with a mixed
+    precision dgami_ladder_solve_sum_multi_prec, whose blocked-LU models take that form and whose
+    register-resident models stay FP64).
 
     models: list of (chi0_inv QF, gamma_cmp [nchan][nw][n][n], u_r_m [nchan][nb^4], v_r_m or None, beta).
-    Returns a list of (chi_aux_sum [nchan] x QF, info [nchan][nw][nq]) per model.
+    Returns a list of (chi_aux_sum [nchan] x QF, info [nchan][nw][nq]) per model. check_info as in
+    ladder_solve_sum (stalled or non-finite mixed items solved again in FP64, singular ones raise).
     """
     outs, table = [], (_lib.LadderModel * max(1, len(models)))()
     for m, (chi0_inv, gamma_cmp, u_r_m, v_r_m, beta) in enumerate(models):
@@ -239,9 +242,14 @@ def ladder_solve_sum_multi(models, check_info=True):
         table[m] = _lib.LadderModel(nchan, nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp), _p(u_r_m),
                                     _p(v_r_m), _p(out), _p(info, torch.int32))
         outs.append((out, info))
-    _lib.call("dgami_ladder_solve_sum_multi", len(models), ctypes.addressof(table), _stream())
+    if precision == _lib.PLAIN_FP64:
+        _lib.call("dgami_ladder_solve_sum_multi", len(models), ctypes.addressof(table), _stream())
+    else:
+        _lib.call("dgami_ladder_solve_sum_multi_prec", len(models), ctypes.addressof(table), int(precision), _stream())
     if check_info and not capturing():
-        for out, info in outs:
+        for (out, info), (chi0_inv, gamma_cmp, u_r_m, v_r_m, beta) in zip(outs, models):
+            if precision != _lib.PLAIN_FP64:
+                _resolve_fp64(out, info, chi0_inv, gamma_cmp, u_r_m, v_r_m, beta)
             bad = torch.nonzero(info).cpu().numpy()
             if len(bad):
                 raise np.linalg.LinAlgError(f"singular ladder matrix at (chan, w, q) = {bad[:8].tolist()}")

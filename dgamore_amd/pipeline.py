@@ -345,11 +345,16 @@ class ModelBatch:
     in the graph. The reference runs one model per invocation
     (nonlocal_sde.py:777-914); this replaces a loop of such runs."""
 
-    def __init__(self, models, mixing=0.3, symmetries="two_dimensional_square"):
-        self.pipes = [SDEPipeline(m, _lib.PLAIN_FP64, symmetries, mixing) for m in models]
+    def __init__(self, models, mixing=0.3, symmetries="two_dimensional_square", precision=_lib.PLAIN_FP64):
+        """precision: the form of the blocked-LU models' ladder solves (the register-resident ones,
+        compound size <= 64 by default, stay FP64)."""
+        self.precision = precision
+        self.pipes = [SDEPipeline(m, precision, symmetries, mixing) for m in models]
 
     def tune(self, **kw):
-        """Autotune the ladder kernel of every distinct shape of the batch."""
+        """Autotune the ladder kernel of every distinct shape of the batch: the FP64 choice (which
+        also decides the register-resident models), and with a mixed precision the blocked-LU
+        variant of the shapes the FP64 choice does not put in the register kernel."""
         from dgamore_amd import autotune
 
         out, seen = [], set()
@@ -357,7 +362,10 @@ class ModelBatch:
             key = (p.m.nb, p.m.niv, p.nq, p.m.niw + 1)
             if key not in seen:
                 seen.add(key)
-                out.append(autotune.tune_ladder(*key, nchan=len(p.CHANNELS), **kw))
+                r = autotune.tune_ladder(*key, nchan=len(p.CHANNELS), **kw)
+                out.append(r)
+                if self.precision != _lib.PLAIN_FP64 and autotune.FAMILIES.get(r["choice"] // 256) != "reg":
+                    out.append(autotune.tune_ladder(*key, nchan=len(p.CHANNELS), precision=self.precision, **kw))
         return out
 
     def step(self, sigmas=None, mus=None, keep=False, scf=False):
@@ -370,7 +378,9 @@ class ModelBatch:
         mus = [None] * len(self.pipes) if mus is None else mus
         fronts = [p.front(s, mu) for p, s, mu in zip(self.pipes, sigmas, mus)]
         outs = K.ladder_solve_sum_multi([p.ladder_model(f) for p, f in zip(self.pipes, fronts)],
-                                        check_info=not K.capturing())
+                                        check_info=not K.capturing(), precision=self.precision)
+        for p, (_, info) in zip(self.pipes, outs):
+            p.last_info = info
         news = [p.back(f, aux, info, s, None, keep) for p, f, (aux, info), s in zip(self.pipes, fronts, outs, sigmas)]
         if scf:
             news = [p.scf_update(n) for p, n in zip(self.pipes, news)]

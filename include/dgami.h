@@ -172,6 +172,12 @@ typedef struct {
  * batch of models. */
 int dgami_ladder_solve_sum_multi(int nmodels, const dgami_ladder_model* models, void* stream);
 
+/* dgami_ladder_solve_sum_multi with the blocked-LU models solved in `precision`
+ * (DGAMI_PLAIN_FP64 / DGAMI_MIXED_FP32_IR / DGAMI_MIXED_BF16X3_IR, as in dgami_ladder_solve_sum,
+ * info included); the register-resident models (compound size <= 64 by default) stay FP64.
+ * No counterpart in the reference, as dgami_ladder_solve_sum_multi. */
+int dgami_ladder_solve_sum_multi_prec(int nmodels, const dgami_ladder_model* models, int precision, void* stream);
+
 /* Autotuner table of the ladder solves. A kernel id is family * 256 + variant, family 1 = register
  * tiles, 2 = column-cyclic, 3 = blocked LU (variant: panel shape). candidates() writes up to `cap`
  * ids that can run a compound size n with nrhs right-hand sides in `precision` and returns how many

@@ -197,3 +197,68 @@ def test_model_batch_scf_graph_matches_eager_iterations(K, A):
             assert rel(g, e.cpu().numpy()) < 1e-12, (it, m.name)
             assert abs(float(pg.mu_dev.item()) - float(pe.mu_dev.item())) < 1e-12, (it, m.name)
     assert max(abs(float(p.mu_dev.item()) - m) for p, m in zip(mb.pipes, mu0)) > 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp32", "bf16x3"])
+def test_mixed_model_batch_matches_plain_and_replays(K, A, precision):
+    """A batch whose blocked-LU models (n > 64) take the complex64 LU with FP64 refinement: each
+    model's new Sigma within 1e-8 of the complex128 batch, and a captured whole self-consistency
+    iteration (mu secant on the device) replays equal to the eager mixed iteration."""
+    from dgamore_amd import _lib
+    from dgamore_amd.pipeline import ModelBatch
+
+    prec = {"fp32": _lib.MIXED_FP32_IR, "bf16x3": _lib.MIXED_BF16X3_IR}[precision]
+    models = _small_batch()
+    sizes = [m.nb ** 2 * 2 * m.niv for m in models]
+    assert max(sizes) > 64
+    say(f"mixed ({precision}) model batch vs plain batch, n = {sizes}")
+    plain = ModelBatch(models, mixing=0.3).step()
+    mb = ModelBatch(models, mixing=0.3, precision=prec)
+    mixed = mb.step()
+    torch.cuda.synchronize()
+    for m, a, b in zip(models, mixed, plain):
+        err = float((a - b).abs().max() / b.abs().max())
+        assert err < 1e-8, (m.name, err)
+    mb.capture(chain=True, scf=True)
+    eager = ModelBatch(models, mixing=0.3, precision=prec)
+    sig = [s.clone() for s in mb.graph_sigmas]
+    eager.start_scf(sig)
+    for it in range(2):
+        mb.replay()
+        torch.cuda.synchronize()
+        sig = eager.step(sig, scf=True)
+        torch.cuda.synchronize()
+        for g, e, pg, pe, m in zip(mb.graph_sigmas, sig, mb.pipes, eager.pipes, models):
+            assert rel(g, e.cpu().numpy()) < 1e-12, (it, m.name)
+            assert abs(float(pg.mu_dev.item()) - float(pe.mu_dev.item())) < 1e-12, (it, m.name)
+        assert all(int(torch.count_nonzero(p.last_info)) == 0 for p in mb.pipes)
+
+
+def test_config3_batch_mixed_matches_plain(K):
+    """The 16-model batch of the benchmark (seed 180) with its blocked-LU models (n = 72..1080) in
+    split-BF16 + FP64 refinement: one eager self-consistency iteration gives every model's Sigma
+    within 1e-8 of the complex128 iteration, and every model's mu within 1e-10."""
+    from dgamore_amd import _lib
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.pipeline import ModelBatch
+
+    cfgs = S.batch_configs(seed=180)
+    say("config-3 batch seed 180: split-BF16 iteration vs complex128 iteration")
+    models = [S.make_model(c, device=torch.device("cuda"), symmetric=True) for c in cfgs]
+    mb = ModelBatch(models, mixing=0.3, precision=_lib.MIXED_BF16X3_IR)
+    mb.start_scf()
+    state0 = [(p.mu_dev.clone(), p.occ.clone(), p.occ_k.clone()) for p in mb.pipes]
+    mixed = mb.step(scf=True)
+    mu_mixed = [float(p.mu_dev.item()) for p in mb.pipes]
+    for p, (mu, occ, occ_k) in zip(mb.pipes, state0):
+        p.mu_dev.copy_(mu)
+        p.occ.copy_(occ)
+        p.occ_k.copy_(occ_k)
+    mb.precision = _lib.PLAIN_FP64
+    plain = mb.step(scf=True)
+    torch.cuda.synchronize()
+    errs = [float((a - b).abs().max() / b.abs().max()) for a, b in zip(mixed, plain)]
+    dmu = [abs(a - float(p.mu_dev.item())) for a, p in zip(mu_mixed, mb.pipes)]
+    say(f"  max Sigma rel. error {max(errs):.2e}, max mu difference {max(dmu):.2e}")
+    assert max(errs) < 1e-8
+    assert max(dmu) < 1e-10
