@@ -9,6 +9,10 @@
 // Matvec for nvec vectors: gg = fft_k(chi0pp . gap); per R one pass over
 // both matrices (memory bound: the matrices are read once per application,
 // shared by all nvec vectors); then ifft_k with the 0.5/(N beta) norm.
+#include <cstdlib>
+#include <cstring>
+#include <type_traits>
+
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
@@ -124,6 +128,116 @@ __global__ void __launch_bounds__(256) pair_gemv_kernel(const M* __restrict__ g_
   }
 }
 
+// The same product with RB rows per wave at a time, so that every x value read from LDS feeds RB
+// rows (the one-row form spends two LDS reads per two complex FMAs), and, for complex64 matrices
+// (CPL = 2), two adjacent columns per lane (16-byte loads). The channel sign is applied to x(-R)
+// when it is staged. NV >= nvec vectors are held in registers. Requires n even for CPL = 2.
+template <typename M, int NV, int RB, int CPL>
+__global__ void __launch_bounds__(256) pair_gemv_rb_kernel(const M* __restrict__ g_hat, const M* __restrict__ g_flip,
+                                                          const double2* __restrict__ x, double2* __restrict__ y,
+                                                          FftPlan3D plan, int n, int nvec, double sign) {
+  extern __shared__ double2 s_x[];  // [2][nvec][n]: x(R) and sign * x(-R)
+  const int N = plan.ntot;
+  const int R = blockIdx.x;
+  const int z = R % plan.n[2], yy = (R / plan.n[2]) % plan.n[1], xx = R / (plan.n[2] * plan.n[1]);
+  const int Rm = (((plan.n[0] - xx) % plan.n[0]) * plan.n[1] + (plan.n[1] - yy) % plan.n[1]) * plan.n[2] +
+                 (plan.n[2] - z) % plan.n[2];
+  for (int e = threadIdx.x; e < nvec * n; e += blockDim.x) {
+    s_x[e] = x[static_cast<size_t>(e) * N + R];
+    s_x[nvec * n + e] = cscale(x[static_cast<size_t>(e) * N + Rm], sign);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const M* M1 = g_hat + static_cast<size_t>(R) * n * n;
+  const M* M2 = g_flip + static_cast<size_t>(R) * n * n;
+  for (int r0 = wave * RB; r0 < n; r0 += 4 * RB) {
+    double2 acc[RB][NV];
+#pragma unroll
+    for (int r = 0; r < RB; ++r)
+#pragma unroll
+      for (int vv = 0; vv < NV; ++vv) acc[r][vv] = make_double2(0.0, 0.0);
+    for (int c0 = lane * CPL; c0 < n; c0 += 64 * CPL) {
+      double2 xa[CPL][NV], xb[CPL][NV];
+#pragma unroll
+      for (int h = 0; h < CPL; ++h)
+#pragma unroll
+        for (int vv = 0; vv < NV; ++vv) {
+          const bool ok = vv < nvec && c0 + h < n;
+          xa[h][vv] = ok ? s_x[vv * n + c0 + h] : make_double2(0.0, 0.0);
+          xb[h][vv] = ok ? s_x[(nvec + vv) * n + c0 + h] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+      for (int r = 0; r < RB; ++r) {
+        const int row = r0 + r;
+        if (row >= n) break;
+        double2 m1[CPL], m2[CPL];
+        if constexpr (CPL == 2) {
+          const float4 a = *reinterpret_cast<const float4*>(M1 + static_cast<size_t>(row) * n + c0);
+          const float4 b = *reinterpret_cast<const float4*>(M2 + static_cast<size_t>(row) * n + c0);
+          m1[0] = make_double2(a.x, a.y);
+          m1[1] = make_double2(a.z, a.w);
+          m2[0] = make_double2(b.x, b.y);
+          m2[1] = make_double2(b.z, b.w);
+        } else {
+          m1[0] = to_c128(M1[static_cast<size_t>(row) * n + c0]);
+          m2[0] = to_c128(M2[static_cast<size_t>(row) * n + c0]);
+        }
+#pragma unroll
+        for (int h = 0; h < CPL; ++h)
+#pragma unroll
+          for (int vv = 0; vv < NV; ++vv) {
+            acc[r][vv] = cfma(acc[r][vv], m1[h], xa[h][vv]);
+            acc[r][vv] = cfma(acc[r][vv], m2[h], xb[h][vv]);
+          }
+      }
+    }
+#pragma unroll
+    for (int r = 0; r < RB; ++r) {
+      const int row = r0 + r;
+#pragma unroll
+      for (int vv = 0; vv < NV; ++vv) {
+        double re = acc[r][vv].x, im = acc[r][vv].y;
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) {
+          re += __shfl_xor(re, off);
+          im += __shfl_xor(im, off);
+        }
+        if (lane == 0 && row < n && vv < nvec) y[(static_cast<size_t>(vv) * n + row) * N + R] = make_double2(re, im);
+      }
+    }
+  }
+}
+
+// The row-blocked product for nvec vectors: NV = 1, 2, 4 or 8 registers per row, RB rows per
+// wave (RB * NV <= 16 accumulators), two columns per lane for complex64 matrices with up to four
+// vectors and even n.
+template <typename M, int NV, int RB>
+static void pair_gemv_rb_launch(const M* g_hat, const M* g_flip, const double2* t, double2* yv, const FftPlan3D& plan,
+                                int n, int nvec, double sign, size_t lds, hipStream_t stream) {
+  if constexpr (std::is_same<M, float2>::value && NV <= 4) {
+    if (n % 2 == 0) {
+      allow_lds(pair_gemv_rb_kernel<M, NV, RB, 2>, lds);
+      hipLaunchKernelGGL((pair_gemv_rb_kernel<M, NV, RB, 2>), dim3(plan.ntot), dim3(256), lds, stream, g_hat, g_flip,
+                         t, yv, plan, n, nvec, sign);
+      return;
+    }
+  }
+  allow_lds(pair_gemv_rb_kernel<M, NV, RB, 1>, lds);
+  hipLaunchKernelGGL((pair_gemv_rb_kernel<M, NV, RB, 1>), dim3(plan.ntot), dim3(256), lds, stream, g_hat, g_flip, t, yv,
+                     plan, n, nvec, sign);
+}
+
+// Rows per wave: measured on the 2-orbital 48x48 operator (n = 512, 2304 lattice vectors), whole
+// application with one / four vectors, complex128 and complex64 matrices (ms):
+//   one-row form 3.83 / 6.65, 2.50 / 6.36;  RB = 2: 3.65 / 5.05, 2.13 / 3.69;
+//   RB = 4: 3.62 / 4.89, 2.11 / 3.34;       RB = 8: 3.62 / 5.01, 2.10 / 3.91.
+template <typename M, int NV>
+static void pair_gemv_nv_launch(const M* g_hat, const M* g_flip, const double2* t, double2* yv, const FftPlan3D& plan,
+                                int n, int nvec, double sign, size_t lds, hipStream_t stream) {
+  constexpr int RB = NV >= 8 ? 2 : 4;
+  pair_gemv_rb_launch<M, NV, RB>(g_hat, g_flip, t, yv, plan, n, nvec, sign, lds, stream);
+}
+
 // out[vec][k][a][b][v] <- t[vec][(a,b,v)][k]
 __global__ void pair_out_kernel(const double2* __restrict__ t, double2* __restrict__ out, int N, int n, int nvec) {
   const size_t total = static_cast<size_t>(nvec) * N * n;
@@ -184,6 +298,12 @@ static int pairing_setup(int nx, int ny, int nz, int nb, int np2, int channel, c
   return 0;
 }
 
+// DGAMI_PAIR_GEMV=row selects the one-row product (for comparisons).
+static bool pair_gemv_one_row() {
+  const char* v = getenv("DGAMI_PAIR_GEMV");
+  return v && strcmp(v, "row") == 0;
+}
+
 template <typename M>
 static int pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, double beta, const M* g_hat,
                           const M* g_hat_flip, const dgami_c128* chi0pp, int nvec, const dgami_c128* gap,
@@ -209,9 +329,20 @@ static int pairing_matvec(int nx, int ny, int nz, int nb, int np2, int channel, 
   DGAMI_HIP_CHECK(hipGetLastError());
   int rc = launch_fft_grid(t, t, static_cast<int64_t>(nvec) * n, nx, ny, nz, -1, 1.0, stream);
   if (rc) return rc;
-  allow_lds(pair_gemv_kernel<M>, lds);
-  hipLaunchKernelGGL(pair_gemv_kernel<M>, dim3(N), dim3(256), lds, stream, g_hat, g_hat_flip, t, yv, plan, n, nvec,
-                     channel == 0 ? 1.0 : -1.0);
+  const double sign = channel == 0 ? 1.0 : -1.0;
+  if (pair_gemv_one_row()) {
+    allow_lds(pair_gemv_kernel<M>, lds);
+    hipLaunchKernelGGL(pair_gemv_kernel<M>, dim3(N), dim3(256), lds, stream, g_hat, g_hat_flip, t, yv, plan, n, nvec,
+                       sign);
+  } else if (nvec == 1) {
+    pair_gemv_nv_launch<M, 1>(g_hat, g_hat_flip, t, yv, plan, n, nvec, sign, lds, stream);
+  } else if (nvec == 2) {
+    pair_gemv_nv_launch<M, 2>(g_hat, g_hat_flip, t, yv, plan, n, nvec, sign, lds, stream);
+  } else if (nvec <= 4) {
+    pair_gemv_nv_launch<M, 4>(g_hat, g_hat_flip, t, yv, plan, n, nvec, sign, lds, stream);
+  } else {
+    pair_gemv_nv_launch<M, 8>(g_hat, g_hat_flip, t, yv, plan, n, nvec, sign, lds, stream);
+  }
   DGAMI_HIP_CHECK(hipGetLastError());
   rc = launch_fft_grid(yv, yv, static_cast<int64_t>(nvec) * n, nx, ny, nz, 1, 0.5 / (static_cast<double>(N) * beta) / N,
                        stream);
