@@ -18,4 +18,5 @@ lambda_correction = SimpleNamespace(perform_lambda_correction=False, type="spch"
 eliashberg = SimpleNamespace(n_eig=4, epsilon=1e-12, symmetry="random", save_fq=False, construct_fq_cheap=False,
                              matvec_precision="fp64")
 memory = SimpleNamespace(save_memory_for_chiq_aux=False, save_memory_for_chi0q=False, save_memory_for_sde=False)
+dmft = SimpleNamespace(type="w2dyn", input_path="", fname_1p="1p-data.hdf5", fname_2p="g4iw_sym.hdf5")
 precision = SimpleNamespace(ladder=0)  # 0 plain FP64, 1 FP32 + FP64 refinement, 2 split-BF16 + FP64 refinement

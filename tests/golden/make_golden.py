@@ -68,6 +68,9 @@ W_SUBSETS = {
 FILES.update({
     "end_2_end/g_loc.npy": "e2e_g_loc.npy",
     "srvo3_end2end/g_loc.npy": "srvo3_g_loc.npy",
+    # the DMFT input of end_2_end (w2dynamics one-particle output, HDF5): beta, mu, G, Sigma, the
+    # occupation and the interaction parameters, read by dgamore_amd.dmft_interface
+    "end_2_end/1p-data.hdf5": "e2e_1p-data.hdf5",
 })
 
 

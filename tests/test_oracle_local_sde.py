@@ -5,7 +5,8 @@ Fixtures (complex64 files of the reference's test data, so agreement is bounded 
 * end_2_end: gchi0_loc.npy is the local bubble of g_loc.npy (beta = 12.5, niw = 20, niv_full = 30);
   siw_dga_local.npy is the local Sigma of vrg_{dens,magn}_loc, chi_{dens,magn}_loc and g_dmft.npy
   (U = 8 on aaaa), whose Hartree-Fock constant uses the DMFT occupation of the reference's
-  1p-data.hdf5, which is not read here: it is checked as a constant.
+  1p-data.hdf5: checked here as a constant, and exactly, with the occupation read from that file, in
+  test_dmft_interface.py.
 * srvo3_end2end: f_{dens,magn}_loc.npy is the full vertex of gamma_{dens,magn}_loc.npy with the bubble
   of g_loc.npy, no shell (niv_shell = 0), 3 orbitals. The run's beta is not stored in these files; the
   tail of G, Im G(v_n) -> -1 / v_n, gives beta = 1.0 (checked below), and F matches only with it.
