@@ -245,16 +245,24 @@ __device__ __forceinline__ void dft_small<3>(double2* v, double sign) {
 
 template <>
 __device__ __forceinline__ void dft_small<5>(double2* v, double sign) {
-  double2 in[5] = {v[0], v[1], v[2], v[3], v[4]};
-  for (int k = 0; k < 5; ++k) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int j = 0; j < 5; ++j) {
-      double s, c;
-      sincospi(sign * 2.0 * ((k * j) % 5) / 5.0, &s, &c);
-      acc = cfma(acc, in[j], make_double2(c, s));
-    }
-    v[k] = acc;
-  }
+  // w = exp(sign 2 pi i / 5); with t1 = x1 + x4, t2 = x2 + x3, t3 = x1 - x4, t4 = x2 - x3:
+  //   X0 = x0 + t1 + t2,  X1,4 = a1 +- i b1,  X2,3 = a2 +- i b2,
+  //   a1 = x0 + c1 t1 + c2 t2, a2 = x0 + c2 t1 + c1 t2 (c_k = cos(2 pi k / 5)),
+  //   b1 = sign (s1 t3 + s2 t4), b2 = sign (s2 t3 - s1 t4) (s_k = sin(2 pi k / 5))
+  const double c1 = 0.30901699437494742410, c2 = -0.80901699437494742410;
+  const double s1 = 0.95105651629515357212 * sign, s2 = 0.58778525229247312917 * sign;
+  const double2 t1 = cadd(v[1], v[4]), t2 = cadd(v[2], v[3]);
+  const double2 t3 = csub(v[1], v[4]), t4 = csub(v[2], v[3]);
+  const double2 x0 = v[0];
+  const double2 a1 = make_double2(x0.x + c1 * t1.x + c2 * t2.x, x0.y + c1 * t1.y + c2 * t2.y);
+  const double2 a2 = make_double2(x0.x + c2 * t1.x + c1 * t2.x, x0.y + c2 * t1.y + c1 * t2.y);
+  const double2 b1 = make_double2(s1 * t3.x + s2 * t4.x, s1 * t3.y + s2 * t4.y);
+  const double2 b2 = make_double2(s2 * t3.x - s1 * t4.x, s2 * t3.y - s1 * t4.y);
+  v[0] = cadd(x0, cadd(t1, t2));
+  v[1] = make_double2(a1.x - b1.y, a1.y + b1.x);
+  v[4] = make_double2(a1.x + b1.y, a1.y - b1.x);
+  v[2] = make_double2(a2.x - b2.y, a2.y + b2.x);
+  v[3] = make_double2(a2.x + b2.y, a2.y - b2.x);
 }
 
 // One Stockham stage of radix R over `nfft` transforms of length L held in
