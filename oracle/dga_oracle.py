@@ -38,6 +38,9 @@ PARITY = {
     "v_as_channel": "the reference's expectations of tests/test_interaction.py:166-200 (test_oracle_interaction.py)",
     "sum_over_vn": "(1/beta) sum over nu; part of the vrg -> gchi_aux_q_*_sum reference chain",
     "to_full_niw": "F(-w) = conj F(w); reproduces the reference's lambda-correction files (test_oracle_golden.py)",
+    "local_uncompound": "layout helper, inverse of compound_local_vertex; pinned through gamma_shell, which returns "
+                        "the planted core vertex, and full_vertex_from_gamma, which reproduces the reference's SrVO3 "
+                        "f_{dens,magn}_loc.npy (test_oracle_local_sde.py)",
     "apply_lambda": "closed form; with find_lambda reproduces the reference's corrected chi files to 1e-5 "
                     "(the complex64 limit, test_oracle_golden.py)",
     "map_irr_to_fbz": "K_FBZ[k] = K_irr[irrk_inv[k]]; sigma_fft through it equals the direct q-sum form",
