@@ -70,6 +70,11 @@ def parse():
     ap.add_argument("--eliashberg", action="store_true",
                     help="instead of the SDE step: the linearised-gap matvec at the 2-orbital 48x48, 2np=128 "
                          "workload with the vertex stored in complex128 and in complex64")
+    ap.add_argument("--eliashberg-solve", action="store_true",
+                    help="instead of the SDE step: the whole Eliashberg solve of the 2-orbital 48x48, 2np=128 "
+                         "workload from the ladder (bubble and ladder of both channels, three-leg vertices, "
+                         "pairing vertices from the full vertices with the complex128 and the split-BF16 "
+                         "factorisation, the four leading eigenpairs by Arnoldi), each stage timed")
     return ap.parse_args()
 
 
@@ -129,6 +134,71 @@ def eliashberg_matvec(steps, warmup):
     print(json.dumps({"workload": f"eliashberg matvec: 2 orbitals, k-grid 48x48x1 ({nq} irreducible q), 2np={np2}, "
                                   f"n={n}, {N} lattice vectors", "results": results,
                       "rel_err_c64_vs_c128": err}), flush=True)
+
+
+def eliashberg_solve():
+    """The Eliashberg equation of the 2-orbital e_g model (U = 5, J = 0.8, beta = 20, 48x48 k-grid,
+    325 irreducible q, niw = 127, niv = np = 64: 2np = 128, operator dimension 1,179,648) from the
+    ladder, stage by stage: the SDE front (G, bubble, chi0^-1), the ladder sums of both channels,
+    the three-leg vertices and physical susceptibilities, the singlet pairing vertex from the full
+    vertices (complex128 LU, then split-BF16 LU), the R-space operator, and the four leading
+    eigenpairs by Arnoldi with their residuals, for the pairing vertex of each factorisation."""
+    import numpy as np
+    import torch
+
+    from dgamore_amd import eliashberg_solver as es
+    from dgamore_amd import kernels as K
+    from dgamore_amd import synthetic as S
+    from dgamore_amd.brillouin_zone import KGrid
+    from dgamore_amd.pipeline import SDEPipeline
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        out = fn()
+        torch.cuda.synchronize()
+        return out, time.perf_counter() - t
+
+    nk, niw, niv = (48, 48, 1), 127, 64
+    (m, pipe), t_model = timed(lambda: (lambda mm: (mm, SDEPipeline(mm)))(
+        S.make_model(4, nk=nk, niw=niw, niv=niv, device=torch.device("cuda"))))
+    front, t_front = timed(lambda: pipe.front(pipe.sigma_dmft))
+    (aux, _), t_ladder = timed(lambda: K.ladder_solve_sum(front["chi0_inv"], pipe.gamma, pipe.u_r, pipe.v_r, m.beta))
+
+    def three_leg():
+        out = [K.vrg_chi_phys(front["chi0_inv"], aux[c], front["full"], front["core"], pipe.u_r[c], None, m.nb, m.beta)
+               for c in range(2)]
+        return [o[0] for o in out], [o[1] for o in out]
+
+    (vrgs, chis), t_vrg = timed(three_leg)
+    chi0pp = K.chi0_pp(front["g"], niv)
+    solves = 2 * pipe.nq * (niw + 1)
+    stages = {"model_setup_s": t_model, "front_s": t_front, "ladder_s": t_ladder, "ladder_solves": solves,
+              "three_leg_s": t_vrg}
+    results = {}
+    v0 = np.random.default_rng(7).standard_normal(48 * 48 * 4 * 2 * niv) + 0j
+    for name, prec in (("c128", 0), ("bf16x3", 2)):
+        (gs, _), t_pp = timed(lambda: es.pairing_vertices_from_ladder(front["chi0_inv"], pipe.gamma, pipe.u_r,
+                                                                       pipe.v_r, vrgs, chis, m.beta, m.nb, niv,
+                                                                       precision=prec))
+        op, t_op = timed(lambda: es.PairingOperator(gs, chi0pp, KGrid(nk), m.beta, "sing", "fp64"))
+        del gs
+        t0 = time.perf_counter()
+        lams, _, res = es.solve_eliashberg_arnoldi(op, 4, tol=1e-12, v0=v0)
+        t_arn = time.perf_counter() - t0
+        results[name] = {"pairing_vertex_s": t_pp, "operator_setup_s": t_op, "arnoldi_s": t_arn,
+                         "lambda": [[float(x.real), float(x.imag)] for x in lams], "residuals": res.tolist()}
+        del op
+        torch.cuda.empty_cache()
+        print(f"[bench] eliashberg {name}: pairing vertex {t_pp:.2f} s, operator {t_op:.2f} s, Arnoldi {t_arn:.2f} s, "
+              f"lambda {np.round(lams, 6).tolist()}", file=sys.stderr, flush=True)
+    la = np.array([complex(*x) for x in results["c128"]["lambda"]])
+    lb = np.array([complex(*x) for x in results["bf16x3"]["lambda"]])
+    print(json.dumps({"workload": "eliashberg solve: 2 orbitals (e_g), U=5, J=0.8, beta=20, k-grid 48x48 "
+                                  f"({pipe.nq} irreducible q), niw={niw}, 2np={2 * niv}, singlet, operator "
+                                  f"dimension {48 * 48 * 4 * 2 * niv}",
+                      "stages": stages, "results": results,
+                      "lambda_rel_err_bf16x3_vs_c128": float(np.abs(la - lb).max() / np.abs(la).max())}), flush=True)
 
 
 def batch_graph(steps, warmup, seed, symmetric=True, precision="plain"):
@@ -522,6 +592,9 @@ def main():
         return
     if args.eliashberg:
         eliashberg_matvec(args.steps, args.warmup)
+        return
+    if args.eliashberg_solve:
+        eliashberg_solve()
         return
     if args.batch:
         batch_graph(args.steps if "--steps" in sys.argv else 10, args.warmup, args.batch_seed, not args.no_symmetric,
