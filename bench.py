@@ -186,12 +186,17 @@ def eliashberg_solve():
         t0 = time.perf_counter()
         lams, _, res = es.solve_eliashberg_arnoldi(op, 4, tol=1e-12, v0=v0)
         t_arn = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        lks, _, rks = es.solve_eliashberg_krylov_schur(op, 4, tol=1e-12, v0=v0)
+        t_ks = time.perf_counter() - t0
         results[name] = {"pairing_vertex_s": t_pp, "operator_setup_s": t_op, "arnoldi_s": t_arn,
-                         "lambda": [[float(x.real), float(x.imag)] for x in lams], "residuals": res.tolist()}
+                         "lambda": [[float(x.real), float(x.imag)] for x in lams], "residuals": res.tolist(),
+                         "krylov_schur_s": t_ks, "krylov_schur_residuals": rks.tolist(),
+                         "krylov_schur_vs_arnoldi": float(np.abs(lks - lams).max() / np.abs(lams).max())}
         del op
         torch.cuda.empty_cache()
         print(f"[bench] eliashberg {name}: pairing vertex {t_pp:.2f} s, operator {t_op:.2f} s, Arnoldi {t_arn:.2f} s, "
-              f"lambda {np.round(lams, 6).tolist()}", file=sys.stderr, flush=True)
+              f"device Krylov-Schur {t_ks:.2f} s, lambda {np.round(lams, 6).tolist()}", file=sys.stderr, flush=True)
     la = np.array([complex(*x) for x in results["c128"]["lambda"]])
     lb = np.array([complex(*x) for x in results["bf16x3"]["lambda"]])
     print(json.dumps({"workload": "eliashberg solve: 2 orbitals (e_g), U=5, J=0.8, beta=20, k-grid 48x48 "
