@@ -40,7 +40,7 @@ SIGNATURES = {
     "dgami_block_invert": ([_i, _i, _c, _c, _c, _c], _i),
     "dgami_ladder_invert": ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c, _c], _i),
     "dgami_full_vertex": ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c, _c, _c, _c], _i),
-    "dgami_full_vertex_prec": ([_i, _i, _i, _i, _d, _i, _c, _c, _c, _c, _c, _c, _c, _c, _c], _i),
+    "dgami_full_vertex_prec": ([_i, _i, _i, _i, _d, _i, _i, _i, _c, _c, _c, _c, _c, _c, _c, _c, _c], _i),
     "dgami_pairing_vertex_pp": ([_i, _i, _i, _i, _i, _c, _c, _c, _c, _c], _i),
     "dgami_g_full": ([_i, _i, _i, _d, _d, _c, _c, _i, _i, _c, _c], _i),
     "dgami_vrg_chi_phys": ([_i, _i, _i, _i, _d, _c, _c, _c, _c, _c, _c, _c, _c, _c, _c], _i),

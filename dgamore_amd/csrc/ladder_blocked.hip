@@ -137,7 +137,7 @@ __global__ void __launch_bounds__(256) blk_assemble_kernel(LadderArgs a, Blocked
               if (vi == vj[h]) val[h] = cadd(val[h], Ck[k][(static_cast<size_t>(ri * T + cj[h]) * nv + vi) * a.nq]);
             }
           } else if (j < ncol) {
-            val[h] = a.rhs_mode == LADDER_RHS_VERTEX ? ladder_vertex_part(a, Gk[k], Uk[k], Vk[k], i, j - n)
+            val[h] = a.rhs_mode == LADDER_RHS_VERTEX ? ladder_vertex_part(a, Gk[k], Uk[k], Vk[k], i, ladder_rhs_col(a, j - n))
                                                      : ladder_rhs_at(a, wk[k], qk[k], i, j - n);
           }
         }

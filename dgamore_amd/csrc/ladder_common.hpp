@@ -20,7 +20,17 @@ struct LadderArgs {
   int n;     // T * nv
   double inv_beta2, inv_beta;
   int rhs_mode = 0;  // LadderRhs: right-hand sides carried by the blocked LU
+  // CHI0INV / VERTEX right-hand sides restricted to the columns (c, v') with v' in
+  // [rhs_vlo, rhs_vlo + rhs_w) for every orbital pair c (rhs_w = 0: every column)
+  int rhs_vlo = 0, rhs_w = 0;
 };
+
+// Compound column of right-hand side r (CHI0INV / VERTEX modes).
+__device__ __forceinline__ int ladder_rhs_col(const LadderArgs& a, int r) {
+  if (a.rhs_w == 0) return r;
+  const int c = r / a.rhs_w;
+  return c * a.nv + a.rhs_vlo + (r - c * a.rhs_w);
+}
 
 // Right-hand sides of the blocked LU: the nb^2 indicator columns of the summed ladder, the
 // identity (full inverse), the block-diagonal chi0^-1 (X chi0^-1 for the full vertex) or the
@@ -63,7 +73,8 @@ __device__ __forceinline__ double2 ladder_rhs_at(const LadderArgs& a, int w, int
   if (a.rhs_mode == LADDER_RHS_CHI0INV) {
     const int nv = a.nv, T = a.T;
     const int ri = i / nv, vi = i - ri * nv;
-    const int cj = r / nv, vj = r - cj * nv;
+    const int col = ladder_rhs_col(a, r);
+    const int cj = col / nv, vj = col - cj * nv;
     if (vi != vj) return make_double2(0.0, 0.0);
     return a.chi0_inv[((static_cast<size_t>(w) * T * T + ri * T + cj) * nv + vi) * a.nq + q];
   }
@@ -107,7 +118,7 @@ __device__ __forceinline__ LadderSrc ladder_src(const LadderArgs& a, int item) {
 // Element (i, j) of the augmented matrix [A | B] of an item, j < n + (right-hand sides).
 __device__ __forceinline__ double2 ladder_aug_at(const LadderArgs& a, const LadderSrc& s, int i, int j) {
   if (j < a.n) return ladder_elem(a, s.G, s.U, s.V, s.w, s.q, i, j);
-  if (a.rhs_mode == LADDER_RHS_VERTEX) return ladder_vertex_part(a, s.G, s.U, s.V, i, j - a.n);
+  if (a.rhs_mode == LADDER_RHS_VERTEX) return ladder_vertex_part(a, s.G, s.U, s.V, i, ladder_rhs_col(a, j - a.n));
   return ladder_rhs_at(a, s.w, s.q, i, j - a.n);
 }
 

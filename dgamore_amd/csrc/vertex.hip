@@ -113,9 +113,12 @@ struct VertexArgs {
   double2* out;             // [nw][nq][n][n]
   double beta2;
   int z_form;               // 1: the right-hand sides hold Z = X (A - chi0^-1), 0: Y = X chi0^-1
+  int vlo, vw;              // vw > 0: only rows and columns with v in [vlo, vlo + vw) are formed
 };
 
-// Full vertex of one problem, rows (ri, v) for every ri < T at one v, every column j = (cj, v'):
+// Full vertex of one problem, rows (ri, v) for every ri < T at one v, every column j = (cj, v')
+// (with a window vw > 0: v = vlo + blockIdx.x and the columns with v' in the window, whose
+// right-hand sides are r = cj vw + v' - vlo):
 //   F = beta^2 (D(v) delta_vv' - sum_s D(v)[ri][s] Y[(s, v)][j]) + sum_s P(v)[ri][s] V(v')[s][cj]
 // with Y = X chi0^-1 (right-hand-side columns), D(v) = chi0^-1(q,w,v), V(v) = vrg(q,w,v) and
 // P(v) = (u - u chi_r u) V(v), all nb^2 x nb^2 in M order. With z_form the columns hold
@@ -125,7 +128,7 @@ struct VertexArgs {
 template <int T, typename S>
 __global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedState<S> st, VertexArgs o, int item0) {
   constexpr int TT = T * T;
-  const int v = blockIdx.x, b = blockIdx.y;
+  const int v = o.vlo + blockIdx.x, b = blockIdx.y;
   const int item = item0 + b;
   const int nq = a.nq, nv = a.nv, n = a.n;
   const int q = item % nq, w = item / nq;
@@ -171,11 +174,14 @@ __global__ void __launch_bounds__(256) vtx_full_kernel(LadderArgs a, BlockedStat
   }
   const S* Y = st.W + static_cast<size_t>(b) * n * st.ld + n;
   double2* O = o.out + static_cast<size_t>(item) * n * n;
-  for (int j = tid; j < n; j += 256) {
-    const int cj = j / nv, vj = j - cj * nv;
+  const int ncols = o.vw > 0 ? T * o.vw : n;
+  for (int rc = tid; rc < ncols; rc += 256) {
+    const int cj = o.vw > 0 ? rc / o.vw : rc / nv;
+    const int vj = o.vw > 0 ? o.vlo + (rc - cj * o.vw) : rc - cj * nv;
+    const int j = cj * nv + vj;
     double2 y[T], vc[T];
 #pragma unroll
-    for (int s = 0; s < T; ++s) y[s] = to_c128(Y[static_cast<size_t>(s * nv + v) * st.ld + j]);
+    for (int s = 0; s < T; ++s) y[s] = to_c128(Y[static_cast<size_t>(s * nv + v) * st.ld + rc]);
     if (f2) {
 #pragma unroll
       for (int s = 0; s < T; ++s) vc[s] = o.vrg[((static_cast<size_t>(w) * TT + s * T + cj) * nv + vj) * nq + q];
@@ -245,12 +251,15 @@ enum VertexOut { VTX_OUT_INVERSE = 0, VTX_OUT_FULL = 1 };
 
 template <typename S>
 static int vertex_solve_t(LadderArgs a, int what, const VertexArgs& o_in, int gemm_mode, hipStream_t stream) {
-  const int n = a.n, nr = n, ncol = 2 * n;
+  DGAMI_ARG_CHECK(o_in.vw == 0 || what == VTX_OUT_FULL, "vertex: a frequency window needs the full vertex");
+  const int n = a.n, nr = o_in.vw > 0 ? a.T * o_in.vw : n, ncol = n + nr;
   const int ld = (ncol + 3) & ~3;
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "vertex: compound size above 1536 is not supported");
   // complex64 factorisations carry the vertex part as right-hand sides (no cancellation in F)
   constexpr bool c64 = std::is_same<S, float2>::value;
   a.rhs_mode = what == VTX_OUT_FULL ? (c64 ? LADDER_RHS_VERTEX : LADDER_RHS_CHI0INV) : LADDER_RHS_IDENTITY;
+  a.rhs_vlo = o_in.vlo;
+  a.rhs_w = o_in.vw;
   VertexArgs o = o_in;
   o.z_form = c64 && what == VTX_OUT_FULL;
   const size_t items = static_cast<size_t>(a.nw) * a.nq;
@@ -292,7 +301,8 @@ static int vertex_solve_t(LadderArgs a, int what, const VertexArgs& o_in, int ge
                          o.out, static_cast<int>(item0));
     } else {
 #define DGAMI_VTX_FULL(TV)                                                                                       \
-  hipLaunchKernelGGL((vtx_full_kernel<TV, S>), dim3(a.nv, nb), dim3(256), 0, stream, a, st, o, static_cast<int>(item0))
+  hipLaunchKernelGGL((vtx_full_kernel<TV, S>), dim3(o.vw > 0 ? o.vw : a.nv, nb), dim3(256), 0, stream, a, st, o, \
+                     static_cast<int>(item0))
       switch (a.T) {
         case 1: DGAMI_VTX_FULL(1); break;
         case 4: DGAMI_VTX_FULL(4); break;
@@ -348,7 +358,7 @@ extern "C" int dgami_ladder_invert(int nq, int nb, int nw, int niv, double beta,
   DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && out, "ladder_invert: null pointer");
   DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "ladder_invert: invalid sizes");
   const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
-  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(out), 0.0, 0};
+  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(out), 0.0, 0, 0, 0};
   return vertex_solve(a, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
 }
 
@@ -363,7 +373,7 @@ extern "C" int dgami_invert_dense(int nmat, int n, const dgami_c128* a, dgami_c1
   la.nv = n;
   la.n = n;
   la.inv_beta2 = 1.0;
-  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(a_inv), 0.0, 0};
+  VertexArgs o{nullptr, nullptr, reinterpret_cast<double2*>(a_inv), 0.0, 0, 0, 0};
   return vertex_solve(la, VTX_OUT_INVERSE, o, static_cast<hipStream_t>(stream));
 }
 
@@ -377,22 +387,24 @@ extern "C" int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, c
   DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "full_vertex: invalid sizes");
   const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
   VertexArgs o{reinterpret_cast<const double2*>(vrg), reinterpret_cast<const double2*>(chi_phys),
-               reinterpret_cast<double2*>(out), beta * beta, 0};
+               reinterpret_cast<double2*>(out), beta * beta, 0, 0, 0};
   return vertex_solve(a, VTX_OUT_FULL, o, static_cast<hipStream_t>(stream));
 }
 
-extern "C" int dgami_full_vertex_prec(int nq, int nb, int nw, int niv, double beta, int precision,
-                                      const dgami_c128* chi0_inv, const dgami_c128* gamma, const dgami_c128* u_r,
-                                      const dgami_c128* v_r, const dgami_c128* vrg, const dgami_c128* chi_phys,
-                                      dgami_c128* out, int32_t* info, void* stream) {
+extern "C" int dgami_full_vertex_prec(int nq, int nb, int nw, int niv, double beta, int precision, int v_lo,
+                                      int v_hi, const dgami_c128* chi0_inv, const dgami_c128* gamma,
+                                      const dgami_c128* u_r, const dgami_c128* v_r, const dgami_c128* vrg,
+                                      const dgami_c128* chi_phys, dgami_c128* out, int32_t* info, void* stream) {
   using namespace dgami;
   DGAMI_ARG_CHECK(chi0_inv && gamma && u_r && out, "full_vertex: null pointer");
   DGAMI_ARG_CHECK((vrg == nullptr) == (chi_phys == nullptr), "full_vertex: vrg and chi_phys go together");
   DGAMI_ARG_CHECK(nq >= 1 && nb >= 1 && nb <= 4 && nw >= 1 && niv >= 1 && beta > 0, "full_vertex: invalid sizes");
   DGAMI_ARG_CHECK(precision >= 0 && precision <= 2, "full_vertex: precision must be 0, 1 or 2");
+  DGAMI_ARG_CHECK((v_lo == 0 && v_hi == 0) || (0 <= v_lo && v_lo < v_hi && v_hi <= 2 * niv),
+                  "full_vertex: the window needs 0 <= v_lo < v_hi <= 2 niv (or both 0)");
   const LadderArgs a = vertex_ladder_args(nq, nb, nw, niv, beta, chi0_inv, gamma, u_r, v_r, info);
   VertexArgs o{reinterpret_cast<const double2*>(vrg), reinterpret_cast<const double2*>(chi_phys),
-               reinterpret_cast<double2*>(out), beta * beta, 0};
+               reinterpret_cast<double2*>(out), beta * beta, 0, v_lo, v_hi - v_lo};
   return vertex_solve(a, VTX_OUT_FULL, o, static_cast<hipStream_t>(stream), precision);
 }
 

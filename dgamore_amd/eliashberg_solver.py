@@ -79,7 +79,7 @@ def create_pairing_vertices_pp(f_dens, f_magn, niv_pp):
 
 
 def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta, nb, niv_pp, w_chunk=16,
-                                 precision=0):
+                                 precision=0, window_rhs=True):
     """Singlet and triplet pairing vertices [nq, nb, nb, nb, nb, 2np, 2np] on the device from the
     ladder of both channels (the chain of solve, eliashberg_solver.py:584-629, with
     create_full_vertex_q_r, :78-147, and the w = 0 transform, :44-75).
@@ -90,18 +90,26 @@ def pairing_vertices_from_ladder(chi0_inv, gammas, u_rs, v_rs, vrgs, chis, beta,
     full vertices are formed for w_chunk bosonic frequencies at a time and transformed before the
     next range, so F of every w is never held at once; pp entries with |v - v'| >= nw stay zero,
     as in the reference. precision 1 / 2 factors the ladder matrices in complex64 with FP32 /
-    split-BF16 matrix-core updates (dgami_full_vertex_prec)."""
+    split-BF16 matrix-core updates (dgami_full_vertex_prec). With window_rhs, the full vertices of
+    each range are formed only at the fermionic frequencies the transform reads (the LU carries
+    those right-hand sides only); bosonic frequencies |w| >= 2 np, whose entries stay zero, are
+    skipped."""
     import torch
 
     nw, nb4, nv, nq = chi0_inv.shape
     shape = (nq, nb, nb, nb, nb, 2 * niv_pp, 2 * niv_pp)
     gs = torch.zeros(shape, dtype=K.C128, device=chi0_inv.device)
     gt = torch.zeros(shape, dtype=K.C128, device=chi0_inv.device)
+    off = nv // 2 - niv_pp
     for w0 in range(0, nw, w_chunk):
         w1 = min(nw, w0 + w_chunk)
+        if w0 >= 2 * niv_pp:  # no pairing entry has |v - v'| this large
+            break
+        # the transform of |w| in [w0, w1) reads F(w) only at fermionic indices in [w0 + off, 2 np + off)
+        window = (w0 + off, 2 * niv_pp + off) if window_rhs else None
         fs = [K.full_vertex(chi0_inv[w0:w1].contiguous(), gammas[c, w0:w1].contiguous(), u_rs[c],
                             None if v_rs is None else v_rs[c], vrgs[c][w0:w1].contiguous(), chis[c][w0:w1].contiguous(),
-                            beta, precision=precision)[0] for c in range(2)]
+                            beta, precision=precision, window=window)[0] for c in range(2)]
         K.pairing_vertex_pp_range(fs[0], fs[1], nb, niv_pp, w0, gs, gt)
         del fs
     return gs, gt

@@ -294,11 +294,13 @@ def invert_dense(a, check_info=True):
     return out.reshape(a.shape)
 
 
-def full_vertex(chi0_inv, gamma_cmp, u_r_m, v_r_m, vrg, chi_phys, beta, check_info=True, precision=0):
+def full_vertex(chi0_inv, gamma_cmp, u_r_m, v_r_m, vrg, chi_phys, beta, check_info=True, precision=0, window=None):
     """Full vertex F_r [nw][nq][n][n] of one channel (vrg QF, chi_phys QW; both None: first term only).
 
     precision 0: complex128 LU; 1 / 2: complex64 LU with FP32 / split-BF16 matrix-core updates and
-    no refinement (relative error of F about 1e-6 of its largest element)."""
+    no refinement (relative error of F about 1e-6 of its largest element). window (v_lo, v_hi): only
+    the rows and columns with fermionic index in [v_lo, v_hi) are formed; the other entries of the
+    returned tensor are undefined."""
     nw, nb4, nv, nq = chi0_inv.shape
     nb = int(round(nb4 ** 0.25))
     n = nb * nb * nv
@@ -310,13 +312,14 @@ def full_vertex(chi0_inv, gamma_cmp, u_r_m, v_r_m, vrg, chi_phys, beta, check_in
         raise ValueError(f"chi_phys must be [{nw},{nb4},{nq}]")
     out = torch.empty((nw, nq, n, n), dtype=C128, device=chi0_inv.device)
     info = torch.zeros((nw, nq), dtype=torch.int32, device=chi0_inv.device)
-    if precision == 0:
+    v_lo, v_hi = (0, 0) if window is None else (int(window[0]), int(window[1]))
+    if precision == 0 and window is None:
         _lib.call("dgami_full_vertex", nq, nb, nw, nv // 2, float(beta), _p(chi0_inv), _p(gamma_cmp), _p(u_r_m),
                   _p(v_r_m), _p(vrg), _p(chi_phys), _p(out), _p(info, torch.int32), _stream())
     else:
-        _lib.call("dgami_full_vertex_prec", nq, nb, nw, nv // 2, float(beta), int(precision), _p(chi0_inv),
-                  _p(gamma_cmp), _p(u_r_m), _p(v_r_m), _p(vrg), _p(chi_phys), _p(out), _p(info, torch.int32),
-                  _stream())
+        _lib.call("dgami_full_vertex_prec", nq, nb, nw, nv // 2, float(beta), int(precision), v_lo, v_hi,
+                  _p(chi0_inv), _p(gamma_cmp), _p(u_r_m), _p(v_r_m), _p(vrg), _p(chi_phys), _p(out),
+                  _p(info, torch.int32), _stream())
     if check_info:
         _raise_singular(info, "ladder matrix")
     return out, info

@@ -219,19 +219,26 @@ int dgami_full_vertex(int nq, int nb, int nw, int niv, double beta, const dgami_
                       const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r, const dgami_c128* vrg,
                       const dgami_c128* chi_phys, dgami_c128* out, int32_t* info, void* stream);
 
-/* dgami_full_vertex with the LU in a chosen precision: DGAMI_PLAIN_FP64 is dgami_full_vertex;
+/* dgami_full_vertex with the LU in a chosen precision and an optional frequency window:
+ * DGAMI_PLAIN_FP64 is dgami_full_vertex;
  * DGAMI_MIXED_FP32_IR / DGAMI_MIXED_BF16X3_IR factor in complex64 with FP32 / split-BF16
  * matrix-core trailing updates, without refinement (refining n right-hand sides would cost
  * more than the complex128 LU). Their n right-hand sides are the vertex part
  * A - chi0^-1 = (gamma - u_r - v_r)/beta^2, so that the first term is formed as
  * beta^2 chi0^-1 X (A - chi0^-1), which equals beta^2 (chi0^-1 - chi0^-1 X chi0^-1) without its
  * cancellation; the output sums run in FP64.
+ * v_lo < v_hi (0 <= v_lo, v_hi <= 2 niv) restricts the output to the rows and columns whose
+ * fermionic index (0..2 niv-1) lies in [v_lo, v_hi), for every orbital pair: only those right-hand
+ * sides are carried through the LU, and the other entries of `out` are left as they were (v_lo =
+ * v_hi = 0: the whole vertex). The pairing transform of a range of bosonic frequencies
+ * [w0, w1) reads only the window [w0 + niv - np, niv + np).
  * Replaces the same reference code as dgami_full_vertex, for the pairing vertices of the
- * Eliashberg equation whose leading eigenvalue tolerates ~1e-6 relative error in F. */
-int dgami_full_vertex_prec(int nq, int nb, int nw, int niv, double beta, int precision, const dgami_c128* chi0_inv,
-                           const dgami_c128* gamma, const dgami_c128* u_r, const dgami_c128* v_r,
-                           const dgami_c128* vrg, const dgami_c128* chi_phys, dgami_c128* out, int32_t* info,
-                           void* stream);
+ * Eliashberg equation (precision 1 or 2 where the leading eigenvalue tolerates ~1e-6 relative
+ * error in F). */
+int dgami_full_vertex_prec(int nq, int nb, int nw, int niv, double beta, int precision, int v_lo, int v_hi,
+                           const dgami_c128* chi0_inv, const dgami_c128* gamma, const dgami_c128* u_r,
+                           const dgami_c128* v_r, const dgami_c128* vrg, const dgami_c128* chi_phys,
+                           dgami_c128* out, int32_t* info, void* stream);
 
 /* Singlet and triplet pairing vertices at w_pp = 0 from the particle-hole full vertices of the
  * density and magnetic channels (f_dens, f_magn: [nw][nq][n][n], half bosonic range, n = nb^2*2niv):

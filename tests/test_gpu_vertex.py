@@ -167,6 +167,38 @@ def test_pairing_vertex_chain(K):
     assert rel(gt, ref_t) < RTOL
 
 
+@pytest.mark.parametrize("precision", [0, 2])
+def test_pairing_vertices_from_ladder_window(K, precision):
+    """The Eliashberg build from both channels' ladders in chunks of bosonic frequencies: the full
+    vertices formed only in the frequency window each chunk's pp transform reads give the same
+    singlet and triplet vertices, bit for bit, as the whole full vertices; in complex128 both equal
+    the oracle chain."""
+    from dgamore_amd import eliashberg_solver as es
+
+    nb, niv, niw, nq, niv_pp = 2, 6, 5, 2, 4  # fermionic box wider than the pairing box: offset 2
+    say(f"pairing vertices from the ladder, window vs whole, precision={precision}")
+    cases = [ladder_case(nb, niv, niw, nq, seed=s) for s in (31, 32)]
+    chi0_inv, beta = cases[0][0], cases[0][4]
+    vc = [_vrg_chi(nb, chi0_inv.shape, seed=s) for s in (41, 42)]
+    d_inv = K.ref4_to_qf(K.to_dev(chi0_inv))
+    gam = torch.stack([K.vertex_to_cmp(K.to_dev(c[1])) for c in cases]).contiguous()
+    ur = torch.stack([K.orb_to_m(K.to_dev(c[2])) for c in cases]).contiguous()
+    vr = torch.stack([K.orb_to_m(K.to_dev(c[3])) for c in cases]).contiguous()
+    vrgs = [K.ref4_to_qf(K.to_dev(v)) for v, _ in vc]
+    chis = [K.ref4w_to_qw(K.to_dev(c)) for _, c in vc]
+    args = (d_inv, gam, ur, vr, vrgs, chis, beta, nb, niv_pp)
+    gs_w, gt_w = es.pairing_vertices_from_ladder(*args, w_chunk=2, precision=precision, window_rhs=True)
+    gs_f, gt_f = es.pairing_vertices_from_ladder(*args, w_chunk=2, precision=precision, window_rhs=False)
+    torch.cuda.synchronize()
+    assert torch.equal(gs_w, gs_f) and torch.equal(gt_w, gt_f)
+    if precision == 0:
+        refs = [O.vertex_pp_w0(O.compound_to_ref8(O.full_vertex_q(chi0_inv, c[1], c[2], c[3], v, x, beta), nb), niv_pp)
+                for c, (v, x) in zip(cases, vc)]
+        ref_s, ref_t = O.pairing_vertices(*refs)
+        assert rel(gs_w, ref_s) < RTOL
+        assert rel(gt_w, ref_t) < RTOL
+
+
 def test_pairing_vertex_fixture_combination(K):
     """Place the reference's pp vertices f_irrq_{dens,magn}_pp at their particle-hole positions
     (F_pp(i, j) = -F(w = i - j)[i][2np-1-j], the w < 0 half by conjugation), run the device
