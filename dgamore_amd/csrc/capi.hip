@@ -4,6 +4,7 @@
 
 #include "dgami.h"
 #include "dgami_internal.hpp"
+#include "dgami_launch.hpp"
 
 namespace dgami {
 
@@ -94,6 +95,55 @@ void* workspace(int slot, size_t bytes) {
   s.bytes = want;
   s.device = dev;
   return s.ptr;
+}
+
+namespace {
+struct ExtraStreams {
+  hipStream_t stream[MAX_EXTRA_STREAMS] = {};
+  hipEvent_t fork[MAX_EXTRA_STREAMS] = {}, join[MAX_EXTRA_STREAMS] = {};
+};
+std::mutex g_xs_mutex;
+std::vector<ExtraStreams> g_xs;  // per device
+}  // namespace
+
+int extra_stream_fork(hipStream_t caller, int k, hipStream_t* out) {
+  DGAMI_ARG_CHECK(k >= 1 && k <= MAX_EXTRA_STREAMS, "extra stream index out of range");
+  int dev = 0;
+  DGAMI_HIP_CHECK(hipGetDevice(&dev));
+  hipStream_t s = nullptr;
+  hipEvent_t fork = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_xs_mutex);
+    if (dev >= static_cast<int>(g_xs.size())) g_xs.resize(dev + 1);
+    ExtraStreams& x = g_xs[dev];
+    if (x.stream[k - 1] == nullptr) {
+      DGAMI_HIP_CHECK(hipStreamCreateWithFlags(&x.stream[k - 1], hipStreamNonBlocking));
+      DGAMI_HIP_CHECK(hipEventCreateWithFlags(&x.fork[k - 1], hipEventDisableTiming));
+      DGAMI_HIP_CHECK(hipEventCreateWithFlags(&x.join[k - 1], hipEventDisableTiming));
+    }
+    s = x.stream[k - 1];
+    fork = x.fork[k - 1];
+  }
+  DGAMI_HIP_CHECK(hipEventRecord(fork, caller));
+  DGAMI_HIP_CHECK(hipStreamWaitEvent(s, fork, 0));
+  *out = s;
+  return 0;
+}
+
+int extra_stream_join(hipStream_t caller, int k) {
+  DGAMI_ARG_CHECK(k >= 1 && k <= MAX_EXTRA_STREAMS, "extra stream index out of range");
+  int dev = 0;
+  DGAMI_HIP_CHECK(hipGetDevice(&dev));
+  hipStream_t s = nullptr;
+  hipEvent_t join = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(g_xs_mutex);
+    s = g_xs[dev].stream[k - 1];
+    join = g_xs[dev].join[k - 1];
+  }
+  DGAMI_HIP_CHECK(hipEventRecord(join, s));
+  DGAMI_HIP_CHECK(hipStreamWaitEvent(caller, join, 0));
+  return 0;
 }
 
 }  // namespace dgami

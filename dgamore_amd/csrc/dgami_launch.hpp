@@ -24,7 +24,18 @@ enum WorkspaceSlot {
   WS_SELFCONS = 13,  // Hartree-Fock transforms
   WS_MU = 14,        // chemical-potential secant: per-k occupations and the secant state
   WS_LADDER_G32 = 15, // complex64 scaled vertex of the fused first block of the blocked LU
+  // work issued on the library's extra stream k (1..MAX_EXTRA_STREAMS) uses its own slots:
+  // slot + k * WS_STREAM_STRIDE
+  WS_STREAM_STRIDE = 16,
 };
+constexpr int MAX_EXTRA_STREAMS = 3;
+
+// The library's extra stream k of the current device (created on first use, default priority,
+// not blocking the null stream): fork makes it wait for the work queued so far on `caller`,
+// join makes `caller` wait for the work queued on it, so that a captured caller stream takes
+// the forked work into its graph.
+int extra_stream_fork(hipStream_t caller, int k, hipStream_t* out);
+int extra_stream_join(hipStream_t caller, int k);
 
 int launch_fft_grid(const double2* in, double2* out, int64_t batch, int nx, int ny, int nz, int sign, double scale,
                     hipStream_t stream);

@@ -24,6 +24,7 @@
 #include "ladder_tune.hpp"
 #include "tri_solve.hpp"
 
+#include <algorithm>
 #include <cstdlib>
 #include <string>
 #include <type_traits>
@@ -1064,7 +1065,7 @@ static int blocked_factor_nb(const LadderArgs& a, BlockedState<S>& st, int item0
   if (fuse && c64) {
     const int g0 = item0 / a.nq, g1 = (item0 + nb - 1) / a.nq;
     float2* g32 = static_cast<float2*>(
-        workspace(WS_LADDER_G32, static_cast<size_t>(g1 - g0 + 1) * n * n * sizeof(float2)));
+        workspace(st.ws_g32, static_cast<size_t>(g1 - g0 + 1) * n * n * sizeof(float2)));
     if (!g32) return -2;
     hipLaunchKernelGGL(blk_vertex32_kernel, dim3(8192), dim3(256), 0, stream, a, g32, g0, g1 - g0 + 1);
     DGAMI_HIP_CHECK(hipGetLastError());
@@ -1156,35 +1157,62 @@ int blocked_gemm_c64(BlockedState<float2>& st, int nb, int r0, int r1, int c0, i
   return k.gemm(r0, r1, c0, c1, k0, kw);
 }
 
+int ladder_lanes() {
+  const char* v = getenv("DGAMI_LADDER_STREAMS");
+  const int s = v ? atoi(v) : 0;
+  return s >= 1 && s <= 1 + MAX_EXTRA_STREAMS ? s : 2;
+}
+
 int ladder_blocked_fp64(const LadderArgs& a, int variant, hipStream_t stream) {
   const int n = a.n, ncol = n + a.T;
   const int ld = (ncol + 3) & ~3;
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);
-  // MI355X holds 288 GB of HBM: factor up to 48 GiB of problems per pass
-  size_t batch = (size_t(48) << 30) / per;
+  const int nl = static_cast<int>(std::min<size_t>(ladder_lanes(), items));
+  // MI355X holds 288 GB of HBM: factor up to 48 GiB of problems per pass (over all lanes)
+  size_t batch = (size_t(48) << 30) / (per * nl);
   if (batch < 1) batch = 1;
-  if (batch > items) batch = items;
+  if (batch > (items + nl - 1) / nl) batch = (items + nl - 1) / nl;
   if (batch > 65535) batch = 65535;
-  BlockedState<double2> st;
-  if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
-  st.variant = variant;
   const int nblk = static_cast<int>(ceil_div(n, TRI_B));
-  double2* tri = static_cast<double2*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(double2)));
-  if (!tri) return -2;
-  for (size_t item0 = 0; item0 < items; item0 += batch) {
+  struct Lane {
+    BlockedState<double2> st;
+    double2* tri;
+    hipStream_t s;
+  };
+  Lane lanes[1 + MAX_EXTRA_STREAMS];
+  for (int l = 0; l < nl; ++l) {
+    Lane& L = lanes[l];
+    if (!blocked_alloc(L.st, n, ld, ncol, batch, WS_LADDER + l * WS_STREAM_STRIDE)) return -2;
+    L.st.variant = variant;
+    L.tri = static_cast<double2*>(
+        workspace(WS_TRI + l * WS_STREAM_STRIDE, batch * nblk * 2 * TRI_B * TRI_B * sizeof(double2)));
+    if (!L.tri) return -2;
+    L.s = stream;
+    if (l > 0) {
+      const int rc = extra_stream_fork(stream, l, &L.s);
+      if (rc) return rc;
+    }
+  }
+  int k = 0;
+  for (size_t item0 = 0; item0 < items; item0 += batch, k = (k + 1) % nl) {
+    Lane& L = lanes[k];
     const int nb = static_cast<int>(items - item0 < batch ? items - item0 : batch);
-    int rc = blocked_factor<double2>(a, st, static_cast<int>(item0), nb, stream);
+    int rc = blocked_factor<double2>(a, L.st, static_cast<int>(item0), nb, L.s);
     if (rc) return rc;
     // back substitution on the carried right-hand sides, x / beta straight to the output
-    TriFactors<double2> f{st.W, tri, st.dinv, n, ld, nblk};
-    rc = tri_diag_inverse(f, nb, false, stream);
+    TriFactors<double2> f{L.st.W, L.tri, L.st.dinv, n, ld, nblk};
+    rc = tri_diag_inverse(f, nb, false, L.s);
     if (rc) return rc;
-    TriVec<double2> y{st.W + n, static_cast<size_t>(n) * ld, ld};
-    rc = tri_backward(f, nb, a.T, y, TriVec<double2>{nullptr, 0, 0}, &a, static_cast<int>(item0), stream);
+    TriVec<double2> y{L.st.W + n, static_cast<size_t>(n) * ld, ld};
+    rc = tri_backward(f, nb, a.T, y, TriVec<double2>{nullptr, 0, 0}, &a, static_cast<int>(item0), L.s);
     if (rc) return rc;
-    rc = blocked_copy_info(st.info, a.info, static_cast<int>(item0), nb, stream);
+    rc = blocked_copy_info(L.st.info, a.info, static_cast<int>(item0), nb, L.s);
+    if (rc) return rc;
+  }
+  for (int l = 1; l < nl; ++l) {
+    const int rc = extra_stream_join(stream, l);
     if (rc) return rc;
   }
   return 0;

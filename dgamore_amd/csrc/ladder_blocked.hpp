@@ -2,6 +2,7 @@
 #pragma once
 
 #include "dgami_internal.hpp"
+#include "dgami_launch.hpp"
 #include "ladder_common.hpp"
 
 namespace dgami {
@@ -38,6 +39,7 @@ struct BlockedState {
   // groups in complex64 ([group - g32_first][n][n]), the fused update's source off the v = v' diagonal
   const float2* g32;
   int g32_first;
+  int ws_g32;  // workspace slot of the complex64 vertex copy
 };
 
 // Allocate the workspace of `batch` problems in slots ws and ws+1; returns false on failure.
@@ -58,6 +60,7 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
   st.src = LadderArgs{};
   st.g32 = nullptr;
   st.g32_first = 0;
+  st.ws_g32 = WS_LADDER_G32 + ws / WS_STREAM_STRIDE * WS_STREAM_STRIDE;  // the lane's slot range
   st.n = n;
   st.ld = ld;
   st.ncol = ncol;
@@ -70,6 +73,14 @@ inline bool blocked_alloc(BlockedState<S>& st, int n, int ld, int ncol, size_t b
 
 template <typename S>
 int blocked_factor(const LadderArgs& a, BlockedState<S>& st, int item0, int nb, hipStream_t stream);
+
+// Lanes of a batched blocked-LU solve: with L > 1 the problems are split into L parts solved on
+// the caller's stream and on the library's extra streams, each with its own workspace slots, so
+// that the latency-bound steps of one part (panels, interchanges, sweeps) overlap the
+// matrix-core or vector updates of another. Measured on the 3-orbital 24x24 workload (5642
+// problems of n = 720, split-BF16): 1 lane 100.9 ms, 2 lanes 97.5, 3 and 4 lanes 97.5-97.8, with
+// bit-identical results. DGAMI_LADDER_STREAMS (1..4) overrides the default of 2.
+int ladder_lanes();
 
 // W[r0:r1, c0:c1] -= W[r0:r1, k0:k0+kw] W[k0:k0+kw, c0:c1] on every complex128 problem of the
 // batch (the trailing-update kernel of the factorisation, reused by the block back-substitution).

@@ -269,61 +269,78 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
                           static_cast<size_t>(nblk) * 2 * TRI_B * TRI_B * sizeof(float2) +
                           2 * static_cast<size_t>(n) * T * sizeof(double2) + 2 * static_cast<size_t>(n) * T * sizeof(float2) +
                           static_cast<size_t>(n) * sizeof(int);
-  size_t gb = (size_t(48) << 30) / (per_item * gq);
-  gb = std::max<size_t>(1, std::min(gb, groups));
+  // lanes (ladder_lanes): the (channel, w) groups split over the caller's and the extra streams
+  const int nl = static_cast<int>(std::min<size_t>(ladder_lanes(), groups));
+  size_t gb = (size_t(48) << 30) / (per_item * gq * nl);
+  gb = std::max<size_t>(1, std::min(gb, (groups + nl - 1) / nl));
   gb = std::min<size_t>(gb, 65535 / static_cast<size_t>(gq) > 0 ? 65535 / gq : 1);
   const size_t bitems = gb * gq;
   DGAMI_ARG_CHECK(bitems <= 65535, "ladder: too many q-points in one (channel, w) group");
-
-  BlockedState<float2> st;
-  if (!blocked_alloc(st, n, ld, ncol, bitems, WS_MIXED)) return -2;
-  st.swap_left = 1;
-  st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
   const int tuned = ladder_tuned_kernel(n, T, precision);
-  if (tuned >= 0 && ladder_kernel_family(tuned) == LADDER_FAM_BLOCKED) st.variant = ladder_kernel_variant(tuned);
-  char* vec = static_cast<char*>(workspace(WS_MIXED_VEC, 2 * bitems * n * T * sizeof(double2) + bitems * n * sizeof(int) +
-                                                             2 * bitems * sizeof(int) + bitems * sizeof(float) +
-                                                             gb * sizeof(int)));
-  if (!vec) return -2;
-  MixedArgs m;
-  m.a = a;
-  m.W = st.W;
-  m.X = reinterpret_cast<double2*>(vec);
-  m.R = m.X + bitems * n * T;
-  int* perm = reinterpret_cast<int*>(m.R + bitems * n * T);
-  m.perm = perm;
-  m.status = perm + bitems * n;
-  m.gactive = m.status + bitems;
-  m.prev = reinterpret_cast<float*>(m.gactive + gb);
-  m.dinv = st.dinv;
-  m.ld = ld;
-  m.gq = gq;
-  float2* tri = static_cast<float2*>(workspace(WS_TRI, bitems * nblk * 2 * TRI_B * TRI_B * sizeof(float2)));
-  float2* yv = static_cast<float2*>(workspace(WS_TRI_VEC, 2 * bitems * n * T * sizeof(float2)));
-  if (!tri || !yv) return -2;
-  float2* xv = yv + bitems * n * T;
-  const TriVec<float2> Yv{yv, static_cast<size_t>(n) * T, T}, Xv{xv, static_cast<size_t>(n) * T, T};
 
-  for (size_t g0 = 0; g0 < groups; g0 += gb) {
+  // one lane per stream: factors, refinement vectors and sweep buffers in its own workspace slots
+  struct Lane {
+    BlockedState<float2> st;
+    MixedArgs m;
+    float2* tri;
+    float2* yv;
+    float2* xv;
+    int* perm;  // the permutation the perm kernel writes (m.perm reads it)
+    hipStream_t s;
+  };
+  auto setup = [&](Lane& L, int off, hipStream_t s) -> int {
+    if (!blocked_alloc(L.st, n, ld, ncol, bitems, WS_MIXED + off)) return -2;
+    L.st.swap_left = 1;
+    L.st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
+    if (tuned >= 0 && ladder_kernel_family(tuned) == LADDER_FAM_BLOCKED) L.st.variant = ladder_kernel_variant(tuned);
+    char* vec = static_cast<char*>(workspace(WS_MIXED_VEC + off, 2 * bitems * n * T * sizeof(double2) +
+                                                                    bitems * n * sizeof(int) + 2 * bitems * sizeof(int) +
+                                                                    bitems * sizeof(float) + gb * sizeof(int)));
+    if (!vec) return -2;
+    MixedArgs& m = L.m;
+    m.a = a;
+    m.W = L.st.W;
+    m.X = reinterpret_cast<double2*>(vec);
+    m.R = m.X + bitems * n * T;
+    int* perm = reinterpret_cast<int*>(m.R + bitems * n * T);
+    m.perm = perm;
+    L.perm = perm;
+    m.status = perm + bitems * n;
+    m.gactive = m.status + bitems;
+    m.prev = reinterpret_cast<float*>(m.gactive + gb);
+    m.dinv = L.st.dinv;
+    m.ld = ld;
+    m.gq = gq;
+    L.tri = static_cast<float2*>(workspace(WS_TRI + off, bitems * nblk * 2 * TRI_B * TRI_B * sizeof(float2)));
+    L.yv = static_cast<float2*>(workspace(WS_TRI_VEC + off, 2 * bitems * n * T * sizeof(float2)));
+    if (!L.tri || !L.yv) return -2;
+    L.xv = L.yv + bitems * n * T;
+    L.s = s;
+    return 0;
+  };
+  auto run = [&](Lane& L, size_t g0) -> int {
+    hipStream_t s = L.s;
+    MixedArgs& m = L.m;
+    BlockedState<float2>& st = L.st;
+    const TriVec<float2> Yv{L.yv, static_cast<size_t>(n) * T, T}, Xv{L.xv, static_cast<size_t>(n) * T, T};
     const int ng = static_cast<int>(std::min(gb, groups - g0));
     const int nit = ng * gq;
     const int item0 = static_cast<int>(g0 * gq);
     m.group0 = static_cast<int>(g0);
     m.item0 = item0;
-    int rc = blocked_factor<float2>(a, st, item0, nit, stream);
+    int rc = blocked_factor<float2>(a, st, item0, nit, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), stream, st.piv, perm, n, m.status,
-                       m.prev);
+    hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), s, st.piv, L.perm, n, m.status, m.prev);
     DGAMI_HIP_CHECK(hipGetLastError());
-    TriFactors<float2> f{st.W, tri, st.dinv, n, ld, nblk};
-    rc = tri_diag_inverse(f, nit, true, stream);
+    TriFactors<float2> f{st.W, L.tri, st.dinv, n, ld, nblk};
+    rc = tri_diag_inverse(f, nit, true, s);
     if (rc) return rc;
     f.skip = m.status;  // the sweeps leave converged items alone
     const dim3 vgrid(ceil_div(static_cast<size_t>(n) * T, 256), nit);
-    for (int s = 0; s < max_solves; ++s) {
-      if (s > 0) {
-        hipLaunchKernelGGL(mix_group_active_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, stream, m.status, m.gactive,
-                           gq, ng);
+    for (int sv = 0; sv < max_solves; ++sv) {
+      if (sv > 0) {
+        hipLaunchKernelGGL(mix_group_active_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, s, m.status, m.gactive, gq,
+                           ng);
         DGAMI_HIP_CHECK(hipGetLastError());
         ZgemmArgs g;
         g.A = a.gamma + static_cast<size_t>(g0) * n * n;
@@ -341,34 +358,49 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         g.alpha = make_double2(1.0, 0.0);
         g.beta = make_double2(0.0, 0.0);
         g.active = m.gactive;
-        rc = launch_zgemm(g, ng, stream);
+        rc = launch_zgemm(g, ng, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, stream, m);
+        hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, s, m);
         DGAMI_HIP_CHECK(hipGetLastError());
       }
       // D = U^-1 L^-1 P R in complex64, X += D; the first R is B, whose L^-1 P B the
       // factorisation left in columns [n, n + T)
-      if (s == 0) {
-        rc = tri_backward(f, nit, T, TriVec<float2>{st.W + n, static_cast<size_t>(n) * ld, ld}, Yv, nullptr, 0, stream);
+      if (sv == 0) {
+        rc = tri_backward(f, nit, T, TriVec<float2>{st.W + n, static_cast<size_t>(n) * ld, ld}, Yv, nullptr, 0, s);
         if (rc) return rc;
       } else {
-        hipLaunchKernelGGL(mix_gather_kernel, vgrid, dim3(256), 0, stream, m, yv, 0);
+        hipLaunchKernelGGL(mix_gather_kernel, vgrid, dim3(256), 0, s, m, L.yv, 0);
         DGAMI_HIP_CHECK(hipGetLastError());
-        rc = tri_forward(f, nit, T, Yv, Xv, stream);
+        rc = tri_forward(f, nit, T, Yv, Xv, s);
         if (rc) return rc;
-        rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, stream);
+        rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, s);
         if (rc) return rc;
       }
-      hipLaunchKernelGGL(mix_accumulate_kernel, dim3(nit), dim3(256), 0, stream, m, yv, s,
-                         s == max_solves - 1 ? 1 : 0, tol);
+      hipLaunchKernelGGL(mix_accumulate_kernel, dim3(nit), dim3(256), 0, s, m, L.yv, sv, sv == max_solves - 1 ? 1 : 0,
+                         tol);
       DGAMI_HIP_CHECK(hipGetLastError());
     }
     if (a.info) {
-      hipLaunchKernelGGL(mix_info_kernel, dim3(ceil_div(nit, 256)), dim3(256), 0, stream, st.info, m.status, a.info,
-                         item0, nit);
+      hipLaunchKernelGGL(mix_info_kernel, dim3(ceil_div(nit, 256)), dim3(256), 0, s, st.info, m.status, a.info, item0,
+                         nit);
       DGAMI_HIP_CHECK(hipGetLastError());
     }
+    return 0;
+  };
+
+  Lane lanes[1 + MAX_EXTRA_STREAMS];
+  int rc = setup(lanes[0], 0, stream);
+  if (rc) return rc;
+  for (int l = 1; l < nl; ++l) {
+    hipStream_t sx = nullptr;
+    if ((rc = extra_stream_fork(stream, l, &sx)) != 0) return rc;
+    if ((rc = setup(lanes[l], l * WS_STREAM_STRIDE, sx)) != 0) return rc;
   }
+  int k = 0;
+  for (size_t g0 = 0; g0 < groups; g0 += gb, k = (k + 1) % nl)
+    if ((rc = run(lanes[k], g0)) != 0) return rc;
+  for (int l = 1; l < nl; ++l)
+    if ((rc = extra_stream_join(stream, l)) != 0) return rc;
   return 0;
 }
 
