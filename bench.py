@@ -164,6 +164,12 @@ def eliashberg_solve():
         S.make_model(4, nk=nk, niw=niw, niv=niv, device=torch.device("cuda"))))
     front, t_front = timed(lambda: pipe.front(pipe.sigma_dmft))
     (aux, _), t_ladder = timed(lambda: K.ladder_solve_sum(front["chi0_inv"], pipe.gamma, pipe.u_r, pipe.v_r, m.beta))
+    from dgamore_amd import _lib
+
+    (aux_m, _), t_ladder_m = timed(lambda: K.ladder_solve_sum(front["chi0_inv"], pipe.gamma, pipe.u_r, pipe.v_r, m.beta,
+                                                             _lib.MIXED_BF16X3_IR))
+    ladder_err = float((aux_m - aux).abs().max() / aux.abs().max())
+    del aux_m
 
     def three_leg():
         out = [K.vrg_chi_phys(front["chi0_inv"], aux[c], front["full"], front["core"], pipe.u_r[c], None, m.nb, m.beta)
@@ -174,7 +180,7 @@ def eliashberg_solve():
     chi0pp = K.chi0_pp(front["g"], niv)
     solves = 2 * pipe.nq * (niw + 1)
     stages = {"model_setup_s": t_model, "front_s": t_front, "ladder_s": t_ladder, "ladder_solves": solves,
-              "three_leg_s": t_vrg}
+              "ladder_bf16x3_s": t_ladder_m, "ladder_bf16x3_rel_err": ladder_err, "three_leg_s": t_vrg}
     results = {}
     v0 = np.random.default_rng(7).standard_normal(48 * 48 * 4 * 2 * niv) + 0j
     for name, prec in (("c128", 0), ("bf16x3", 2)):

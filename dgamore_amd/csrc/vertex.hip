@@ -16,6 +16,7 @@
 #include "ladder_blocked.hpp"
 #include "tri_solve.hpp"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace dgami {
@@ -264,19 +265,38 @@ static int vertex_solve_t(LadderArgs a, int what, const VertexArgs& o_in, int ge
   o.z_form = c64 && what == VTX_OUT_FULL;
   const size_t items = static_cast<size_t>(a.nw) * a.nq;
   const size_t per = static_cast<size_t>(n) * ld * sizeof(S);
-  // up to 32 GiB of problems (factors and n right-hand sides) per pass
-  size_t batch = (size_t(32) << 30) / per;
+  const int nl = static_cast<int>(std::min<size_t>(ladder_lanes(), items));
+  // up to 32 GiB of problems (factors and n right-hand sides) per pass over all lanes
+  size_t batch = (size_t(32) << 30) / (per * nl);
   if (batch < 1) batch = 1;
-  if (batch > items) batch = items;
+  if (batch > (items + nl - 1) / nl) batch = (items + nl - 1) / nl;
   if (batch > 65535) batch = 65535;
-  BlockedState<S> st;
-  if (!blocked_alloc(st, n, ld, ncol, batch, WS_LADDER)) return -2;
-  if constexpr (std::is_same<S, float2>::value) st.gemm_mode = gemm_mode;
   const int nblk = static_cast<int>(ceil_div(n, TRI_B));
-  S* tri = static_cast<S*>(workspace(WS_TRI, batch * nblk * 2 * TRI_B * TRI_B * sizeof(S)));
-  if (!tri) return -2;
+  // lanes (ladder_lanes): parts of the problems on the caller's and the library's extra streams
+  struct Lane {
+    BlockedState<S> st;
+    S* tri;
+    hipStream_t s;
+  };
+  Lane lanes[1 + MAX_EXTRA_STREAMS];
+  for (int l = 0; l < nl; ++l) {
+    Lane& L = lanes[l];
+    if (!blocked_alloc(L.st, n, ld, ncol, batch, WS_LADDER + l * WS_STREAM_STRIDE)) return -2;
+    if constexpr (std::is_same<S, float2>::value) L.st.gemm_mode = gemm_mode;
+    L.tri = static_cast<S*>(workspace(WS_TRI + l * WS_STREAM_STRIDE, batch * nblk * 2 * TRI_B * TRI_B * sizeof(S)));
+    if (!L.tri) return -2;
+    L.s = stream;
+    if (l > 0) {
+      const int rc = extra_stream_fork(stream, l, &L.s);
+      if (rc) return rc;
+    }
+  }
   const int ntile = static_cast<int>(ceil_div(nr, 64));
-  for (size_t item0 = 0; item0 < items; item0 += batch) {
+  int lane = 0;
+  for (size_t item0 = 0; item0 < items; item0 += batch, lane = (lane + 1) % nl) {
+    BlockedState<S>& st = lanes[lane].st;
+    S* tri = lanes[lane].tri;
+    hipStream_t stream = lanes[lane].s;  // this part's stream
     const int nb = static_cast<int>(items - item0 < batch ? items - item0 : batch);
     int rc = blocked_factor<S>(a, st, static_cast<int>(item0), nb, stream);
     if (rc) return rc;
@@ -314,6 +334,10 @@ static int vertex_solve_t(LadderArgs a, int what, const VertexArgs& o_in, int ge
     }
     DGAMI_HIP_CHECK(hipGetLastError());
     rc = blocked_copy_info(st.info, a.info, static_cast<int>(item0), nb, stream);
+    if (rc) return rc;
+  }
+  for (int l = 1; l < nl; ++l) {
+    const int rc = extra_stream_join(stream, l);
     if (rc) return rc;
   }
   return 0;
