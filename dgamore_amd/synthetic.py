@@ -239,16 +239,16 @@ def local_vertices(g_loc, niw, niv, niv_full, beta, rng, rel=0.1, inter=0.05, de
 
 
 CONFIGS = {
-    # 1 band, 16x16, beta=10, niv=20, niw=20
+    # 0: 1 band, 16x16, beta=10, niv=20, niw=20 (small enough for the CPU restatement)
     0: dict(name="1band_16x16_b10_niv20", nb=1, nk=(16, 16, 1), beta=10.0, niw=20, niv=20, U=8.0, n=0.85,
             disp=("1band", 1.0, -0.2, 0.1)),
-    # 1 band, 64x64, beta=25, niv=60, niw=60
+    # 1: 1 band, 64x64, beta=25, niv=60, niw=60
     1: dict(name="1band_64x64_b25_niv60", nb=1, nk=(64, 64, 1), beta=25.0, niw=60, niv=60, U=8.0, n=0.85,
             disp=("1band", 1.0, -0.2, 0.1)),
-    # 3 orbitals (t2g-like), 24x24, beta=15, niv=40, niw=30, Kanamori U=4, J=0.6
+    # 2: 3 orbitals (t2g-like), 24x24, beta=15, niv=40, niw=30, Kanamori U=4, J=0.6 (the benchmark workload)
     2: dict(name="3orb_24x24_b15_niv40", nb=3, nk=(24, 24, 1), beta=15.0, niw=30, niv=40, U=4.0, J=0.6, n=0.85,
             disp=("t2g",)),
-    # 2 orbitals (e_g-like), 48x48, beta=20, U=5, J=0.8
+    # 4: 2 orbitals (e_g-like), 48x48, beta=20, U=5, J=0.8 (the Eliashberg workload)
     4: dict(name="2orb_48x48_b20", nb=2, nk=(48, 48, 1), beta=20.0, niw=64, niv=64, U=5.0, J=0.8, n=0.85,
             disp=("eg",)),
 }
@@ -287,7 +287,8 @@ def orbital_average(x, us, nidx):
 
 def make_model(cfg, seed=None, nk=None, niw=None, niv=None, niv_shell=0, with_vertices=True, device=None,
                symmetric=False):
-    """Build the synthetic model of a BASELINE config (optionally with smaller boxes/grids for tests).
+    """Build the synthetic model of one of the workloads in CONFIGS, or of a parameter dict from
+    batch_configs (optionally with smaller boxes/grids for tests).
     device: torch device for the per-frequency inverses of the local vertices (None: NumPy).
     symmetric: average the local self-energy and vertices over the orbital representation of the
     lattice's point group (the signed orbital permutations under which e_k is invariant), so
