@@ -282,11 +282,16 @@ __device__ __forceinline__ void fft_stage(double2* buf, int L, int nfft, int nf2
   const int step = L / (Ns * R);
   double2 v[MB][R];
   int boff[MB], dst[MB];  // each butterfly's transform base and output index, kept for the write
+  // butterfly b -> (transform f, butterfly j): along a strided axis consecutive threads take
+  // consecutive transforms, whose elements are adjacent in LDS (the butterfly-major order puts a
+  // whole wavefront on one bank there); along a contiguous axis consecutive butterflies
+  const bool fmajor = xstride > 1;
 #pragma unroll
   for (int c = 0; c < MB; ++c) {
     const int b = threadIdx.x + c * blockDim.x;
     if (b < total) {
-      const int f = b / nbf, j = b - f * nbf;
+      const int f = fmajor ? b % nfft : b / nbf;
+      const int j = fmajor ? b / nfft : b - f * nbf;
       const int jq = j / Ns, k = j - jq * Ns;
       const int fq = f / nf2;
       boff[c] = fq * s1 + (f - fq * nf2) * s2;
