@@ -145,6 +145,102 @@ __global__ void __launch_bounds__(256) mix_residual_kernel(MixedArgs m) {
   }
 }
 
+// The same residual with the q-points of a (channel, w) group as the fastest index, so that
+// X, R and chi0^-1 (whose q is also fastest) are read in whole cache lines. It runs in two
+// kernels: mix_vsum_kernel forms the U_r/V_r term, and mix_residual_rows_kernel forms one row i
+// of every item of the group. The sums run in the same order as in mix_residual_kernel, so
+// the results are the same bit for bit.
+//
+// UVS[g][ri][q][t] = sum_c (U_r + V_r(q))[ri][c] S[c][q][t], S[c][q][t] = sum_nu X[(c, nu)][q][t].
+// grid (ceil(gq*T/128), groups of the batch), one thread per (q, t).
+template <int T>
+__global__ void __launch_bounds__(128) mix_vsum_kernel(MixedArgs m, double2* __restrict__ uvs) {
+  const LadderArgs& a = m.a;
+  const int gl = blockIdx.y;
+  if (!m.gactive[gl]) return;
+  const int nv = a.nv, gq = m.gq, n = a.n;
+  const int e = blockIdx.x * 128 + threadIdx.x;
+  if (e >= gq * T) return;
+  const int q = e / T;
+  const int chan = (m.group0 + gl) / a.nw;
+  const size_t rstride = static_cast<size_t>(gq) * T;  // one row of X
+  const double2* X = m.X + static_cast<size_t>(gl) * n * rstride + e;
+  double2 s[T];
+#pragma unroll
+  for (int c = 0; c < T; ++c) s[c] = make_double2(0.0, 0.0);
+  for (int v = 0; v < nv; ++v) {
+#pragma unroll
+    for (int c = 0; c < T; ++c) s[c] = cadd(s[c], X[static_cast<size_t>(c * nv + v) * rstride]);
+  }
+  const double2* u = a.u_r + static_cast<size_t>(chan) * T * T;
+  const double2* vr = a.v_r ? a.v_r + (static_cast<size_t>(chan) * a.nq + q) * T * T : nullptr;
+  double2* out = uvs + static_cast<size_t>(gl) * T * rstride + e;
+  for (int ri = 0; ri < T; ++ri) {
+    double2 acc = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int c = 0; c < T; ++c) {
+      double2 uv = u[ri * T + c];
+      if (vr) uv = cadd(uv, vr[ri * T + c]);
+      acc = cfma(acc, uv, s[c]);
+    }
+    out[static_cast<size_t>(ri) * rstride] = acc;
+  }
+}
+
+// R[i][q][t] = B - A X for row i = (ri, vi) of every active item of a group, given R = Gamma X
+// and UVS. grid (ceil(gq*T/256), n rows ordered (vi, ri), groups), one thread per (q, t): the
+// T rows that share vi read the same rows of X and run next to each other.
+template <int T>
+__global__ void __launch_bounds__(256) mix_residual_rows_kernel(MixedArgs m, const double2* __restrict__ uvs) {
+  const LadderArgs& a = m.a;
+  const int gl = blockIdx.z;
+  if (!m.gactive[gl]) return;
+  const int nv = a.nv, gq = m.gq, n = a.n;
+  const int e = blockIdx.x * 256 + threadIdx.x;
+  if (e >= gq * T) return;
+  const int q = e / T, t = e - q * T;
+  if (m.status[gl * gq + q] != MIX_ACTIVE) return;
+  const int vi = blockIdx.y / T, ri = blockIdx.y - vi * T;
+  const int i = ri * nv + vi;
+  const int w = (m.group0 + gl) % a.nw;
+  const size_t rstride = static_cast<size_t>(gq) * T;
+  const double2* X = m.X + static_cast<size_t>(gl) * n * rstride + e;
+  const double2* ci = a.chi0_inv + ((static_cast<size_t>(w) * T + ri) * T * nv + vi) * a.nq + q;
+  double2 cx = make_double2(0.0, 0.0);
+#pragma unroll
+  for (int c = 0; c < T; ++c)
+    cx = cfma(cx, ci[static_cast<size_t>(c) * nv * a.nq], X[static_cast<size_t>(c * nv + vi) * rstride]);
+  const size_t o = (static_cast<size_t>(gl) * n + i) * rstride + e;
+  const double2 us = uvs[(static_cast<size_t>(gl) * T + ri) * rstride + e];
+  const double2 ax = cadd(cx, cscale(csub(m.R[o], us), a.inv_beta2));
+  m.R[o] = csub(ladder_rhs(a, i, t), ax);
+}
+
+// The residual of the items of a batch; uvs holds ng * T * gq * T complex128 values.
+static int mix_residual(const MixedArgs& m, int ng, int nit, double2* uvs, hipStream_t s) {
+  const LadderArgs& a = m.a;
+  const int T = a.T, gq = m.gq;
+  auto rows = [&](auto kv, auto kr) -> int {
+    hipLaunchKernelGGL(kv, dim3(ceil_div(gq * T, 128), ng), dim3(128), 0, s, m, uvs);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    hipLaunchKernelGGL(kr, dim3(ceil_div(gq * T, 256), a.n, ng), dim3(256), 0, s, m, uvs);
+    DGAMI_HIP_CHECK(hipGetLastError());
+    return 0;
+  };
+  if (!a.dense) {
+    switch (T) {
+      case 1: return rows(mix_vsum_kernel<1>, mix_residual_rows_kernel<1>);
+      case 4: return rows(mix_vsum_kernel<4>, mix_residual_rows_kernel<4>);
+      case 9: return rows(mix_vsum_kernel<9>, mix_residual_rows_kernel<9>);
+      case 16: return rows(mix_vsum_kernel<16>, mix_residual_rows_kernel<16>);
+      default: break;
+    }
+  }
+  hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, s, m);
+  DGAMI_HIP_CHECK(hipGetLastError());
+  return 0;
+}
+
 // ---------------------------------------------------------------------------
 // Y = P R in complex64 (R = B on the first solve). grid (ceil(n*T/256), batch).
 __global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __restrict__ Y, int first) {
@@ -360,8 +456,9 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         g.active = m.gactive;
         rc = launch_zgemm(g, ng, s);
         if (rc) return rc;
-        hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, s, m);
-        DGAMI_HIP_CHECK(hipGetLastError());
+        // the sweep buffers are free until the gather below: they hold the U_r/V_r term
+        rc = mix_residual(m, ng, nit, reinterpret_cast<double2*>(L.yv), s);
+        if (rc) return rc;
       }
       // D = U^-1 L^-1 P R in complex64, X += D; the first R is B, whose L^-1 P B the
       // factorisation left in columns [n, n + T)
