@@ -1165,7 +1165,8 @@ int ladder_lanes() {
 
 int ladder_blocked_fp64(const LadderArgs& a, int variant, hipStream_t stream) {
   const int n = a.n, ncol = n + a.T;
-  const int ld = (ncol + 3) & ~3;
+  // rows start on 128-byte boundaries (8 complex128 values)
+  const int ld = (ncol + 7) & ~7;
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const size_t per = static_cast<size_t>(n) * ld * sizeof(double2);

@@ -355,7 +355,9 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
   DGAMI_ARG_CHECK(n <= BLK_MAX_N, "ladder: compound size above 1536 is not supported");
   // the factorisation carries the nb^2 indicator columns, so the first solve needs only the
   // backward sweep
-  const int ncol = n + T, ld = (ncol + 7) & ~7;
+  // rows start on 128-byte boundaries (16 complex64 values), so a row's 32-column block segment
+  // spans two cache lines, not three
+  const int ncol = n + T, ld = (ncol + 15) & ~15;
   const size_t items = static_cast<size_t>(a.nchan) * a.nw * a.nq;
   const int gq = a.dense ? 1 : a.nq;
   const size_t groups = items / gq;
