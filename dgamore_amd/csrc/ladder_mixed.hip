@@ -258,7 +258,7 @@ __global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __
       first ? from_c128<float2>(ladder_rhs(a, src, t)) : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
 }
 
-// X += D (FP64) and out = X / beta for the active items, then the item's convergence test
+// X += D (FP64) for the active items, then the item's convergence test
 // from its largest |D| and |X| (|Re|+|Im|): r = |D|/|X| estimates the relative error before this
 // correction and r / r_prev the contraction of one refinement step, so the error after it is
 // predicted as r * contraction; the item converges when that is below tol. A non-finite D or X
@@ -271,8 +271,6 @@ __global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const 
   if (m.status[lb] != MIX_ACTIVE) return;
   const int gl = lb / m.gq, q = lb - gl * m.gq;
   const int n = a.n, T = a.T, gq = m.gq;
-  const int item = m.item0 + lb;
-  const int qt = item % a.nq, w = (item / a.nq) % a.nw, chan = item / (a.nq * a.nw);
   double2* X = m.X + static_cast<size_t>(gl) * n * gq * T;
   const float2* Db = D + static_cast<size_t>(lb) * n * T;
   float dmax = 0.f, xmax = 0.f;
@@ -283,7 +281,6 @@ __global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const 
     const double2 d = to_c128(Db[idx]);
     const double2 x = step == 0 ? d : cadd(X[o], d);
     X[o] = x;
-    a.out[ladder_out_index(a, chan, w, qt, i, t)] = cscale(x, a.inv_beta);
     const float dv = static_cast<float>(cabs1(d)), xv = static_cast<float>(cabs1(x));
     bad |= !(dv <= 3.0e38f) || !(xv <= 3.0e38f);  // NaN or infinite (fmaxf would drop a NaN)
     dmax = fmaxf(dmax, dv);
@@ -317,6 +314,29 @@ __global__ void __launch_bounds__(256) mix_accumulate_kernel(MixedArgs m, const 
     }
     m.prev[lb] = static_cast<float>(r);
   }
+}
+
+// out = X / beta for every item of the batch, after the last refinement step (an item's X
+// stops changing when it leaves the active state). grid (n, groups of the batch): one row i of
+// every item of a group, written with q fastest as ladder_out_index orders it.
+__global__ void __launch_bounds__(256) mix_output_kernel(MixedArgs m) {
+  const LadderArgs& a = m.a;
+  const int i = blockIdx.x, gl = blockIdx.y;
+  const int n = a.n, T = a.T, gq = m.gq;
+  const double2* X = m.X + (static_cast<size_t>(gl) * n + i) * gq * T;
+  const int g = m.group0 + gl, w = g % a.nw, chan = g / a.nw;
+  for (int e = threadIdx.x; e < gq * T; e += 256) {
+    const int t = e / gq, q = e - t * gq;
+    a.out[ladder_out_index(a, chan, w, q, i, t)] = cscale(X[q * T + t], a.inv_beta);
+  }
+}
+
+// The same in dense mode, where X of the batch and out share the [item][n][T] layout.
+__global__ void __launch_bounds__(256) mix_output_dense_kernel(MixedArgs m, size_t count) {
+  const size_t e = static_cast<size_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (e >= count) return;
+  const LadderArgs& a = m.a;
+  a.out[static_cast<size_t>(m.item0) * a.n * a.T + e] = cscale(m.X[e], a.inv_beta);
 }
 
 // gactive[g] = 1 iff an item of group g of the batch is still active (one thread per group).
@@ -479,6 +499,13 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
                          tol);
       DGAMI_HIP_CHECK(hipGetLastError());
     }
+    if (a.dense) {
+      const size_t count = static_cast<size_t>(nit) * n * T;
+      hipLaunchKernelGGL(mix_output_dense_kernel, dim3(ceil_div(count, 256)), dim3(256), 0, s, m, count);
+    } else {
+      hipLaunchKernelGGL(mix_output_kernel, dim3(n, ng), dim3(256), 0, s, m);
+    }
+    DGAMI_HIP_CHECK(hipGetLastError());
     if (a.info) {
       hipLaunchKernelGGL(mix_info_kernel, dim3(ceil_div(nit, 256)), dim3(256), 0, s, st.info, m.status, a.info, item0,
                          nit);

@@ -254,10 +254,12 @@ template <typename S>
 static int vertex_solve_t(LadderArgs a, int what, const VertexArgs& o_in, int gemm_mode, hipStream_t stream) {
   DGAMI_ARG_CHECK(o_in.vw == 0 || what == VTX_OUT_FULL, "vertex: a frequency window needs the full vertex");
   const int n = a.n, nr = o_in.vw > 0 ? a.T * o_in.vw : n, ncol = n + nr;
-  const int ld = (ncol + 3) & ~3;
-  DGAMI_ARG_CHECK(n <= BLK_MAX_N, "vertex: compound size above 1536 is not supported");
   // complex64 factorisations carry the vertex part as right-hand sides (no cancellation in F)
   constexpr bool c64 = std::is_same<S, float2>::value;
+  // rows start on 128-byte boundaries
+  constexpr int align = c64 ? 16 : 8;
+  const int ld = (ncol + align - 1) & ~(align - 1);
+  DGAMI_ARG_CHECK(n <= BLK_MAX_N, "vertex: compound size above 1536 is not supported");
   a.rhs_mode = what == VTX_OUT_FULL ? (c64 ? LADDER_RHS_VERTEX : LADDER_RHS_CHI0INV) : LADDER_RHS_IDENTITY;
   a.rhs_vlo = o_in.vlo;
   a.rhs_w = o_in.vw;
