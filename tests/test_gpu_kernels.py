@@ -212,6 +212,20 @@ def test_ladder_mixed_split_bf16_ir(K, nb, niv, niw, nq):
     assert rel(got[0], ref) < MIXED_RTOL
 
 
+@pytest.mark.parametrize("niv,nq", [(2, 3), (6, 2)])   # n = 64 (register kernel), n = 192 (blocked LU)
+@pytest.mark.parametrize("precision", [0, 1, 2])
+def test_ladder_four_orbitals_every_form(K, niv, nq, precision):
+    """nb = 4, the largest orbital count: 16 right-hand sides per problem, in the register
+    kernel and in every form of the blocked LU."""
+    nb, niw = 4, 1
+    say(f"ladder nb=4 niv={niv} niw={niw} nq={nq} n={nb*nb*2*niv} precision={precision}")
+    chi0_inv, gamma, u, v, beta = _ladder_case(nb, niv, niw, nq, seed=400 + niv)
+    ref = O.ladder_sum_solve(chi0_inv, gamma, u, v, beta)
+    got, info = _run_ladder(K, chi0_inv, [gamma], [u], [v], beta, precision=precision)
+    assert int(info.abs().sum()) == 0
+    assert rel(got[0], ref) < (RTOL if precision == 0 else MIXED_RTOL)
+
+
 def test_ladder_unknown_precision_is_refused(K):
     say("ladder precision 7 -> error")
     chi0_inv, gamma, u, v, beta = _ladder_case(1, 4, 1, 2, seed=5)
