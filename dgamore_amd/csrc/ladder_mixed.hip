@@ -58,6 +58,7 @@ struct MixedArgs {
   LadderArgs a;
   const float2* W;      // factors [item][n][ld]
   const int* perm;      // [item][n] row i of P r is r[perm[i]]
+  const int* pinv;      // [item][n] row j of r is row pinv[j] of P r
   const double2* dinv;  // [item][n] inverse pivots
   double2* X;           // [group][n][gq][T]
   double2* R;           // same
@@ -68,10 +69,11 @@ struct MixedArgs {
 };
 
 // ---------------------------------------------------------------------------
-// Row permutation of each factorisation from its sequence of interchanges.
+// Row permutation of each factorisation from its sequence of interchanges, and its inverse.
 // Also starts the item's refinement (status active, no previous correction).
-__global__ void __launch_bounds__(64) mix_perm_kernel(const int* __restrict__ piv, int* __restrict__ perm, int n,
-                                                      int* __restrict__ status, float* __restrict__ prev) {
+__global__ void __launch_bounds__(64) mix_perm_kernel(const int* __restrict__ piv, int* __restrict__ perm,
+                                                      int* __restrict__ pinv, int n, int* __restrict__ status,
+                                                      float* __restrict__ prev) {
   extern __shared__ int s_perm[];
   const int b = blockIdx.x;
   if (threadIdx.x == 0) {
@@ -90,7 +92,10 @@ __global__ void __launch_bounds__(64) mix_perm_kernel(const int* __restrict__ pi
     }
   }
   __syncthreads();
-  for (int i = threadIdx.x; i < n; i += 64) perm[static_cast<size_t>(b) * n + i] = s_perm[i];
+  for (int i = threadIdx.x; i < n; i += 64) {
+    perm[static_cast<size_t>(b) * n + i] = s_perm[i];
+    pinv[static_cast<size_t>(b) * n + s_perm[i]] = i;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -145,11 +150,29 @@ __global__ void __launch_bounds__(256) mix_residual_kernel(MixedArgs m) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// Y = P R in complex64 (R = B on the first solve). grid (ceil(n*T/256), batch).
+__global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __restrict__ Y, int first) {
+  const LadderArgs& a = m.a;
+  const int lb = blockIdx.y;
+  if (m.status[lb] != MIX_ACTIVE) return;
+  const int gl = lb / m.gq, q = lb - gl * m.gq;
+  const int n = a.n, T = a.T, gq = m.gq;
+  const int idx = blockIdx.x * 256 + threadIdx.x;
+  if (idx >= n * T) return;
+  const int i = idx / T, t = idx - (idx / T) * T;
+  const int src = m.perm[static_cast<size_t>(lb) * n + i];
+  const double2* R = m.R + static_cast<size_t>(gl) * n * gq * T;
+  Y[static_cast<size_t>(lb) * n * T + idx] =
+      first ? from_c128<float2>(ladder_rhs(a, src, t)) : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
+}
+
 // The same residual with the q-points of a (channel, w) group as the fastest index, so that
 // X, R and chi0^-1 (whose q is also fastest) are read in whole cache lines. It runs in two
 // kernels: mix_vsum_kernel forms the U_r/V_r term, and mix_residual_rows_kernel forms one row i
-// of every item of the group. The sums run in the same order as in mix_residual_kernel, so
-// the results are the same bit for bit.
+// of every item of the group and writes it, in complex64, to its row of Y = P R (the
+// sweeps' input), so R is not written back and no gather runs. The sums run in the same order
+// as in mix_residual_kernel, so the results are the same bit for bit.
 //
 // UVS[g][ri][q][t] = sum_c (U_r + V_r(q))[ri][c] S[c][q][t], S[c][q][t] = sum_nu X[(c, nu)][q][t].
 // grid (ceil(gq*T/128), groups of the batch), one thread per (q, t).
@@ -187,20 +210,24 @@ __global__ void __launch_bounds__(128) mix_vsum_kernel(MixedArgs m, double2* __r
   }
 }
 
-// R[i][q][t] = B - A X for row i = (ri, vi) of every active item of a group, given R = Gamma X
-// and UVS. grid (ceil(gq*T/256), n rows ordered (vi, ri), groups), one thread per (q, t): the
-// T rows that share vi read the same rows of X and run next to each other.
+// Y[item][pinv[i]][t] = B - A X for row i = (ri, vi) of every active item of a group, given
+// R = Gamma X and UVS. One thread per (q, t); the workgroups of one (group, vi), whose T rows
+// read the same rows of X, run on one XCD (xcd_tile over (group, vi) x (ri, chunk of (q, t))).
 template <int T>
-__global__ void __launch_bounds__(256) mix_residual_rows_kernel(MixedArgs m, const double2* __restrict__ uvs) {
+__global__ void __launch_bounds__(256) mix_residual_rows_kernel(MixedArgs m, const double2* __restrict__ uvs,
+                                                                float2* __restrict__ Y, int ng, int xb) {
   const LadderArgs& a = m.a;
-  const int gl = blockIdx.z;
-  if (!m.gactive[gl]) return;
   const int nv = a.nv, gq = m.gq, n = a.n;
-  const int e = blockIdx.x * 256 + threadIdx.x;
+  int unit, tile;
+  if (!xcd_tile(blockIdx.x, ng * nv, T * xb, unit, tile)) return;
+  const int gl = unit / nv, vi = unit - gl * nv;
+  const int ri = tile / xb, bx = tile - ri * xb;
+  if (!m.gactive[gl]) return;
+  const int e = bx * 256 + threadIdx.x;
   if (e >= gq * T) return;
   const int q = e / T, t = e - q * T;
-  if (m.status[gl * gq + q] != MIX_ACTIVE) return;
-  const int vi = blockIdx.y / T, ri = blockIdx.y - vi * T;
+  const int lb = gl * gq + q;
+  if (m.status[lb] != MIX_ACTIVE) return;
   const int i = ri * nv + vi;
   const int w = (m.group0 + gl) % a.nw;
   const size_t rstride = static_cast<size_t>(gq) * T;
@@ -213,21 +240,27 @@ __global__ void __launch_bounds__(256) mix_residual_rows_kernel(MixedArgs m, con
   const size_t o = (static_cast<size_t>(gl) * n + i) * rstride + e;
   const double2 us = uvs[(static_cast<size_t>(gl) * T + ri) * rstride + e];
   const double2 ax = cadd(cx, cscale(csub(m.R[o], us), a.inv_beta2));
-  m.R[o] = csub(ladder_rhs(a, i, t), ax);
+  const int row = m.pinv[static_cast<size_t>(lb) * n + i];
+  Y[(static_cast<size_t>(lb) * n + row) * T + t] = from_c128<float2>(csub(ladder_rhs(a, i, t), ax));
 }
 
-// The residual of the items of a batch; uvs holds ng * T * gq * T complex128 values.
-static int mix_residual(const MixedArgs& m, int ng, int nit, double2* uvs, hipStream_t s) {
+// The residual of the items of a batch and Y = P R in complex64, in yv. uvs (the forward
+// sweep's output buffer, free until that sweep) holds ng * T * gq * T complex128 values.
+static int mix_residual(const MixedArgs& m, int ng, int nit, double2* uvs, float2* yv, hipStream_t s) {
   const LadderArgs& a = m.a;
   const int T = a.T, gq = m.gq;
   auto rows = [&](auto kv, auto kr) -> int {
     hipLaunchKernelGGL(kv, dim3(ceil_div(gq * T, 128), ng), dim3(128), 0, s, m, uvs);
     DGAMI_HIP_CHECK(hipGetLastError());
-    hipLaunchKernelGGL(kr, dim3(ceil_div(gq * T, 256), a.n, ng), dim3(256), 0, s, m, uvs);
+    const int xb = static_cast<int>(ceil_div(gq * T, 256));
+    const size_t grid = xcd_grid(static_cast<size_t>(ng) * a.nv, static_cast<size_t>(T) * xb);
+    DGAMI_ARG_CHECK(grid < (size_t(1) << 31), "ladder: residual grid too large");
+    hipLaunchKernelGGL(kr, dim3(static_cast<unsigned>(grid)), dim3(256), 0, s, m, uvs, yv, ng, xb);
     DGAMI_HIP_CHECK(hipGetLastError());
     return 0;
   };
-  if (!a.dense) {
+  // uvs fits the forward sweep's buffer (bitems * n * T complex64) since n = T * nv with nv >= 2
+  if (!a.dense && a.nv >= 2) {
     switch (T) {
       case 1: return rows(mix_vsum_kernel<1>, mix_residual_rows_kernel<1>);
       case 4: return rows(mix_vsum_kernel<4>, mix_residual_rows_kernel<4>);
@@ -238,24 +271,9 @@ static int mix_residual(const MixedArgs& m, int ng, int nit, double2* uvs, hipSt
   }
   hipLaunchKernelGGL(mix_residual_kernel, dim3(nit), dim3(256), 0, s, m);
   DGAMI_HIP_CHECK(hipGetLastError());
+  hipLaunchKernelGGL(mix_gather_kernel, dim3(ceil_div(static_cast<size_t>(a.n) * T, 256), nit), dim3(256), 0, s, m, yv, 0);
+  DGAMI_HIP_CHECK(hipGetLastError());
   return 0;
-}
-
-// ---------------------------------------------------------------------------
-// Y = P R in complex64 (R = B on the first solve). grid (ceil(n*T/256), batch).
-__global__ void __launch_bounds__(256) mix_gather_kernel(MixedArgs m, float2* __restrict__ Y, int first) {
-  const LadderArgs& a = m.a;
-  const int lb = blockIdx.y;
-  if (m.status[lb] != MIX_ACTIVE) return;
-  const int gl = lb / m.gq, q = lb - gl * m.gq;
-  const int n = a.n, T = a.T, gq = m.gq;
-  const int idx = blockIdx.x * 256 + threadIdx.x;
-  if (idx >= n * T) return;
-  const int i = idx / T, t = idx - (idx / T) * T;
-  const int src = m.perm[static_cast<size_t>(lb) * n + i];
-  const double2* R = m.R + static_cast<size_t>(gl) * n * gq * T;
-  Y[static_cast<size_t>(lb) * n * T + idx] =
-      first ? from_c128<float2>(ladder_rhs(a, src, t)) : from_c128<float2>(R[(static_cast<size_t>(src) * gq + q) * T + t]);
 }
 
 // X += D (FP64) for the active items, then the item's convergence test
@@ -404,6 +422,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     float2* yv;
     float2* xv;
     int* perm;  // the permutation the perm kernel writes (m.perm reads it)
+    int* pinv;  // and its inverse (m.pinv)
     hipStream_t s;
   };
   auto setup = [&](Lane& L, int off, hipStream_t s) -> int {
@@ -412,7 +431,7 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     L.st.gemm_mode = precision == DGAMI_MIXED_BF16X3_IR ? BLK_GEMM_BF16X3 : BLK_GEMM_F32_MFMA;
     if (tuned >= 0 && ladder_kernel_family(tuned) == LADDER_FAM_BLOCKED) L.st.variant = ladder_kernel_variant(tuned);
     char* vec = static_cast<char*>(workspace(WS_MIXED_VEC + off, 2 * bitems * n * T * sizeof(double2) +
-                                                                    bitems * n * sizeof(int) + 2 * bitems * sizeof(int) +
+                                                                    2 * bitems * n * sizeof(int) + 2 * bitems * sizeof(int) +
                                                                     bitems * sizeof(float) + gb * sizeof(int)));
     if (!vec) return -2;
     MixedArgs& m = L.m;
@@ -423,7 +442,9 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     int* perm = reinterpret_cast<int*>(m.R + bitems * n * T);
     m.perm = perm;
     L.perm = perm;
-    m.status = perm + bitems * n;
+    L.pinv = perm + bitems * n;
+    m.pinv = L.pinv;
+    m.status = L.pinv + bitems * n;
     m.gactive = m.status + bitems;
     m.prev = reinterpret_cast<float*>(m.gactive + gb);
     m.dinv = L.st.dinv;
@@ -448,13 +469,13 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
     m.item0 = item0;
     int rc = blocked_factor<float2>(a, st, item0, nit, s);
     if (rc) return rc;
-    hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), s, st.piv, L.perm, n, m.status, m.prev);
+    hipLaunchKernelGGL(mix_perm_kernel, dim3(nit), dim3(64), n * sizeof(int), s, st.piv, L.perm, L.pinv, n, m.status,
+                       m.prev);
     DGAMI_HIP_CHECK(hipGetLastError());
     TriFactors<float2> f{st.W, L.tri, st.dinv, n, ld, nblk};
     rc = tri_diag_inverse(f, nit, true, s);
     if (rc) return rc;
     f.skip = m.status;  // the sweeps leave converged items alone
-    const dim3 vgrid(ceil_div(static_cast<size_t>(n) * T, 256), nit);
     for (int sv = 0; sv < max_solves; ++sv) {
       if (sv > 0) {
         hipLaunchKernelGGL(mix_group_active_kernel, dim3(ceil_div(ng, 256)), dim3(256), 0, s, m.status, m.gactive, gq,
@@ -478,8 +499,8 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         g.active = m.gactive;
         rc = launch_zgemm(g, ng, s);
         if (rc) return rc;
-        // the sweep buffers are free until the gather below: they hold the U_r/V_r term
-        rc = mix_residual(m, ng, nit, reinterpret_cast<double2*>(L.yv), s);
+        // Y = P R into yv; xv, free until the forward sweep, holds the U_r/V_r term
+        rc = mix_residual(m, ng, nit, reinterpret_cast<double2*>(L.xv), L.yv, s);
         if (rc) return rc;
       }
       // D = U^-1 L^-1 P R in complex64, X += D; the first R is B, whose L^-1 P B the
@@ -488,8 +509,6 @@ int ladder_solve_mixed(const LadderArgs& a, int precision, hipStream_t stream) {
         rc = tri_backward(f, nit, T, TriVec<float2>{st.W + n, static_cast<size_t>(n) * ld, ld}, Yv, nullptr, 0, s);
         if (rc) return rc;
       } else {
-        hipLaunchKernelGGL(mix_gather_kernel, vgrid, dim3(256), 0, s, m, L.yv, 0);
-        DGAMI_HIP_CHECK(hipGetLastError());
         rc = tri_forward(f, nit, T, Yv, Xv, s);
         if (rc) return rc;
         rc = tri_backward(f, nit, T, Xv, Yv, nullptr, 0, s);
