@@ -1,7 +1,7 @@
 """DMFT input from w2dynamics output files (dmft_interface.py:125-330).
 
 W2dynInterface keeps the reference's method names, arguments and return shapes; it reads the
-files with the module's own HDF5 parser (hdf5_read), since h5py is not part of this build.
+files with the package's own HDF5 parser (hdf5_read), so that h5py is not a dependency.
 The reference wraps G and Sigma in GreensFunction / SelfEnergy objects; here they are the
 arrays those objects hold: get_giw -> [nb, nb, 2 niv_dmft] and get_siw -> [1, 1, 1, nb, nb,
 2 niv_dmft] (double counting added), both diagonal in the orbitals, spin-averaged.

@@ -1,7 +1,7 @@
 """Read-only access to HDF5 files in the layout h5py writes by default, without h5py.
 
 The DMFT input of the reference (w2dynamics output, read with h5py in
-dmft_interface.py:125-262) is an HDF5 file. h5py is not part of this build, so this module
+dmft_interface.py:125-262) is an HDF5 file. So that the package needs no h5py, this module
 parses the file format itself, for the structures such files use (those of the reference's
 end_2_end/1p-data.hdf5): superblock version 0, groups as symbol tables (version-1 B-trees of
 symbol-table nodes with a local heap of names, soft links), version-1 object headers with their
