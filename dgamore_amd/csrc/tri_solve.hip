@@ -1,4 +1,6 @@
 // Batched blocked triangular solves (see tri_solve.hpp for the scheme).
+#include <type_traits>
+
 #include "dgami.h"
 #include "dgami_internal.hpp"
 #include "dgami_launch.hpp"
@@ -88,8 +90,8 @@ __device__ __forceinline__ void load_pair<double2>(const double2* src, double2& 
 // in LDS, the owners of block b keep x_b, and every thread updates its rows below (L) or
 // above (U) block b with its own row of the factor's block column.
 template <typename S, int T, bool LOWER, bool TO_LADDER, int RPT>
-__global__ void __launch_bounds__(256) tri_sweep_kernel(TriFactors<S> f, TriVec<S> y, TriVec<S> out, LadderArgs la,
-                                                        int item0) {
+__device__ __forceinline__ void tri_sweep_body(const TriFactors<S>& f, const TriVec<S>& y, const TriVec<S>& out,
+                                               const LadderArgs& la, int item0) {
   const int b = blockIdx.x;
   if (f.skip && f.skip[b] != 0) return;
   const int tid = threadIdx.x;
@@ -180,6 +182,21 @@ __global__ void __launch_bounds__(256) tri_sweep_kernel(TriFactors<S> f, TriVec<
   }
 }
 
+template <typename S, int T, bool LOWER, bool TO_LADDER, int RPT>
+__global__ void __launch_bounds__(256) tri_sweep_kernel(TriFactors<S> f, TriVec<S> y, TriVec<S> out, LadderArgs la,
+                                                        int item0) {
+  tri_sweep_body<S, T, LOWER, TO_LADDER, RPT>(f, y, out, la, item0);
+}
+
+// Complex64 sweeps of three rows of nine right-hand sides per thread (n <= 768, nb = 3) fit
+// 128 VGPRs without spilling: four wavefronts per SIMD instead of three (the forward sweep took
+// 152). Other shapes keep tri_sweep_kernel: under the same cap some of them spill.
+template <int T, bool LOWER, int RPT>
+__global__ void __launch_bounds__(256, 4) tri_sweep_c64_kernel(TriFactors<float2> f, TriVec<float2> y,
+                                                               TriVec<float2> out, LadderArgs la, int item0) {
+  tri_sweep_body<float2, T, LOWER, false, RPT>(f, y, out, la, item0);
+}
+
 // ---------------------------------------------------------------------------
 template <typename S>
 int tri_diag_inverse(const TriFactors<S>& f, int batch, bool need_lower, hipStream_t stream) {
@@ -195,9 +212,13 @@ static int tri_sweep_t(const TriFactors<S>& f, int batch, TriVec<S> y, TriVec<S>
   LadderArgs args{};
   if (la) args = *la;
   const int rpt = static_cast<int>(ceil_div(f.n, 256));
-#define DGAMI_TRI_LAUNCH(R)                                                                                   \
-  hipLaunchKernelGGL((tri_sweep_kernel<S, T, LOWER, TO_LADDER, R>), dim3(batch), dim3(256), 0, stream, f, y, out, \
-                     args, item0)
+#define DGAMI_TRI_LAUNCH(R)                                                                                      \
+  if constexpr (std::is_same<S, float2>::value && !TO_LADDER && R == 3 && T == 9)                                      \
+    hipLaunchKernelGGL((tri_sweep_c64_kernel<T, LOWER, R>), dim3(batch), dim3(256), 0, stream, f, y, out, args,  \
+                       item0);                                                                                   \
+  else                                                                                                           \
+    hipLaunchKernelGGL((tri_sweep_kernel<S, T, LOWER, TO_LADDER, R>), dim3(batch), dim3(256), 0, stream, f, y, out, \
+                       args, item0)
   switch (rpt) {
     case 1: DGAMI_TRI_LAUNCH(1); break;
     case 2: DGAMI_TRI_LAUNCH(2); break;
